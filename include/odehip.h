@@ -1,0 +1,237 @@
+/* odehip.h -- C ABI of the MI355X (gfx950) embedded Runge-Kutta library.
+ *
+ * The HIP execution policy for the adaptive embedded RK path of the
+ * reference (noether17/ode_prototyping).  Every entry point is extern "C",
+ * takes plain pointers and sizes, returns an int status (ODEHIP_OK == 0) and
+ * never throws; odehip_last_error() gives the message of the last failure on
+ * a context.  Pointers documented as "device" are HIP device allocations
+ * (from odehip_alloc or any other HIP allocator on the context's device);
+ * work is enqueued on the context's stream in call order.
+ *
+ * Each declaration names the reference code it replaces (file:line in the
+ * reference tree).
+ */
+#ifndef ODEHIP_H
+#define ODEHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----------------------------------------------------------- status codes */
+#define ODEHIP_OK 0
+#define ODEHIP_ERR_INVALID (-1)     /* bad argument (null pointer, bad size) */
+#define ODEHIP_ERR_HIP (-2)         /* a HIP runtime call failed */
+#define ODEHIP_ERR_NOMEM (-3)       /* device or host allocation failed */
+#define ODEHIP_ERR_UNSUPPORTED (-4) /* combination not implemented */
+
+/* Tableaux (src/integrator/RKEmbeddedParallel/butcher_tableaus/BT*.hpp; DOP853
+ * in the same format from NumericalRecipes/Dopr853_constants.hpp). */
+#define ODEHIP_TABLEAU_HE21 0
+#define ODEHIP_TABLEAU_RKF45 1
+#define ODEHIP_TABLEAU_DOPRI5 2
+#define ODEHIP_TABLEAU_DVERK 3
+#define ODEHIP_TABLEAU_RKF78 4
+#define ODEHIP_TABLEAU_DOP853 5
+#define ODEHIP_NUM_TABLEAUS 6
+
+/* Right-hand sides. */
+#define ODEHIP_RHS_NBODY 0       /* src/ode/NBodyODE.hpp:31-39 */
+#define ODEHIP_RHS_EXPONENTIAL 1 /* src/ode/ExponentialODE.hpp:14-18 */
+#define ODEHIP_RHS_VANDERPOL 2   /* src/ode/VanDerPolODE.hpp:16-25 */
+
+/* Arithmetic of the N-body pair term. */
+#define ODEHIP_NBODY_EXACT 0 /* the reference's FP64 expression, correctly
+                                rounded sqrt and divisions, partners summed
+                                in increasing index: bit-identical to
+                                NBodyODE with SingleThreadedExecutor */
+#define ODEHIP_NBODY_FAST 1  /* FP64 with a refined hardware rsqrt and
+                                parallel partial sums (~1 ulp per term) */
+#define ODEHIP_NBODY_MIXED 2 /* displacement rounded to FP32, pair force in
+                                FP32, accumulation in FP64 */
+
+/* Solve exit status (the reference only ever exits with REACHED_END). */
+#define ODEHIP_STATUS_RUNNING (-1)
+#define ODEHIP_STATUS_REACHED_END 0  /* t >= tf */
+#define ODEHIP_STATUS_MAX_ACCEPTED 1 /* accepted-step budget reached */
+#define ODEHIP_STATUS_NONFINITE 2    /* error norm not finite */
+#define ODEHIP_STATUS_MAX_ATTEMPTS 3 /* attempt budget exhausted */
+
+/* The canonical order of every reduction the library performs (the error
+ * norm): elements are taken in groups of 256 (zero padded); each group is the
+ * sum of its four 64-element quads combined as (q0 + q1) + (q2 + q3), each quad
+ * reduced by halving strides 32, 16, ..., 1 (v[i] += v[i + s]); the group
+ * sums are then added in group order starting from 0.0.  (The reference's
+ * order is executor-defined: src/par_exec/SingleThreadedExecutor.hpp:17-29,
+ * ThreadPoolExecutor.hpp:81-114, CudaExecutor.cuh:21-78.) */
+#define ODEHIP_REDUCE_GROUP 256
+
+typedef struct odehip_context odehip_context;
+
+/* --------------------------------------------------------------- context
+ * Replaces the CUDA executor object (src/par_exec/CudaExecutor.cuh:80-126),
+ * which used the legacy default stream; a context owns a HIP stream and the
+ * scratch the reductions and the device-resident controller need, so no call
+ * allocates per step (CudaExecutor.cuh:103-107,121-122 does). */
+int odehip_create(int device, odehip_context** out);
+int odehip_destroy(odehip_context* ctx);
+const char* odehip_last_error(const odehip_context* ctx);
+/* Use an external hipStream_t (e.g. torch's current stream); NULL restores
+ * the context's own stream. */
+int odehip_set_stream(odehip_context* ctx, void* hip_stream);
+int odehip_synchronize(odehip_context* ctx);
+/* Library version string, e.g. "odehip 1.0 gfx950". */
+const char* odehip_version(void);
+/* HIP-event timing on the context's stream (no reference counterpart; the
+ * reference times with Google Benchmark, src/benchmark/n_body_benchmark.cu):
+ * odehip_timer_start records a start event, odehip_timer_stop records a stop
+ * event, waits for it and writes the elapsed milliseconds. */
+int odehip_timer_start(odehip_context* ctx);
+int odehip_timer_stop(odehip_context* ctx, double* elapsed_ms);
+
+/* ----------------------------------------------------------------- memory
+ * Backing store of HipState (replaces src/state/CudaState.cuh:15-50). */
+int odehip_alloc(odehip_context* ctx, size_t n_doubles, double** dev_out);
+int odehip_free(odehip_context* ctx, double* dev);
+int odehip_copy_h2d(odehip_context* ctx, double* dev_dst, const double* host_src,
+                    size_t n);
+int odehip_copy_d2h(odehip_context* ctx, double* host_dst, const double* dev_src,
+                    size_t n);
+int odehip_copy_d2d(odehip_context* ctx, double* dev_dst, const double* dev_src,
+                    size_t n);
+/* src/state/StateUtils.hpp:5-16 (fill). */
+int odehip_fill(odehip_context* ctx, double* dev, double value, size_t n);
+
+/* ------------------------------------------------ executor-level kernels
+ * One call per reference kernel launch; all pointers are device pointers. */
+
+/* NBodyODE::operator() (src/ode/NBodyODE.hpp:31-39, kernels :51-56 and
+ * :68-133): dxdt = [vel | acc(pos)], x = [pos xyz | vel xyz], n_var = 6 n.
+ * masses may be NULL (unit masses, as the reference; otherwise the pair term
+ * is multiplied by m_j after the division, the reference's commented-out mass
+ * code NBodyODE.hpp:104-124).  G = 1. */
+int odehip_nbody_rhs(odehip_context* ctx, const double* x, double* dxdt,
+                     size_t n_particles, double softening, const double* masses,
+                     int variant);
+/* NBodyODE::acc (src/ode/NBodyODE.hpp:41-46): acc (3 n) from pos (3 n). */
+int odehip_nbody_acc(odehip_context* ctx, const double* pos, double* acc,
+                     size_t n_particles, double softening, const double* masses,
+                     int variant);
+/* ExponentialODE / VanDerPolODE (src/ode/ExponentialODE.hpp:14-18,
+ * src/ode/VanDerPolODE.hpp:16-25). */
+int odehip_ode_rhs(odehip_context* ctx, int rhs_kind, const double* x,
+                   double* dxdt, size_t n_var);
+
+/* detail::rk_stage_kernel (RKEmbeddedParallel.hpp:91-101): for stage >= 1,
+ * temp = x0 + dt * sum_{j<stage} a[stage-1][j] * ks[j*n_var + .]. */
+int odehip_rk_stage(odehip_context* ctx, int tableau, int stage, double dt,
+                    double* temp, const double* ks, const double* x0,
+                    size_t n_var);
+/* detail::update_state_and_error_kernel (RKEmbeddedParallel.hpp:103-123). */
+int odehip_rk_update_state_and_error(odehip_context* ctx, int tableau,
+                                     double dt, double* x, double* err,
+                                     const double* ks, const double* x0,
+                                     size_t n_var);
+/* detail::update_error_target_kernel (RKEmbeddedParallel.hpp:125-130). */
+int odehip_rk_update_error_target(odehip_context* ctx, double* sc,
+                                  const double* atol, const double* rtol,
+                                  const double* x, const double* x0,
+                                  size_t n_var);
+/* detail::rk_norm (RKEmbeddedParallel.hpp:141-149) = sqrt(sum (v/sc)^2 / n)
+ * in the canonical order above; blocks until the host value is available
+ * (as CudaExecutor::transform_reduce does, CudaExecutor.cuh:117-119). */
+int odehip_rk_norm(odehip_context* ctx, const double* v, const double* sc,
+                   size_t n_var, double* host_result);
+/* detail::estimate_initial_step (RKEmbeddedParallel.hpp:170-200) on the
+ * device; writes the step to *host_dt. */
+int odehip_rk_initial_step(odehip_context* ctx, int tableau, int rhs_kind,
+                           const double* x0, const double* atol,
+                           const double* rtol, size_t n_var, double softening,
+                           const double* masses, int variant, double* host_dt);
+
+/* --------------------------------------------------- device-resident solve
+ * RKEmbeddedParallel<BT>::integrate (RKEmbeddedParallel.hpp:13-81) with the
+ * attempt loop, error norm, accept/reject and step-size controller kept on
+ * the device: the host only enqueues captured graphs of attempts and reads a
+ * completion flag; no per-step host round trip. */
+typedef struct {
+  int rhs_kind;          /* ODEHIP_RHS_* */
+  int nbody_variant;     /* ODEHIP_NBODY_* (N-body only) */
+  size_t n_var;          /* state length (6 n for N-body) */
+  double softening;      /* N-body softening length */
+  const double* masses;  /* device, n entries, or NULL (unit masses) */
+} odehip_ode;
+
+typedef struct {
+  double t0, tf;
+  const double* atol;    /* device, n_var entries (per component, as the
+                            reference's StateType atol) */
+  const double* rtol;    /* device, n_var entries */
+  long max_accepted;     /* stop after this many accepted steps; 0 = none */
+  long max_attempts;     /* stop after this many attempts; 0 = none */
+  int fsal;              /* 1: reuse the last stage as the next first stage
+                            when the tableau is FSAL (DOPRI5), and keep k1
+                            across a rejection -- same results, fewer RHS
+                            evaluations.  0: the reference's S evaluations
+                            per attempt. */
+  int attempts_per_launch; /* attempts per captured graph; 0 = automatic */
+  double* times;         /* host, times_cap entries: t0 then t after each
+                            accepted step (RawOutput::times()); may be NULL */
+  long times_cap;
+  double* trace;         /* host, trace_cap x 3: per attempt (err, dt,
+                            accepted); may be NULL */
+  long trace_cap;
+} odehip_solve_options;
+
+typedef struct {
+  long accepted, rejected, attempts, rhs_evals;
+  long n_times;          /* entries written to times (accepted + 1, capped) */
+  long n_trace;          /* entries written to trace */
+  int status;            /* ODEHIP_STATUS_* */
+  double t, dt, dt_initial, last_err;
+} odehip_solve_stats;
+
+/* x: device, n_var; in: x0, out: the state at the final time. */
+int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
+                    const odehip_solve_options* opt, double* x,
+                    odehip_solve_stats* stats);
+
+/* ----------------------------------------------------- conserved quantities
+ * scripts/nbody_physics/energy.py:118-192,224-241 (softened PE + KE), with
+ * masses; and total momentum (absent from the reference).  host_out: ke, pe /
+ * px, py, pz. */
+int odehip_nbody_energy(odehip_context* ctx, const double* x, size_t n,
+                        double softening, const double* masses,
+                        double* host_out);
+int odehip_nbody_momentum(odehip_context* ctx, const double* x, size_t n,
+                          const double* masses, double* host_out);
+
+/* ------------------------------------------------------ initial conditions
+ * Host-side generators of the synthetic systems (seeded mt19937_64 with an
+ * explicit 53-bit mapping and Box-Muller).  x_out: 6 n doubles
+ * [pos | vel], masses_out: n doubles (may be NULL). */
+/* Positions uniform in [-half_width, half_width]^3, velocities N(0, sigma^2)
+ * per component, masses total_mass / n, momentum removed. */
+int odehip_ic_uniform_cube(size_t n, uint64_t seed, double half_width,
+                           double vel_sigma, double total_mass, double* x_out,
+                           double* masses_out);
+/* Plummer sphere (Aarseth, Henon & Wielen 1974), scale radius a, total mass
+ * M, G = 1, isotropic, virial equilibrium; centre of mass and momentum
+ * removed. */
+int odehip_ic_plummer(size_t n, uint64_t seed, double scale_radius,
+                      double total_mass, double* x_out, double* masses_out);
+/* Two Plummer spheres of n/2 bodies each, centres at (+-separation/2, 0, 0)
+ * moving with -+ rel_speed/2 along x; total mass M. */
+int odehip_ic_colliding_plummer(size_t n, uint64_t seed, double scale_radius,
+                                double total_mass, double separation,
+                                double rel_speed, double* x_out,
+                                double* masses_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ODEHIP_H */

@@ -1,0 +1,23 @@
+"""MI355X (gfx950) embedded Runge-Kutta integration of N-body gravity.
+
+Host-side Python mirror of the reference's interfaces
+(noether17/ode_prototyping): an execution policy (:class:`HipExecutor`, the
+counterpart of ``src/par_exec/CudaExecutor.cuh``), device state containers
+(:class:`HipState`, ``src/state/CudaState.cuh``), ODE functors
+(:class:`NBodyODE`, :class:`ExponentialODE`, :class:`VanDerPolODE`,
+``src/ode``), output policies (:class:`RawOutput`, ``src/output/RawOutput.hpp``)
+and the integrator (:func:`integrate`, ``RKEmbeddedParallel<BT>::integrate``,
+``src/integrator/RKEmbeddedParallel/RKEmbeddedParallel.hpp:13-81``).  All
+numerical work runs in libodehip.so through its C ABI (include/odehip.h).
+"""
+from ._lib import (NBODY_VARIANTS, REDUCE_GROUP, STAGES, STATUS, TABLEAUS, lib)
+from .core import (ExponentialODE, HipExecutor, HipState, MostRecentOutput, NBodyODE,
+                   RawOutput, SolveResult, VanDerPolODE, integrate, integrate_host_driven)
+from . import scenarios
+
+__all__ = [
+    "HipExecutor", "HipState", "NBodyODE", "ExponentialODE", "VanDerPolODE",
+    "RawOutput", "MostRecentOutput", "SolveResult", "integrate",
+    "integrate_host_driven", "scenarios", "TABLEAUS", "STAGES", "STATUS",
+    "NBODY_VARIANTS", "REDUCE_GROUP", "lib",
+]

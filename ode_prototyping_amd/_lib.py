@@ -1,0 +1,115 @@
+"""ctypes declarations of libodehip.so (include/odehip.h).
+
+The library is built in-tree (``__graft_entry__.build()`` or ``make -C
+ode_prototyping_amd``); importing this module never builds anything and there
+is no CPU fallback: if the shared library is missing, :func:`lib` raises.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libodehip.so")
+
+OK = 0
+ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED = -1, -2, -3, -4
+
+TABLEAUS = {"HE21": 0, "RKF45": 1, "DOPRI5": 2, "DVERK": 3, "RKF78": 4, "DOP853": 5}
+STAGES = {"HE21": 2, "RKF45": 6, "DOPRI5": 7, "DVERK": 8, "RKF78": 13, "DOP853": 12}
+RHS_NBODY, RHS_EXPONENTIAL, RHS_VANDERPOL = 0, 1, 2
+NBODY_VARIANTS = {"exact": 0, "fast": 1, "mixed": 2}
+STATUS = {-1: "running", 0: "reached_end", 1: "max_accepted", 2: "nonfinite", 3: "max_attempts"}
+REDUCE_GROUP = 256
+
+_dp = C.POINTER(C.c_double)
+_ctxp = C.c_void_p
+
+
+class OdeStruct(C.Structure):
+    _fields_ = [("rhs_kind", C.c_int), ("nbody_variant", C.c_int), ("n_var", C.c_size_t),
+                ("softening", C.c_double), ("masses", C.c_void_p)]
+
+
+class SolveOptions(C.Structure):
+    _fields_ = [("t0", C.c_double), ("tf", C.c_double), ("atol", C.c_void_p),
+                ("rtol", C.c_void_p), ("max_accepted", C.c_long), ("max_attempts", C.c_long),
+                ("fsal", C.c_int), ("attempts_per_launch", C.c_int),
+                ("times", C.c_void_p), ("times_cap", C.c_long),
+                ("trace", C.c_void_p), ("trace_cap", C.c_long)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("accepted", C.c_long), ("rejected", C.c_long), ("attempts", C.c_long),
+                ("rhs_evals", C.c_long), ("n_times", C.c_long), ("n_trace", C.c_long),
+                ("status", C.c_int), ("t", C.c_double), ("dt", C.c_double),
+                ("dt_initial", C.c_double), ("last_err", C.c_double)]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "odehip_version": (C.c_char_p, []),
+    "odehip_create": (C.c_int, [C.c_int, C.POINTER(_ctxp)]),
+    "odehip_destroy": (C.c_int, [_ctxp]),
+    "odehip_last_error": (C.c_char_p, [_ctxp]),
+    "odehip_set_stream": (C.c_int, [_ctxp, C.c_void_p]),
+    "odehip_synchronize": (C.c_int, [_ctxp]),
+    "odehip_timer_start": (C.c_int, [_ctxp]),
+    "odehip_timer_stop": (C.c_int, [_ctxp, _dp]),
+    "odehip_alloc": (C.c_int, [_ctxp, C.c_size_t, C.POINTER(C.c_void_p)]),
+    "odehip_free": (C.c_int, [_ctxp, C.c_void_p]),
+    "odehip_copy_h2d": (C.c_int, [_ctxp, C.c_void_p, _dp, C.c_size_t]),
+    "odehip_copy_d2h": (C.c_int, [_ctxp, _dp, C.c_void_p, C.c_size_t]),
+    "odehip_copy_d2d": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "odehip_fill": (C.c_int, [_ctxp, C.c_void_p, C.c_double, C.c_size_t]),
+    "odehip_nbody_rhs": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_size_t, C.c_double,
+                                   C.c_void_p, C.c_int]),
+    "odehip_nbody_acc": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_size_t, C.c_double,
+                                   C.c_void_p, C.c_int]),
+    "odehip_ode_rhs": (C.c_int, [_ctxp, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "odehip_rk_stage": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_size_t]),
+    "odehip_rk_update_state_and_error": (C.c_int, [_ctxp, C.c_int, C.c_double, C.c_void_p,
+                                                   C.c_void_p, C.c_void_p, C.c_void_p,
+                                                   C.c_size_t]),
+    "odehip_rk_update_error_target": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                C.c_void_p, C.c_void_p, C.c_size_t]),
+    "odehip_rk_norm": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_size_t, _dp]),
+    "odehip_rk_initial_step": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_size_t, C.c_double, C.c_void_p,
+                                         C.c_int, _dp]),
+    "odehip_rk_solve": (C.c_int, [_ctxp, C.c_int, C.POINTER(OdeStruct),
+                                  C.POINTER(SolveOptions), C.c_void_p, C.POINTER(SolveStats)]),
+    "odehip_nbody_energy": (C.c_int, [_ctxp, C.c_void_p, C.c_size_t, C.c_double, C.c_void_p,
+                                      _dp]),
+    "odehip_nbody_momentum": (C.c_int, [_ctxp, C.c_void_p, C.c_size_t, C.c_void_p, _dp]),
+    "odehip_ic_uniform_cube": (C.c_int, [C.c_size_t, C.c_uint64, C.c_double, C.c_double,
+                                         C.c_double, _dp, _dp]),
+    "odehip_ic_plummer": (C.c_int, [C.c_size_t, C.c_uint64, C.c_double, C.c_double, _dp, _dp]),
+    "odehip_ic_colliding_plummer": (C.c_int, [C.c_size_t, C.c_uint64, C.c_double, C.c_double,
+                                              C.c_double, C.c_double, _dp, _dp]),
+}
+
+_LIB = None
+
+
+def lib():
+    """Load libodehip.so (raises RuntimeError if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `make -C ode_prototyping_amd` "
+                "(or __graft_entry__.build()); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def dptr(a):
+    """ctypes double* of a contiguous float64 numpy array (or None)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)

@@ -1,0 +1,403 @@
+// N-body gravity right-hand side on gfx950 (CDNA4): dxdt = [vel | acc(pos)]
+// for the state layout of src/ode/NBodyODE.hpp:12-19 (positions then
+// velocities, xyz interleaved per particle), G = 1, Plummer softening.
+//
+// The reference evaluates the N(N-1)/2 pairs in parallel and scatters six
+// FP64 atomic adds per pair (NBodyODE.hpp:68-133).  Here every kernel is an
+// atomic-free gather: a workgroup owns a tile of 64 target particles (one per
+// lane) and streams the partner particles j; partner coordinates are
+// wave-uniform, so they arrive through scalar loads into SGPRs and cost no
+// VGPR bandwidth or LDS.
+//
+//  * exact:  the reference's FP64 expression (((dx^2 + dy^2) + dz^2) + eps^2,
+//            correctly rounded sqrt, den = dist * dist_sq, three correctly
+//            rounded divisions) and, per target, the partners summed in
+//            increasing j -- the order the single-threaded pair loop adds into
+//            acc[i].  Terms are computed by all eight waves of a workgroup in
+//            parallel and written to LDS; three waves then fold them in order
+//            (one coordinate each), double-buffered so folding tile t-1
+//            overlaps computing tile t.  Result: bit-identical to NBodyODE
+//            with SingleThreadedExecutor.  Must be compiled with
+//            -ffp-contract=off.
+//  * fast:   FP64, r^-3 from the hardware reciprocal square root refined by one
+//            third-order Newton step (~1 ulp), FMA accumulation, the j range
+//            split over 16 waves per workgroup and the partial sums combined
+//            in a fixed order (deterministic, not bit-identical).
+//  * mixed:  displacement computed in FP64 and rounded to FP32, r^-3 and the
+//            pair force in FP32 (hardware rsqrt), per-wave partial sums in
+//            FP32 flushed into FP64 accumulators every 32 partners.
+#include "internal.hpp"
+
+namespace odehip {
+namespace {
+
+// Partner data is read at wave-uniform addresses and never written during a
+// launch; reading it through the constant address space lets the compiler
+// use scalar loads (SGPR results broadcast to all lanes for free).
+typedef const __attribute__((address_space(4))) double* uniform_ptr;
+__device__ inline uniform_ptr as_uniform(const double* p) {
+  return (uniform_ptr)(p);
+}
+
+__device__ inline const double* select_input(const NBodyArgs& a) {
+  if (a.y) return a.y;
+  return a.st->cur ? a.xb1 : a.xb0;
+}
+
+// Returns true if the launch has nothing to compute (solve finished, or k1
+// of this attempt is reused).  On FSAL reuse the launch copies k1 instead.
+__device__ inline bool prologue_skip(const NBodyArgs& a) {
+  if (!a.st) return false;
+  if (a.st->done) return true;
+  if (a.stage0 && a.st->k1_valid) {
+    if (a.st->k1_valid == 1) {
+      const size_t n_var = static_cast<size_t>(a.n) * 6;
+      for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+           e < n_var; e += static_cast<size_t>(gridDim.x) * blockDim.x)
+        a.k[e] = a.kfsal[e];
+    }
+    return true;
+  }
+  return false;
+}
+
+// dxdt[0:3N] = x[3N:6N] for this tile's particles (nbody_init_dxdt_kernel,
+// NBodyODE.hpp:51-56).
+__device__ inline void copy_velocities(const double* __restrict__ y,
+                                       double* __restrict__ k, int n, int i0) {
+  const int e = threadIdx.x;
+  const long base = 3L * i0;
+  const long lim = 3L * n;
+  if (e < 192 && base + e < lim) k[base + e] = y[lim + base + e];
+}
+
+// ------------------------------------------------------------------ exact --
+constexpr int kExactWaves = 8;
+constexpr int kExactJW = 4;  // partners per wave per tile
+constexpr int kExactTJ = kExactWaves * kExactJW;
+
+template <bool kMass>
+__global__ __launch_bounds__(64 * kExactWaves) void k_nbody_exact(NBodyArgs a) {
+  if (prologue_skip(a)) return;
+  __shared__ double terms[2][kExactTJ][3][64];
+  const double* __restrict__ y = select_input(a);
+  const uniform_ptr yu = as_uniform(y);
+  double* __restrict__ k = a.k;
+  const uniform_ptr massu = as_uniform(a.mass);
+  const int n = a.n;
+  const double softening_sq = a.eps2;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i0 = blockIdx.x * 64;
+  const int i = i0 + lane;
+  const bool valid_i = i < n;
+  const double xi = valid_i ? y[3 * i] : 0.0;
+  const double yi = valid_i ? y[3 * i + 1] : 0.0;
+  const double zi = valid_i ? y[3 * i + 2] : 0.0;
+  double acc = 0.0;
+  const int n_tiles = (n + kExactTJ - 1) / kExactTJ;
+  for (int t = 0; t <= n_tiles; ++t) {
+    if (t < n_tiles) {
+      const int jb = t * kExactTJ + wave * kExactJW;
+#pragma unroll
+      for (int jj = 0; jj < kExactJW; ++jj) {
+        const int j = jb + jj;
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        if (j < n) {
+          const double dx = yu[3 * j] - xi;
+          const double dy = yu[3 * j + 1] - yi;
+          const double dz = yu[3 * j + 2] - zi;
+          const double dist_sq = dx * dx + dy * dy + dz * dz + softening_sq;
+          const double dist = __builtin_sqrt(dist_sq);
+          const double denominator = dist * dist_sq;
+          ax = dx / denominator;
+          ay = dy / denominator;
+          az = dz / denominator;
+          if (kMass) {
+            const double mj = massu[j];
+            ax *= mj;
+            ay *= mj;
+            az *= mj;
+          }
+        }
+        terms[t & 1][wave * kExactJW + jj][0][lane] = ax;
+        terms[t & 1][wave * kExactJW + jj][1][lane] = ay;
+        terms[t & 1][wave * kExactJW + jj][2][lane] = az;
+      }
+    }
+    if (t > 0 && wave < 3) {
+      const int base = (t - 1) * kExactTJ;
+      const int cnt = min(kExactTJ, n - base);
+      const double* src = &terms[(t - 1) & 1][0][wave][lane];
+      if (base + cnt <= i0 || base >= i0 + 64) {
+        for (int m = 0; m < cnt; ++m) acc += src[m * 192];
+      } else {
+        for (int m = 0; m < cnt; ++m) {
+          const double v = src[m * 192];
+          if (base + m != i) acc += v;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (wave < 3 && valid_i) k[3L * n + 3 * i + wave] = acc;
+  copy_velocities(y, k, n, i0);
+}
+
+// ------------------------------------------------------------------- fast --
+constexpr int kFastWaves = 16;
+
+template <bool kMass, bool kGuard>
+__device__ inline void fast_term(double xj, double yj, double zj, double mj,
+                                 int j, int i, double xi, double yi, double zi,
+                                 double eps2, double& ax, double& ay,
+                                 double& az) {
+  const double dx = xj - xi;
+  const double dy = yj - yi;
+  const double dz = zj - zi;
+  const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, eps2)));
+  const double y0 = __builtin_amdgcn_rsq(r2);
+  const double h = r2 * y0;
+  const double e = __builtin_fma(-h, y0, 1.0);
+  const double y1 = __builtin_fma(y0 * e, __builtin_fma(e, 0.375, 0.5), y0);
+  double w = (y1 * y1) * y1;
+  if (kMass) w *= mj;
+  if (kGuard) w = (j == i) ? 0.0 : w;
+  ax = __builtin_fma(dx, w, ax);
+  ay = __builtin_fma(dy, w, ay);
+  az = __builtin_fma(dz, w, az);
+}
+
+template <bool kMass, bool kGuard>
+__global__ __launch_bounds__(64 * kFastWaves) void k_nbody_fast(NBodyArgs a) {
+  if (prologue_skip(a)) return;
+  __shared__ double red[kFastWaves][3][64];
+  const double* __restrict__ y = select_input(a);
+  const uniform_ptr yu = as_uniform(y);
+  double* __restrict__ k = a.k;
+  const uniform_ptr mass = as_uniform(a.mass);
+  const int n = a.n;
+  const double eps2 = a.eps2;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i0 = blockIdx.x * 64;
+  const int i = i0 + lane;
+  const bool valid_i = i < n;
+  const int ic = valid_i ? i : n - 1;
+  const double xi = y[3 * ic], yi = y[3 * ic + 1], zi = y[3 * ic + 2];
+  const int seg = (n + kFastWaves - 1) / kFastWaves;
+  const int jbeg = min(n, wave * seg);
+  const int jend = min(n, jbeg + seg);
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  int j = jbeg;
+  for (; j + 4 <= jend; j += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int jj = j + u;
+      const double mj = kMass ? mass[jj] : 1.0;
+      fast_term<kMass, kGuard>(yu[3 * jj], yu[3 * jj + 1], yu[3 * jj + 2], mj, jj,
+                               i, xi, yi, zi, eps2, ax, ay, az);
+    }
+  }
+  for (; j < jend; ++j) {
+    const double mj = kMass ? mass[j] : 1.0;
+    fast_term<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j, i, xi,
+                             yi, zi, eps2, ax, ay, az);
+  }
+  red[wave][0][lane] = ax;
+  red[wave][1][lane] = ay;
+  red[wave][2][lane] = az;
+  __syncthreads();
+  if (wave < 3 && valid_i) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kFastWaves; ++w) s += red[w][wave][lane];
+    k[3L * n + 3 * i + wave] = s;
+  }
+  copy_velocities(y, k, n, i0);
+}
+
+// ------------------------------------------------------------------ mixed --
+constexpr int kMixedWaves = 16;
+constexpr int kMixedFlush = 32;
+
+template <bool kMass, bool kGuard>
+__device__ inline void mixed_term(double xj, double yj, double zj, float mj,
+                                  int j, int i, double xi, double yi,
+                                  double zi, float eps2f, float& ax, float& ay,
+                                  float& az) {
+  const float dx = static_cast<float>(xj - xi);
+  const float dy = static_cast<float>(yj - yi);
+  const float dz = static_cast<float>(zj - zi);
+  const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2f)));
+  const float rinv = __builtin_amdgcn_rsqf(r2);
+  float w = (rinv * rinv) * rinv;
+  if (kMass) w *= mj;
+  if (kGuard) w = (j == i) ? 0.0f : w;
+  ax = __builtin_fmaf(dx, w, ax);
+  ay = __builtin_fmaf(dy, w, ay);
+  az = __builtin_fmaf(dz, w, az);
+}
+
+template <bool kMass, bool kGuard>
+__global__ __launch_bounds__(64 * kMixedWaves) void k_nbody_mixed(NBodyArgs a) {
+  if (prologue_skip(a)) return;
+  __shared__ double red[kMixedWaves][3][64];
+  const double* __restrict__ y = select_input(a);
+  const uniform_ptr yu = as_uniform(y);
+  double* __restrict__ k = a.k;
+  const uniform_ptr mass = as_uniform(a.mass);
+  const int n = a.n;
+  const float eps2f = static_cast<float>(a.eps2);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i0 = blockIdx.x * 64;
+  const int i = i0 + lane;
+  const bool valid_i = i < n;
+  const int ic = valid_i ? i : n - 1;
+  const double xi = y[3 * ic], yi = y[3 * ic + 1], zi = y[3 * ic + 2];
+  const int seg = (n + kMixedWaves - 1) / kMixedWaves;
+  const int jbeg = min(n, wave * seg);
+  const int jend = min(n, jbeg + seg);
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  int j = jbeg;
+  while (j < jend) {
+    const int jstop = min(jend, j + kMixedFlush);
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    for (; j + 4 <= jstop; j += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int jj = j + u;
+        const float mj = kMass ? static_cast<float>(mass[jj]) : 1.0f;
+        mixed_term<kMass, kGuard>(yu[3 * jj], yu[3 * jj + 1], yu[3 * jj + 2], mj,
+                                  jj, i, xi, yi, zi, eps2f, fx, fy, fz);
+      }
+    }
+    for (; j < jstop; ++j) {
+      const float mj = kMass ? static_cast<float>(mass[j]) : 1.0f;
+      mixed_term<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j, i,
+                                xi, yi, zi, eps2f, fx, fy, fz);
+    }
+    ax += static_cast<double>(fx);
+    ay += static_cast<double>(fy);
+    az += static_cast<double>(fz);
+  }
+  red[wave][0][lane] = ax;
+  red[wave][1][lane] = ay;
+  red[wave][2][lane] = az;
+  __syncthreads();
+  if (wave < 3 && valid_i) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kMixedWaves; ++w) s += red[w][wave][lane];
+    k[3L * n + 3 * i + wave] = s;
+  }
+  copy_velocities(y, k, n, i0);
+}
+
+// ----------------------------------------------------------------- energy --
+// Per particle i: 0.5 m_i |v_i|^2 - 0.5 m_i sum_{j != i} m_j / sqrt(r^2+eps^2)
+// (each pair counted from both ends), FP64 with correctly rounded sqrt and
+// division; reduced afterwards in a fixed tree order.
+template <bool kMass>
+__global__ __launch_bounds__(256) void k_energy(const double* __restrict__ y,
+                                                int n, double eps2,
+                                                const double* __restrict__ mass,
+                                                double* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double xi = y[3 * i], yi = y[3 * i + 1], zi = y[3 * i + 2];
+  double pe = 0.0;
+  for (int j = 0; j < n; ++j) {
+    if (j == i) continue;
+    const double dx = y[3 * j] - xi, dy = y[3 * j + 1] - yi, dz = y[3 * j + 2] - zi;
+    const double inv = 1.0 / __builtin_sqrt(dx * dx + dy * dy + dz * dz + eps2);
+    pe += kMass ? mass[j] * inv : inv;
+  }
+  const double mi = kMass ? mass[i] : 1.0;
+  const double* v = y + 3L * n;
+  const double v2 = v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2];
+  out[i] = 0.5 * mi * v2;            // kinetic
+  out[n + i] = -0.5 * mi * pe;       // potential (half of each pair)
+}
+
+// m_i v_i per component: out[c * n + i].
+__global__ void k_momentum_terms(const double* __restrict__ y, int n,
+                                 const double* __restrict__ mass,
+                                 double* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double mi = mass ? mass[i] : 1.0;
+  const double* v = y + 3L * n;
+  for (int c = 0; c < 3; ++c) out[static_cast<long>(c) * n + i] = mi * v[3 * i + c];
+}
+
+}  // namespace
+
+hipError_t launch_momentum_terms(const double* x, int n, const double* mass,
+                                 double* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_momentum_terms, dim3((n + 255) / 256), dim3(256), 0, s,
+                     x, n, mass, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_nbody_rhs(const NBodyArgs& a, int variant, hipStream_t s) {
+  if (a.n <= 0) return hipSuccess;
+  const unsigned blocks = static_cast<unsigned>((a.n + 63) / 64);
+  const bool mass = a.mass != nullptr;
+  const bool guard = !(a.eps2 >= 1e-150);
+  switch (variant) {
+    case ODEHIP_NBODY_EXACT:
+      if (mass)
+        hipLaunchKernelGGL(k_nbody_exact<true>, dim3(blocks), dim3(64 * kExactWaves), 0, s, a);
+      else
+        hipLaunchKernelGGL(k_nbody_exact<false>, dim3(blocks), dim3(64 * kExactWaves), 0, s, a);
+      break;
+    case ODEHIP_NBODY_FAST:
+      if (mass) {
+        if (guard)
+          hipLaunchKernelGGL((k_nbody_fast<true, true>), dim3(blocks), dim3(64 * kFastWaves), 0, s, a);
+        else
+          hipLaunchKernelGGL((k_nbody_fast<true, false>), dim3(blocks), dim3(64 * kFastWaves), 0, s, a);
+      } else {
+        if (guard)
+          hipLaunchKernelGGL((k_nbody_fast<false, true>), dim3(blocks), dim3(64 * kFastWaves), 0, s, a);
+        else
+          hipLaunchKernelGGL((k_nbody_fast<false, false>), dim3(blocks), dim3(64 * kFastWaves), 0, s, a);
+      }
+      break;
+    case ODEHIP_NBODY_MIXED: {
+      const bool guardf = !(static_cast<float>(a.eps2) >= 1e-12f);
+      if (mass) {
+        if (guardf)
+          hipLaunchKernelGGL((k_nbody_mixed<true, true>), dim3(blocks), dim3(64 * kMixedWaves), 0, s, a);
+        else
+          hipLaunchKernelGGL((k_nbody_mixed<true, false>), dim3(blocks), dim3(64 * kMixedWaves), 0, s, a);
+      } else {
+        if (guardf)
+          hipLaunchKernelGGL((k_nbody_mixed<false, true>), dim3(blocks), dim3(64 * kMixedWaves), 0, s, a);
+        else
+          hipLaunchKernelGGL((k_nbody_mixed<false, false>), dim3(blocks), dim3(64 * kMixedWaves), 0, s, a);
+      }
+      break;
+    }
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_nbody_energy(const double* x, int n, double eps2,
+                               const double* mass, double* per_particle,
+                               hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
+  if (mass)
+    hipLaunchKernelGGL(k_energy<true>, dim3(blocks), dim3(256), 0, s, x, n, eps2, mass, per_particle);
+  else
+    hipLaunchKernelGGL(k_energy<false>, dim3(blocks), dim3(256), 0, s, x, n, eps2, mass, per_particle);
+  return hipGetLastError();
+}
+
+}  // namespace odehip

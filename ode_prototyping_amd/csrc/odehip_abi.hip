@@ -1,0 +1,624 @@
+// C ABI of libodehip (declarations and reference citations in odehip.h):
+// context and stream management, device memory, executor-level kernels and
+// the device-resident solve driver.
+//
+// The solve driver replaces the host loop of RKEmbeddedParallel::integrate
+// (RKEmbeddedParallel.hpp:35-80).  The attempt's kernels (S right-hand sides,
+// S-1 stage combinations, the fused final/error/norm pass and the controller)
+// are captured once into a HIP graph holding K attempts; the graph is
+// replayed until the device-side controller raises `done`.  Every kernel of a
+// finished solve returns at once, so over-enqueueing is harmless.  The host
+// only polls the done flag of the previous replay (one replay in flight), so
+// the device never waits on the host between attempts.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "internal.hpp"
+
+using namespace odehip;
+
+struct GraphKey {
+  int tableau = -1, rhs_kind = -1, variant = -1, k = 0, fsal = -1;
+  size_t n_var = 0;
+  double eps2 = 0.0;
+  const void* mass = nullptr;
+  const void* atol = nullptr;
+  const void* rtol = nullptr;
+  const void* ws = nullptr;
+  bool operator==(const GraphKey& o) const {
+    return tableau == o.tableau && rhs_kind == o.rhs_kind &&
+           variant == o.variant && k == o.k && fsal == o.fsal &&
+           n_var == o.n_var && std::memcmp(&eps2, &o.eps2, sizeof eps2) == 0 &&
+           mass == o.mass && atol == o.atol && rtol == o.rtol && ws == o.ws;
+  }
+};
+
+struct odehip_context {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;  // current stream (own or external)
+  std::string err;
+  // solve workspace
+  double* ws = nullptr;
+  size_t ws_cap = 0;
+  SolverState* st = nullptr;
+  // small scratch for reductions
+  double* scratch = nullptr;
+  size_t scratch_cap = 0;
+  // pinned host staging
+  int* h_done = nullptr;           // 2 slots
+  SolverState* h_st = nullptr;
+  double* h_scalar = nullptr;      // 8 doubles
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipEvent_t t_start = nullptr, t_stop = nullptr;  // timing events
+  // graph cache
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  GraphKey key;
+};
+
+namespace {
+
+int fail(odehip_context* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int hip_fail(odehip_context* ctx, hipError_t e, const char* where) {
+  return fail(ctx, ODEHIP_ERR_HIP,
+              std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                                   \
+  do {                                                       \
+    hipError_t _e = (expr);                                  \
+    if (_e != hipSuccess) return hip_fail((ctx), _e, #expr); \
+  } while (0)
+
+int ensure_scratch(odehip_context* ctx, size_t n) {
+  if (ctx->scratch_cap >= n) return ODEHIP_OK;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->scratch) HIP_TRY(ctx, hipFree(ctx->scratch));
+  ctx->scratch = nullptr;
+  ctx->scratch_cap = 0;
+  size_t cap = std::max<size_t>(n, 4096);
+  HIP_TRY(ctx, hipMalloc(&ctx->scratch, cap * sizeof(double)));
+  ctx->scratch_cap = cap;
+  return ODEHIP_OK;
+}
+
+int ensure_ws(odehip_context* ctx, size_t n) {
+  if (ctx->ws_cap >= n) return ODEHIP_OK;
+  HIP_TRY(ctx, hipDeviceSynchronize());
+  if (ctx->exec) {
+    (void)hipGraphExecDestroy(ctx->exec);
+    ctx->exec = nullptr;
+  }
+  if (ctx->graph) {
+    (void)hipGraphDestroy(ctx->graph);
+    ctx->graph = nullptr;
+  }
+  ctx->key = GraphKey{};
+  if (ctx->ws) HIP_TRY(ctx, hipFree(ctx->ws));
+  ctx->ws = nullptr;
+  ctx->ws_cap = 0;
+  HIP_TRY(ctx, hipMalloc(&ctx->ws, n * sizeof(double)));
+  ctx->ws_cap = n;
+  return ODEHIP_OK;
+}
+
+bool valid_tableau(int t) { return t >= 0 && t < ODEHIP_NUM_TABLEAUS; }
+
+int check_ode(odehip_context* ctx, const odehip_ode* ode) {
+  if (!ode) return fail(ctx, ODEHIP_ERR_INVALID, "ode is null");
+  if (ode->n_var == 0) return fail(ctx, ODEHIP_ERR_INVALID, "n_var is 0");
+  if (ode->rhs_kind == ODEHIP_RHS_NBODY) {
+    if (ode->n_var % 6 != 0)
+      return fail(ctx, ODEHIP_ERR_INVALID, "N-body n_var must be a multiple of 6");
+    if (ode->n_var / 6 > (1u << 26))
+      return fail(ctx, ODEHIP_ERR_INVALID, "too many particles");
+    if (ode->nbody_variant < 0 || ode->nbody_variant > 2)
+      return fail(ctx, ODEHIP_ERR_INVALID, "bad nbody_variant");
+  } else if (ode->rhs_kind == ODEHIP_RHS_VANDERPOL) {
+    if (ode->n_var != 2) return fail(ctx, ODEHIP_ERR_INVALID, "Van der Pol has n_var 2");
+  } else if (ode->rhs_kind != ODEHIP_RHS_EXPONENTIAL) {
+    return fail(ctx, ODEHIP_ERR_INVALID, "unknown rhs_kind");
+  }
+  return ODEHIP_OK;
+}
+
+// Lays out the solve workspace and returns its length in doubles.
+size_t layout(SolveBufs* b, double* base, SolverState* st, size_t n, int S,
+              long times_cap, long trace_cap) {
+  const size_t ng = static_cast<size_t>(n_groups_for(n));
+  size_t off = 0;
+  auto take = [&](size_t len) {
+    double* p = base ? base + off : nullptr;
+    off += (len + 31) & ~static_cast<size_t>(31);  // 256-byte alignment
+    return p;
+  };
+  b->st = st;
+  b->xb0 = take(n);
+  b->xb1 = take(n);
+  b->ks = take(static_cast<size_t>(S) * n);
+  b->ytmp0 = take(n);
+  b->ytmp1 = take(n);
+  b->kfsal = take(n);
+  b->sc0 = take(n);
+  b->partA = take(ng);
+  b->partB = take(ng);
+  b->times = times_cap > 0 ? take(static_cast<size_t>(times_cap)) : nullptr;
+  b->trace = trace_cap > 0 ? take(3 * static_cast<size_t>(trace_cap)) : nullptr;
+  b->times_cap = times_cap;
+  b->trace_cap = trace_cap;
+  return off;
+}
+
+int prepare_bufs(odehip_context* ctx, int tableau, const odehip_ode* ode,
+                 const double* atol, const double* rtol, long times_cap,
+                 long trace_cap, SolveBufs* b) {
+  const size_t n = ode->n_var;
+  const int S = kTableaus[tableau].S;
+  size_t need = layout(b, nullptr, ctx->st, n, S, times_cap, trace_cap);
+  int rc = ensure_ws(ctx, need);
+  if (rc) return rc;
+  layout(b, ctx->ws, ctx->st, n, S, times_cap, trace_cap);
+  b->atol = atol;
+  b->rtol = rtol;
+  b->n_var = n;
+  b->tableau = tableau;
+  b->rhs_kind = ode->rhs_kind;
+  b->nbody_variant = ode->nbody_variant;
+  b->n_particles = static_cast<int>(n / 6);
+  b->eps2 = ode->softening * ode->softening;
+  b->mass = ode->masses;
+  return ODEHIP_OK;
+}
+
+// Work stream for graph capture: the context's own stream, ordered after the
+// caller's stream when an external (possibly legacy default) stream is set.
+hipStream_t begin_work(odehip_context* ctx) {
+  if (ctx->stream != ctx->own) {
+    (void)hipEventRecord(ctx->ev_in, ctx->stream);
+    (void)hipStreamWaitEvent(ctx->own, ctx->ev_in, 0);
+  }
+  return ctx->own;
+}
+
+void end_work(odehip_context* ctx) {
+  if (ctx->stream != ctx->own) {
+    (void)hipEventRecord(ctx->ev_out, ctx->own);
+    (void)hipStreamWaitEvent(ctx->stream, ctx->ev_out, 0);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* odehip_version(void) { return "odehip 1.0 gfx950"; }
+
+int odehip_create(int device, odehip_context** out) {
+  if (!out) return ODEHIP_ERR_INVALID;
+  *out = nullptr;
+  auto* ctx = new (std::nothrow) odehip_context();
+  if (!ctx) return ODEHIP_ERR_NOMEM;
+  ctx->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&ctx->st, sizeof(SolverState));
+  if (e == hipSuccess) e = hipHostMalloc(&ctx->h_done, 2 * sizeof(int));
+  if (e == hipSuccess) e = hipHostMalloc(&ctx->h_st, sizeof(SolverState));
+  if (e == hipSuccess) e = hipHostMalloc(&ctx->h_scalar, 8 * sizeof(double));
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev[0], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev[1], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_in, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_out, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreate(&ctx->t_start);
+  if (e == hipSuccess) e = hipEventCreate(&ctx->t_stop);
+  if (e != hipSuccess) {
+    odehip_destroy(ctx);
+    return ODEHIP_ERR_HIP;
+  }
+  ctx->stream = ctx->own;
+  *out = ctx;
+  return ODEHIP_OK;
+}
+
+int odehip_destroy(odehip_context* ctx) {
+  if (!ctx) return ODEHIP_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->own) (void)hipStreamSynchronize(ctx->own);
+  if (ctx->exec) (void)hipGraphExecDestroy(ctx->exec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  if (ctx->ws) (void)hipFree(ctx->ws);
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->st) (void)hipFree(ctx->st);
+  if (ctx->h_done) (void)hipHostFree(ctx->h_done);
+  if (ctx->h_st) (void)hipHostFree(ctx->h_st);
+  if (ctx->h_scalar) (void)hipHostFree(ctx->h_scalar);
+  for (auto& ev : ctx->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (ctx->ev_in) (void)hipEventDestroy(ctx->ev_in);
+  if (ctx->ev_out) (void)hipEventDestroy(ctx->ev_out);
+  if (ctx->t_start) (void)hipEventDestroy(ctx->t_start);
+  if (ctx->t_stop) (void)hipEventDestroy(ctx->t_stop);
+  if (ctx->own) (void)hipStreamDestroy(ctx->own);
+  delete ctx;
+  return ODEHIP_OK;
+}
+
+const char* odehip_last_error(const odehip_context* ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+int odehip_set_stream(odehip_context* ctx, void* hip_stream) {
+  if (!ctx) return ODEHIP_ERR_INVALID;
+  ctx->stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->own;
+  return ODEHIP_OK;
+}
+
+int odehip_synchronize(odehip_context* ctx) {
+  if (!ctx) return ODEHIP_ERR_INVALID;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->own));
+  return ODEHIP_OK;
+}
+
+int odehip_timer_start(odehip_context* ctx) {
+  if (!ctx) return ODEHIP_ERR_INVALID;
+  HIP_TRY(ctx, hipEventRecord(ctx->t_start, ctx->stream));
+  return ODEHIP_OK;
+}
+
+int odehip_timer_stop(odehip_context* ctx, double* elapsed_ms) {
+  if (!ctx || !elapsed_ms) return ODEHIP_ERR_INVALID;
+  HIP_TRY(ctx, hipEventRecord(ctx->t_stop, ctx->stream));
+  HIP_TRY(ctx, hipEventSynchronize(ctx->t_stop));
+  float ms = 0.0f;
+  HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->t_start, ctx->t_stop));
+  *elapsed_ms = ms;
+  return ODEHIP_OK;
+}
+
+// ------------------------------------------------------------------ memory --
+int odehip_alloc(odehip_context* ctx, size_t n, double** out) {
+  if (!ctx || !out) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  *out = nullptr;
+  hipError_t e = hipMalloc(out, std::max<size_t>(n, 1) * sizeof(double));
+  if (e != hipSuccess) return fail(ctx, ODEHIP_ERR_NOMEM, hipGetErrorString(e));
+  // CudaState() zero-fills (CudaState.cuh:15-18).
+  e = launch_fill(*out, 0.0, n, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "fill");
+  return ODEHIP_OK;
+}
+
+int odehip_free(odehip_context* ctx, double* p) {
+  if (!ctx) return ODEHIP_ERR_INVALID;
+  if (!p) return ODEHIP_OK;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipFree(p));
+  return ODEHIP_OK;
+}
+
+int odehip_copy_h2d(odehip_context* ctx, double* dst, const double* src, size_t n) {
+  if (!ctx || (n && (!dst || !src))) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  HIP_TRY(ctx, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return ODEHIP_OK;
+}
+
+int odehip_copy_d2h(odehip_context* ctx, double* dst, const double* src, size_t n) {
+  if (!ctx || (n && (!dst || !src))) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  HIP_TRY(ctx, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return ODEHIP_OK;
+}
+
+int odehip_copy_d2d(odehip_context* ctx, double* dst, const double* src, size_t n) {
+  if (!ctx || (n && (!dst || !src))) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  HIP_TRY(ctx, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  return ODEHIP_OK;
+}
+
+int odehip_fill(odehip_context* ctx, double* dst, double v, size_t n) {
+  if (!ctx || (n && !dst)) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  HIP_TRY(ctx, launch_fill(dst, v, n, ctx->stream));
+  return ODEHIP_OK;
+}
+
+// --------------------------------------------------------- executor ops ----
+int odehip_nbody_rhs(odehip_context* ctx, const double* x, double* dxdt,
+                     size_t n, double softening, const double* masses,
+                     int variant) {
+  if (!ctx || !x || !dxdt) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  if (n == 0 || n > (1u << 26)) return fail(ctx, ODEHIP_ERR_INVALID, "bad particle count");
+  if (variant < 0 || variant > 2) return fail(ctx, ODEHIP_ERR_INVALID, "bad variant");
+  NBodyArgs a{};
+  a.y = x;
+  a.k = dxdt;
+  a.mass = masses;
+  a.n = static_cast<int>(n);
+  a.eps2 = softening * softening;
+  HIP_TRY(ctx, launch_nbody_rhs(a, variant, ctx->stream));
+  return ODEHIP_OK;
+}
+
+int odehip_nbody_acc(odehip_context* ctx, const double* pos, double* acc,
+                     size_t n, double softening, const double* masses,
+                     int variant) {
+  if (!ctx || !pos || !acc) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  if (n == 0 || n > (1u << 26)) return fail(ctx, ODEHIP_ERR_INVALID, "bad particle count");
+  // The RHS kernels read a full [pos | vel] state and write [vel | acc]; stage
+  // pos with zero velocities in scratch and extract acc.
+  int rc = ensure_scratch(ctx, 12 * n);
+  if (rc) return rc;
+  double* xs = ctx->scratch;
+  double* ks = ctx->scratch + 6 * n;
+  HIP_TRY(ctx, hipMemcpyAsync(xs, pos, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  HIP_TRY(ctx, launch_fill(xs + 3 * n, 0.0, 3 * n, ctx->stream));
+  rc = odehip_nbody_rhs(ctx, xs, ks, n, softening, masses, variant);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipMemcpyAsync(acc, ks + 3 * n, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  return ODEHIP_OK;
+}
+
+int odehip_ode_rhs(odehip_context* ctx, int kind, const double* x, double* dxdt,
+                   size_t n_var) {
+  if (!ctx || !x || !dxdt) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  if (kind == ODEHIP_RHS_NBODY) {
+    if (n_var % 6) return fail(ctx, ODEHIP_ERR_INVALID, "n_var % 6 != 0");
+    return odehip_nbody_rhs(ctx, x, dxdt, n_var / 6, 0.0, nullptr, ODEHIP_NBODY_EXACT);
+  }
+  if (kind == ODEHIP_RHS_VANDERPOL && n_var != 2)
+    return fail(ctx, ODEHIP_ERR_INVALID, "Van der Pol has n_var 2");
+  hipError_t e = launch_simple_rhs(kind, x, nullptr, nullptr, nullptr, dxdt,
+                                   nullptr, 0, n_var, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "ode_rhs");
+  return ODEHIP_OK;
+}
+
+int odehip_rk_stage(odehip_context* ctx, int tableau, int stage, double dt,
+                    double* temp, const double* ks, const double* x0,
+                    size_t n_var) {
+  if (!ctx || !temp || !ks || !x0) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  if (!valid_tableau(tableau)) return fail(ctx, ODEHIP_ERR_INVALID, "bad tableau");
+  if (stage < 1 || stage >= kTableaus[tableau].S)
+    return fail(ctx, ODEHIP_ERR_INVALID, "stage out of range");
+  HIP_TRY(ctx, launch_rk_stage_host_dt(tableau, stage, dt, temp, ks, x0, n_var, ctx->stream));
+  return ODEHIP_OK;
+}
+
+int odehip_rk_update_state_and_error(odehip_context* ctx, int tableau,
+                                     double dt, double* x, double* err,
+                                     const double* ks, const double* x0,
+                                     size_t n_var) {
+  if (!ctx || !x || !err || !ks || !x0) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  if (!valid_tableau(tableau)) return fail(ctx, ODEHIP_ERR_INVALID, "bad tableau");
+  HIP_TRY(ctx, launch_rk_update_host_dt(tableau, dt, x, err, ks, x0, n_var, ctx->stream));
+  return ODEHIP_OK;
+}
+
+int odehip_rk_update_error_target(odehip_context* ctx, double* sc,
+                                  const double* atol, const double* rtol,
+                                  const double* x, const double* x0,
+                                  size_t n_var) {
+  if (!ctx || !sc || !atol || !rtol || !x || !x0)
+    return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  HIP_TRY(ctx, launch_rk_target(sc, atol, rtol, x, x0, n_var, ctx->stream));
+  return ODEHIP_OK;
+}
+
+int odehip_rk_norm(odehip_context* ctx, const double* v, const double* sc,
+                   size_t n_var, double* host_result) {
+  if (!ctx || !v || !sc || !host_result) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  if (n_var == 0) return fail(ctx, ODEHIP_ERR_INVALID, "n_var is 0");
+  const int ng = n_groups_for(n_var);
+  int rc = ensure_scratch(ctx, static_cast<size_t>(ng) + 8);
+  if (rc) return rc;
+  HIP_TRY(ctx, launch_norm_partials(v, nullptr, sc, n_var, ctx->scratch + 8, ctx->stream));
+  HIP_TRY(ctx, launch_fold_norm(ctx->scratch + 8, ng, n_var, ctx->scratch, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->h_scalar, ctx->scratch, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  *host_result = ctx->h_scalar[0];
+  return ODEHIP_OK;
+}
+
+int odehip_rk_initial_step(odehip_context* ctx, int tableau, int rhs_kind,
+                           const double* x0, const double* atol,
+                           const double* rtol, size_t n_var, double softening,
+                           const double* masses, int variant, double* host_dt) {
+  if (!ctx || !x0 || !atol || !rtol || !host_dt)
+    return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  if (!valid_tableau(tableau)) return fail(ctx, ODEHIP_ERR_INVALID, "bad tableau");
+  odehip_ode ode{rhs_kind, variant, n_var, softening, masses};
+  int rc = check_ode(ctx, &ode);
+  if (rc) return rc;
+  SolveBufs b{};
+  rc = prepare_bufs(ctx, tableau, &ode, atol, rtol, 0, 0, &b);
+  if (rc) return rc;
+  hipStream_t s = begin_work(ctx);
+  HIP_TRY(ctx, enqueue_solve_init(b, x0, 0.0, 1.0, 0, 0, 0, s));
+  HIP_TRY(ctx, enqueue_initial_step(b, s));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->h_st, ctx->st, sizeof(SolverState), hipMemcpyDeviceToHost, s));
+  HIP_TRY(ctx, hipStreamSynchronize(s));
+  end_work(ctx);
+  *host_dt = ctx->h_st->dt_initial;
+  return ODEHIP_OK;
+}
+
+// ---------------------------------------------------------------- solve ----
+int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
+                    const odehip_solve_options* opt, double* x,
+                    odehip_solve_stats* stats) {
+  if (!ctx) return ODEHIP_ERR_INVALID;
+  if (!opt || !x) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  if (!valid_tableau(tableau)) return fail(ctx, ODEHIP_ERR_INVALID, "bad tableau");
+  int rc = check_ode(ctx, ode);
+  if (rc) return rc;
+  if (!opt->atol || !opt->rtol) return fail(ctx, ODEHIP_ERR_INVALID, "atol/rtol null");
+  if (opt->max_accepted < 0 || opt->max_attempts < 0 || opt->times_cap < 0 ||
+      opt->trace_cap < 0)
+    return fail(ctx, ODEHIP_ERR_INVALID, "negative budget or capacity");
+  const long times_cap = opt->times ? opt->times_cap : 0;
+  const long trace_cap = opt->trace ? opt->trace_cap : 0;
+  SolveBufs b{};
+  rc = prepare_bufs(ctx, tableau, ode, opt->atol, opt->rtol, times_cap,
+                    trace_cap, &b);
+  if (rc) return rc;
+  const int S = kTableaus[tableau].S;
+  const size_t n = ode->n_var;
+
+  // attempts per replay: aim for several ms of work per replay
+  int K = opt->attempts_per_launch;
+  if (K == 0) {
+    const double work = ode->rhs_kind == ODEHIP_RHS_NBODY
+                            ? static_cast<double>(S) * (n / 6.0) * (n / 6.0)
+                            : static_cast<double>(S) * n;
+    K = work > 4.0e9 ? 2 : (work > 1.0e8 ? 8 : 32);
+  }
+
+  hipStream_t s = begin_work(ctx);
+  HIP_TRY(ctx, enqueue_solve_init(b, x, opt->t0, opt->tf, opt->max_accepted,
+                                  opt->max_attempts, opt->fsal ? 1 : 0, s));
+  HIP_TRY(ctx, enqueue_initial_step(b, s));
+
+  if (K > 0) {
+    GraphKey key;
+    key.tableau = tableau;
+    key.rhs_kind = ode->rhs_kind;
+    key.variant = ode->nbody_variant;
+    key.k = K;
+    key.fsal = opt->fsal ? 1 : 0;
+    key.n_var = n;
+    key.eps2 = b.eps2;
+    key.mass = ode->masses;
+    key.atol = opt->atol;
+    key.rtol = opt->rtol;
+    key.ws = ctx->ws;
+    // times/trace pointers live in ws; their presence changes the layout:
+    key.ws = reinterpret_cast<const char*>(ctx->ws) + (b.times ? 1 : 0) + (b.trace ? 2 : 0);
+    if (!ctx->exec || !(key == ctx->key)) {
+      if (ctx->exec) {
+        (void)hipGraphExecDestroy(ctx->exec);
+        ctx->exec = nullptr;
+      }
+      if (ctx->graph) {
+        (void)hipGraphDestroy(ctx->graph);
+        ctx->graph = nullptr;
+      }
+      HIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      hipError_t ce = hipSuccess;
+      for (int a = 0; a < K && ce == hipSuccess; ++a) ce = enqueue_attempt(b, s);
+      hipGraph_t g = nullptr;
+      hipError_t ee = hipStreamEndCapture(s, &g);
+      if (ce != hipSuccess) return hip_fail(ctx, ce, "capture attempt");
+      if (ee != hipSuccess) return hip_fail(ctx, ee, "hipStreamEndCapture");
+      ctx->graph = g;
+      HIP_TRY(ctx, hipGraphInstantiate(&ctx->exec, g, nullptr, nullptr, 0));
+      ctx->key = key;
+    }
+  }
+
+  const long host_cap_attempts =
+      opt->max_attempts > 0 ? opt->max_attempts : (1L << 40);
+  long launched = 0;
+  int slot = 0;
+  bool have_prev = false;
+  for (;;) {
+    if (K > 0) {
+      HIP_TRY(ctx, hipGraphLaunch(ctx->exec, s));
+      launched += K;
+    } else {
+      const int direct = 16;
+      for (int a = 0; a < direct; ++a) HIP_TRY(ctx, enqueue_attempt(b, s));
+      launched += direct;
+    }
+    HIP_TRY(ctx, hipMemcpyAsync(&ctx->h_done[slot], &ctx->st->done, sizeof(int),
+                                hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[slot], s));
+    if (have_prev) {
+      HIP_TRY(ctx, hipEventSynchronize(ctx->ev[slot ^ 1]));
+      if (ctx->h_done[slot ^ 1]) break;
+    }
+    have_prev = true;
+    slot ^= 1;
+    if (launched >= host_cap_attempts + 2L * std::max(K, 16)) break;
+  }
+  HIP_TRY(ctx, enqueue_solve_finish(b, x, s));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->h_st, ctx->st, sizeof(SolverState),
+                              hipMemcpyDeviceToHost, s));
+  HIP_TRY(ctx, hipStreamSynchronize(s));
+  const SolverState& hs = *ctx->h_st;
+  const long n_times = std::min(hs.accepted + 1, times_cap);
+  const long n_trace = std::min(hs.attempts, trace_cap);
+  if (opt->times && n_times > 0)
+    HIP_TRY(ctx, hipMemcpy(opt->times, b.times, n_times * sizeof(double), hipMemcpyDeviceToHost));
+  if (opt->trace && n_trace > 0)
+    HIP_TRY(ctx, hipMemcpy(opt->trace, b.trace, 3 * n_trace * sizeof(double), hipMemcpyDeviceToHost));
+  end_work(ctx);
+  if (stats) {
+    stats->accepted = hs.accepted;
+    stats->rejected = hs.rejected;
+    stats->attempts = hs.attempts;
+    stats->rhs_evals = hs.rhs_evals;
+    stats->n_times = n_times;
+    stats->n_trace = n_trace;
+    stats->status = hs.status;
+    stats->t = hs.t;
+    stats->dt = hs.dt;
+    stats->dt_initial = hs.dt_initial;
+    stats->last_err = hs.err;
+  }
+  if (hs.status == ODEHIP_STATUS_RUNNING)
+    return fail(ctx, ODEHIP_ERR_INVALID, "solve did not finish within the attempt cap");
+  return ODEHIP_OK;
+}
+
+// ------------------------------------------------------ conserved quantities --
+int odehip_nbody_energy(odehip_context* ctx, const double* x, size_t n,
+                        double softening, const double* masses,
+                        double* host_out) {
+  if (!ctx || !x || !host_out) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  if (n == 0) return fail(ctx, ODEHIP_ERR_INVALID, "n is 0");
+  const size_t ng = static_cast<size_t>(n_groups_for(n));
+  int rc = ensure_scratch(ctx, 2 * n + 2 * (ng + 1) + 8);
+  if (rc) return rc;
+  double* per = ctx->scratch;
+  double* sums = ctx->scratch + 2 * n;
+  HIP_TRY(ctx, launch_nbody_energy(x, static_cast<int>(n), softening * softening,
+                                   masses, per, ctx->stream));
+  HIP_TRY(ctx, launch_sum_tree(per, n, sums, ctx->stream));
+  HIP_TRY(ctx, launch_sum_tree(per + n, n, sums + ng + 1, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->h_scalar, sums, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->h_scalar + 1, sums + ng + 1, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  host_out[0] = ctx->h_scalar[0];
+  host_out[1] = ctx->h_scalar[1];
+  return ODEHIP_OK;
+}
+
+int odehip_nbody_momentum(odehip_context* ctx, const double* x, size_t n,
+                          const double* masses, double* host_out) {
+  if (!ctx || !x || !host_out) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  if (n == 0) return fail(ctx, ODEHIP_ERR_INVALID, "n is 0");
+  const size_t ng = static_cast<size_t>(n_groups_for(n));
+  int rc = ensure_scratch(ctx, 3 * n + 3 * (ng + 1) + 8);
+  if (rc) return rc;
+  double* per = ctx->scratch;
+  double* sums = ctx->scratch + 3 * n;
+  HIP_TRY(ctx, launch_momentum_terms(x, static_cast<int>(n), masses, per, ctx->stream));
+  for (int c = 0; c < 3; ++c) {
+    HIP_TRY(ctx, launch_sum_tree(per + c * n, n, sums + c * (ng + 1), ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->h_scalar + c, sums + c * (ng + 1), sizeof(double),
+                                hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  for (int c = 0; c < 3; ++c) host_out[c] = ctx->h_scalar[c];
+  return ODEHIP_OK;
+}
+
+}  // extern "C"

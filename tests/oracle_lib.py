@@ -1,0 +1,182 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) for the tests.
+
+Test infrastructure only: the oracle is the checker, never the product.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_LIB = None
+
+RHS_NBODY, RHS_EXPONENTIAL, RHS_VANDERPOL = 0, 1, 2
+REDUCE_SEQUENTIAL, REDUCE_CHUNKED, REDUCE_THREADPOOL, REDUCE_HIP = 0, 1, 2, 3
+HIP_ORDER = (REDUCE_HIP, 0)
+POW_LIBM, POW_CORRECT = 0, 1
+STATUS_REACHED_END, STATUS_MAX_ACCEPTED, STATUS_NONFINITE, STATUS_MAX_ATTEMPTS = 0, 1, 2, 3
+
+_dp = C.POINTER(C.c_double)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(ROOT, "oracle", "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(path)
+        L.oracle_rhs.argtypes = [C.c_int, _dp, _dp, C.c_size_t, C.c_double, _dp, C.c_int]
+        L.oracle_rk_norm.argtypes = [_dp, _dp, C.c_size_t, C.c_int, C.c_long]
+        L.oracle_rk_norm.restype = C.c_double
+        L.oracle_integrate.argtypes = [C.c_char_p, C.c_int, _dp, C.c_size_t, C.c_double, C.c_double,
+                                       _dp, _dp, C.c_double, _dp, C.c_int, C.c_long, C.c_int,
+                                       C.c_long, C.c_long, C.c_int, C.c_int, C.c_int]
+        L.oracle_integrate.restype = C.c_void_p
+        L.oracle_result_count.argtypes = [C.c_void_p]
+        L.oracle_result_count.restype = C.c_long
+        L.oracle_result_times.argtypes = [C.c_void_p, _dp]
+        L.oracle_result_state.argtypes = [C.c_void_p, C.c_long, _dp]
+        L.oracle_result_final.argtypes = [C.c_void_p, _dp]
+        L.oracle_result_stats.argtypes = [C.c_void_p, C.POINTER(C.c_long), _dp]
+        L.oracle_result_trace_len.argtypes = [C.c_void_p]
+        L.oracle_result_trace_len.restype = C.c_long
+        L.oracle_result_trace.argtypes = [C.c_void_p, _dp, _dp, C.POINTER(C.c_int)]
+        L.oracle_result_free.argtypes = [C.c_void_p]
+        L.oracle_nr_dop853.argtypes = [C.c_int] + [C.c_double] * 8 + [C.c_double, C.POINTER(C.c_long), _dp]
+        L.oracle_energy.argtypes = [_dp, C.c_size_t, C.c_double, _dp, _dp, _dp]
+        L.oracle_momentum.argtypes = [_dp, C.c_size_t, _dp, _dp]
+        L.oracle_tableau.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int), _dp, _dp, _dp]
+        L.oracle_spinning_cube_ic.argtypes = [C.c_long, _dp, _dp]
+        L.oracle_pow.argtypes = [C.c_double, C.c_double, C.c_int]
+        L.oracle_pow.restype = C.c_double
+        _LIB = L
+    return _LIB
+
+
+class Result:
+    """Outputs of one oracle integration (RawOutput-like)."""
+
+    def __init__(self, handle, n_var):
+        L = lib()
+        self.n_var = n_var
+        n = L.oracle_result_count(handle)
+        self.times = np.zeros(n)
+        L.oracle_result_times(handle, _ptr(self.times))
+        self.x_final = np.zeros(n_var)
+        L.oracle_result_final(handle, _ptr(self.x_final))
+        st = (C.c_long * 4)()
+        ds = np.zeros(3)
+        L.oracle_result_stats(handle, st, _ptr(ds))
+        self.accepted, self.rejected, self.rhs_evals, self.status = list(st)
+        self.t, self.dt, self.dt_initial = ds
+        nt = L.oracle_result_trace_len(handle)
+        self.trace_err = np.zeros(nt)
+        self.trace_dt = np.zeros(nt)
+        acc = (C.c_int * max(nt, 1))()
+        if nt:
+            L.oracle_result_trace(handle, _ptr(self.trace_err), _ptr(self.trace_dt), acc)
+        self.trace_acc = np.array(list(acc)[:nt], dtype=np.int32)
+        self._states = {}
+        self._handle = handle
+
+    def state(self, k):
+        if k < 0:
+            k += len(self.times)
+        out = np.zeros(self.n_var)
+        if lib().oracle_result_state(self._handle, k, _ptr(out)) != 0:
+            raise IndexError(k)
+        return out
+
+    def __del__(self):
+        try:
+            lib().oracle_result_free(self._handle)
+        except Exception:
+            pass
+
+
+def integrate(tableau, rhs_kind, x0, t0, tf, atol, rtol, softening=0.0, masses=None,
+              reduce=(REDUCE_SEQUENTIAL, 0), pow_mode=POW_LIBM, max_accepted=0,
+              max_attempts=0, store_states=True, store_trace=False, threads=1):
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    n = x0.size
+    atol = np.ascontiguousarray(np.broadcast_to(np.asarray(atol, dtype=np.float64), (n,)))
+    rtol = np.ascontiguousarray(np.broadcast_to(np.asarray(rtol, dtype=np.float64), (n,)))
+    m = None if masses is None else np.ascontiguousarray(masses, dtype=np.float64)
+    h = lib().oracle_integrate(tableau.encode(), rhs_kind, _ptr(x0), n, t0, tf, _ptr(atol),
+                               _ptr(rtol), softening, _ptr(m), reduce[0], reduce[1], pow_mode,
+                               max_accepted, max_attempts, int(store_states), int(store_trace),
+                               threads)
+    if not h:
+        raise RuntimeError("oracle_integrate failed")
+    return Result(h, n)
+
+
+def rhs(rhs_kind, x, softening=0.0, masses=None, threads=1):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros_like(x)
+    m = None if masses is None else np.ascontiguousarray(masses, dtype=np.float64)
+    if lib().oracle_rhs(rhs_kind, _ptr(x), _ptr(out), x.size, softening, _ptr(m), threads) != 0:
+        raise RuntimeError("oracle_rhs failed")
+    return out
+
+
+def rk_norm(v, sc, reduce=(REDUCE_SEQUENTIAL, 0)):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    sc = np.ascontiguousarray(sc, dtype=np.float64)
+    return lib().oracle_rk_norm(_ptr(v), _ptr(sc), v.size, reduce[0], reduce[1])
+
+
+def nr_dop853(problem, y0, x1, x2, atol, rtol, h1, hmin=0.0, mu_eps=1e-3):
+    cnt = C.c_long()
+    sel = np.zeros(9)
+    st = lib().oracle_nr_dop853(problem, y0[0], y0[1], x1, x2, atol, rtol, h1, hmin, mu_eps,
+                                C.byref(cnt), _ptr(sel))
+    return st, cnt.value, sel
+
+
+def energy(x, n, softening, masses=None):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    ke, pe = np.zeros(1), np.zeros(1)
+    m = None if masses is None else np.ascontiguousarray(masses, dtype=np.float64)
+    lib().oracle_energy(_ptr(x), n, softening, _ptr(m), _ptr(ke), _ptr(pe))
+    return float(ke[0]), float(pe[0])
+
+
+def momentum(x, n, masses=None):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    p = np.zeros(3)
+    m = None if masses is None else np.ascontiguousarray(masses, dtype=np.float64)
+    lib().oracle_momentum(_ptr(x), n, _ptr(m), _ptr(p))
+    return p
+
+
+def tableau(name):
+    S, p, pt = C.c_int(), C.c_int(), C.c_int()
+    if lib().oracle_tableau(name.encode(), C.byref(S), C.byref(p), C.byref(pt), None, None, None):
+        raise KeyError(name)
+    s = S.value
+    a = np.zeros((s - 1) * (s - 1))
+    b = np.zeros(s)
+    bt = np.zeros(s)
+    lib().oracle_tableau(name.encode(), C.byref(S), C.byref(p), C.byref(pt), _ptr(a), _ptr(b), _ptr(bt))
+    return dict(S=s, p=p.value, pt=pt.value, a=a.reshape(s - 1, s - 1), b=b, bt=bt)
+
+
+def pow_(x, y, mode=POW_LIBM):
+    return lib().oracle_pow(x, y, mode)
+
+
+def spinning_cube_ic(n):
+    """SpinningCubeNBodyTest<n> state and (softening, tf, tol)."""
+    x = np.zeros(6 * n)
+    p = np.zeros(3)
+    lib().oracle_spinning_cube_ic(n, _ptr(x), _ptr(p))
+    return x, float(p[0]), float(p[1]), float(p[2])

@@ -1,0 +1,192 @@
+"""N-body RHS and executor-level kernels on the GPU against the CPU oracle.
+
+Bar: the exact variant is bit-identical to the oracle (itself bit-identical
+to the reference's NBodyODE with SingleThreadedExecutor); the fast FP64
+variant within 5e-14 of the largest acceleration component; the mixed
+variant within 2e-6.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [1, 2, 3, 5, 63, 64, 65, 100, 257, 1000, 1024, 2049]
+
+
+def ic(n, seed=0, masses=True):
+    from ode_prototyping_amd import scenarios
+    x, m = scenarios.uniform_cube(n, seed=seed)
+    return x, (m if masses else None)
+
+
+def gpu_rhs(exe, x, softening, masses, variant):
+    import ode_prototyping_amd as oh
+    xs = oh.HipState(exe, data=x)
+    ks = oh.HipState(exe, x.size)
+    oh.NBodyODE(softening, masses, variant)(exe, xs, ks)
+    return ks.to_numpy()
+
+
+@pytest.mark.parametrize("n", SIZES)
+@pytest.mark.parametrize("masses", [False, True])
+def test_exact_rhs_bit_identical(exe, n, masses):
+    x, m = ic(n, seed=n, masses=masses)
+    got = gpu_rhs(exe, x, 1e-3, m, "exact")
+    want = O.rhs(O.RHS_NBODY, x, 1e-3, m, threads=8)
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), \
+        f"n={n}: max abs diff {np.max(np.abs(got - want))}"
+
+
+def test_exact_rhs_unsoftened(exe):
+    """eps = 0: the diagonal term must be skipped, not evaluated (0/0)."""
+    x, _ = ic(300, seed=3, masses=False)
+    got = gpu_rhs(exe, x, 0.0, None, "exact")
+    want = O.rhs(O.RHS_NBODY, x, 0.0, None)
+    assert np.all(np.isfinite(got))
+    assert np.array_equal(got, want)
+
+
+def test_exact_rhs_reference_spinning_cube(exe):
+    """The reference's own 1024-body test state (SpinningCubeNBodyTest.hpp)."""
+    x, soft, _, _ = O.spinning_cube_ic(1024)
+    got = gpu_rhs(exe, x, soft, None, "exact")
+    want = O.rhs(O.RHS_NBODY, x, soft, None, threads=8)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("n", [2, 65, 1000, 4096])
+@pytest.mark.parametrize("eps", [1e-2, 0.0])
+def test_fast_rhs_close(exe, n, eps):
+    x, m = ic(n, seed=7)
+    got = gpu_rhs(exe, x, eps, m, "fast")
+    want = O.rhs(O.RHS_NBODY, x, eps, m, threads=8)
+    acc_g, acc_w = got[3 * n:], want[3 * n:]
+    assert np.array_equal(got[:3 * n], want[:3 * n])
+    scale = np.max(np.abs(acc_w))
+    assert np.max(np.abs(acc_g - acc_w)) <= 5e-14 * scale
+
+
+@pytest.mark.parametrize("n", [2, 65, 1000, 4096])
+def test_mixed_rhs_close(exe, n):
+    x, m = ic(n, seed=11)
+    got = gpu_rhs(exe, x, 1e-2, m, "mixed")
+    want = O.rhs(O.RHS_NBODY, x, 1e-2, m, threads=8)
+    acc_g, acc_w = got[3 * n:], want[3 * n:]
+    assert np.array_equal(got[:3 * n], want[:3 * n])
+    scale = np.max(np.abs(acc_w))
+    assert np.max(np.abs(acc_g - acc_w)) <= 2e-6 * scale
+
+
+def test_nbody_acc_matches_rhs(exe):
+    """NBodyODE::acc (positions -> accelerations) equals the RHS's acc half."""
+    import ode_prototyping_amd as oh
+    n = 300
+    x, m = ic(n, seed=5)
+    ode = oh.NBodyODE(1e-3, m, "exact")
+    xs = oh.HipState(exe, data=x)
+    acc = oh.HipState(exe, 3 * n)
+    ode.acc(exe, xs, acc)
+    want = O.rhs(O.RHS_NBODY, x, 1e-3, m)[3 * n:]
+    assert np.array_equal(acc.to_numpy(), want)
+
+
+def test_simple_odes(exe):
+    import ode_prototyping_amd as oh
+    x = np.array([2.0, 0.5])
+    xs, ks = oh.HipState(exe, data=x), oh.HipState(exe, 2)
+    oh.VanDerPolODE()(exe, xs, ks)
+    assert np.array_equal(ks.to_numpy(), O.rhs(O.RHS_VANDERPOL, x))
+    x = np.arange(10.0) * 1.5
+    xs, ks = oh.HipState(exe, data=x), oh.HipState(exe, 10)
+    oh.ExponentialODE()(exe, xs, ks)
+    assert np.array_equal(ks.to_numpy(), x)
+
+
+def test_state_semantics(exe):
+    """HipState mirrors CudaState (CudaStateTest.cu:18-92): zero default,
+    host round trip, deep copy, independence, fill."""
+    import ode_prototyping_amd as oh
+    s = oh.HipState(exe, 10)
+    assert np.array_equal(s.to_numpy(), np.zeros(10))
+    data = np.arange(10.0)
+    a = oh.HipState(exe, data=data)
+    assert np.array_equal(a.to_numpy(), data)
+    b = a.copy()
+    exe.fill(a, 3.0)
+    assert np.array_equal(b.to_numpy(), data)
+    assert np.array_equal(a.to_numpy(), np.full(10, 3.0))
+
+
+def test_rk_norm_canonical_order(exe):
+    """rk_norm follows the documented group order bit for bit, for sizes
+    around the group boundaries; with n <= 2 it equals the sequential sum."""
+    import ode_prototyping_amd as oh
+    rng = np.random.default_rng(0)
+    for n in [1, 2, 3, 64, 255, 256, 257, 1000, 98304]:
+        v = rng.normal(size=n) * 1e-9
+        sc = 1e-10 + 1e-10 * np.abs(rng.normal(size=n))
+        got = exe.rk_norm(oh.HipState(exe, data=v), oh.HipState(exe, data=sc))
+        assert got == O.rk_norm(v, sc, O.HIP_ORDER), n
+        if n <= 2:
+            assert got == O.rk_norm(v, sc)
+
+
+@pytest.mark.parametrize("tableau", ["HE21", "RKF45", "DOPRI5", "DVERK", "RKF78", "DOP853"])
+def test_stage_update_target_kernels(exe, tableau):
+    """rk_stage / update_state_and_error / update_error_target kernels equal
+    the reference expressions (RKEmbeddedParallel.hpp:91-130) evaluated in
+    the same order in IEEE double on the host."""
+    import ode_prototyping_amd as oh
+    T = O.tableau(tableau)
+    S, n = T["S"], 777
+    rng = np.random.default_rng(1)
+    ks_h = rng.normal(size=S * n)
+    x0_h = rng.normal(size=n)
+    dt = 0.0123
+    ks, x0 = oh.HipState(exe, data=ks_h), oh.HipState(exe, data=x0_h)
+    temp = oh.HipState(exe, n)
+    K = ks_h.reshape(S, n)
+    for stage in range(1, S):
+        exe.rk_stage(tableau, stage, dt, temp, ks, x0)
+        tmp = np.zeros(n)
+        for j in range(stage):
+            tmp = tmp + T["a"][stage - 1][j] * K[j]
+        assert np.array_equal(temp.to_numpy(), x0_h + tmp * dt), stage
+    x, err = oh.HipState(exe, n), oh.HipState(exe, n)
+    exe.rk_update_state_and_error(tableau, dt, x, err, ks, x0)
+    xs, es = np.zeros(n), np.zeros(n)
+    db = T["b"] - T["bt"]
+    for j in range(S):
+        xs = xs + T["b"][j] * K[j]
+        es = es + db[j] * K[j]
+    assert np.array_equal(x.to_numpy(), x0_h + xs * dt)
+    assert np.array_equal(err.to_numpy(), es * dt)
+    atol = np.full(n, 1e-9)
+    rtol = np.full(n, 1e-8)
+    sc = oh.HipState(exe, n)
+    exe.rk_update_error_target(sc, oh.HipState(exe, data=atol), oh.HipState(exe, data=rtol),
+                               x, x0)
+    xh = x.to_numpy()
+    want = atol + rtol * np.where(np.abs(xh) < np.abs(x0_h), np.abs(x0_h), np.abs(xh))
+    assert np.array_equal(sc.to_numpy(), want)
+
+
+def test_energy_and_momentum(exe):
+    import ode_prototyping_amd as oh
+    from ode_prototyping_amd import _lib
+    import ctypes as C
+    n = 1500
+    x, m = ic(n, seed=9)
+    xs, ms = oh.HipState(exe, data=x), oh.HipState(exe, data=m)
+    out = np.zeros(3)
+    exe.check(_lib.lib().odehip_nbody_energy(exe.handle, xs.ptr, n, 1e-3, ms.ptr,
+                                             out.ctypes.data_as(C.POINTER(C.c_double))), "e")
+    ke, pe = O.energy(x, n, 1e-3, m)
+    assert abs(out[0] - ke) <= 1e-13 * abs(ke)
+    assert abs(out[1] - pe) <= 1e-13 * abs(pe)
+    exe.check(_lib.lib().odehip_nbody_momentum(exe.handle, xs.ptr, n, ms.ptr,
+                                               out.ctypes.data_as(C.POINTER(C.c_double))), "p")
+    p = O.momentum(x, n, m)
+    assert np.all(np.abs(out - p) <= 1e-17)

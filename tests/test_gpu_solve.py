@@ -1,0 +1,252 @@
+"""The device-resident solve against the CPU oracle.
+
+The device controller uses a correctly rounded pow and the library's
+canonical reduction order; the oracle configured the same way (POW_CORRECT,
+HIP_ORDER) must agree bit for bit on the final state, every accepted time,
+the per-attempt (err, dt, accept) trace and the counters.  The host-driven
+integrator (host controller with the C library's pow, as the reference's CUDA
+executor) must agree bit for bit with the oracle in POW_LIBM + HIP_ORDER.
+Against the reference's own goldens the device path holds to the reference's
+EXPECT_DOUBLE_EQ (4 ulp) where the reduction order of a size-2 state is the
+sequential one, and to the thread-pool tests' tolerances otherwise.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                   "reference_goldens.json")))
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def device_solve(exe, tableau, x0, t0, tf, tol, ode, **kw):
+    import ode_prototyping_amd as oh
+    return oh.integrate(tableau, x0, t0, tf, tol, tol, ode, exe=exe, **kw)
+
+
+def oracle_solve(tableau, rhs_kind, x0, t0, tf, tol, softening=0.0, masses=None, **kw):
+    return O.integrate(tableau, rhs_kind, x0, t0, tf, tol, tol, softening=softening,
+                       masses=masses, reduce=O.HIP_ORDER, pow_mode=O.POW_CORRECT,
+                       store_states=False, store_trace=True, threads=8, **kw)
+
+
+def assert_same_solve(res, ref):
+    assert (res.accepted, res.rejected, res.rhs_evals) == (ref.accepted, ref.rejected,
+                                                           ref.rhs_evals)
+    assert np.array_equal(bits(res.times), bits(ref.times))
+    assert np.array_equal(bits(res.x_final), bits(ref.x_final))
+    assert res.t == ref.t and res.dt == ref.dt and res.dt_initial == ref.dt_initial
+    if res.trace is not None:
+        assert np.array_equal(bits(res.trace[:, 0]), bits(ref.trace_err))
+        assert np.array_equal(bits(res.trace[:, 1]), bits(ref.trace_dt))
+        assert np.array_equal(res.trace[:, 2].astype(int), ref.trace_acc)
+
+
+CASES = [
+    # tableau, N, ic, softening, masses, max_accepted
+    ("DOPRI5", 64, "cube", 1e-3, True, 300),
+    ("DOP853", 256, "plummer", 1e-2, True, 40),
+    ("RKF78", 100, "cube", 1e-3, False, 40),
+    ("DOP853", 1000, "plummer", 1e-2, True, 12),
+    ("HE21", 33, "cube", 1e-2, True, 200),
+    ("RKF45", 70, "cube", 1e-3, True, 100),
+    ("DVERK", 129, "cube", 1e-3, True, 60),
+]
+
+
+@pytest.mark.parametrize("tableau,n,kind,eps,use_m,steps", CASES)
+def test_device_solve_bit_identical_to_oracle(exe, tableau, n, kind, eps, use_m, steps):
+    import ode_prototyping_amd as oh
+    from ode_prototyping_amd import scenarios
+    x0, m = (scenarios.plummer(n, seed=1) if kind == "plummer"
+             else scenarios.uniform_cube(n, seed=2))
+    m = m if use_m else None
+    ref = oracle_solve(tableau, O.RHS_NBODY, x0, 0.0, 10.0, 1e-10, eps, m,
+                       max_accepted=steps)
+    res = device_solve(exe, tableau, x0, 0.0, 10.0, 1e-10, oh.NBodyODE(eps, m, "exact"),
+                       max_accepted=steps, trace_cap=4 * steps + 16)
+    assert res.status == "max_accepted" and res.accepted == steps
+    assert_same_solve(res, ref)
+
+
+def test_c1_cpu_configuration_full_solve(exe):
+    """The CPU-runnable configuration (N=64, DOPRI5, t in [0,1], rtol = atol =
+    1e-10, eps = 1e-3, masses 1/64) solved to the end on the device equals
+    the oracle; and the committed oracle fixture (sequential order, C library
+    pow) agrees with it to 1e-12."""
+    import ode_prototyping_amd as oh
+    f = np.load(os.path.join(os.path.dirname(__file__), "golden",
+                             "c1_uniform_cube_n64_seed0.npz"))
+    x0, m = f["x0"], f["masses"]
+    ref = oracle_solve("DOPRI5", O.RHS_NBODY, x0, 0.0, 1.0, 1e-10, 1e-3, m)
+    res = device_solve(exe, "DOPRI5", x0, 0.0, 1.0, 1e-10, oh.NBodyODE(1e-3, m, "exact"),
+                       trace_cap=4096)
+    assert res.status == "reached_end" and res.t == 1.0
+    assert_same_solve(res, ref)
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "c1_oracle_dopri5.npz"))
+    scale = np.max(np.abs(fx["x_final"]))
+    assert np.max(np.abs(res.x_final - fx["x_final"])) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("name", ["DOPRI5_VanDerPol", "RKF78_VanDerPol", "DVERK_VanDerPol",
+                                  "RKF45_VanDerPol", "DOPRI5_Exponential",
+                                  "RKF78_Exponential", "RKF78_NBody"])
+def test_device_solve_reference_goldens(exe, name):
+    """The reference's consistency cases through the device solve: bit-equal
+    to the oracle in the device's configuration, and within the reference's
+    own tolerances of its goldens."""
+    import ode_prototyping_amd as oh
+    g = GOLD["single_threaded"][name]
+    tab, prob = name.split("_", 1)
+    if prob == "VanDerPol":
+        x0, tf, tol, kind, ode = np.array([2.0, 0.0]), 2.0, 1e-10, O.RHS_VANDERPOL, \
+            oh.VanDerPolODE()
+    elif prob == "Exponential":
+        x0, tf, tol, kind, ode = np.arange(10.0), 10.0, 1e-6, O.RHS_EXPONENTIAL, \
+            oh.ExponentialODE()
+    else:
+        x0, tf, tol, kind, ode = np.array(GOLD["nbody5_x0"]["x0"]), 6.3, 1e-10, O.RHS_NBODY, \
+            oh.NBodyODE(0.0, None, "exact")
+    ref = oracle_solve(tab, kind, x0, 0.0, tf, tol)
+    res = device_solve(exe, tab, x0, 0.0, tf, tol, ode, trace_cap=20000)
+    assert_same_solve(res, ref)
+    assert len(res.times) == g["count"] and res.times[-1] == tf
+    last = res.x_final
+    if prob == "VanDerPol":
+        # n_var = 2: the canonical order is the sequential one; only pow differs
+        assert np.max(np.abs(last - g["last"]) / np.abs(g["last"])) < 1e-13
+    elif prob == "Exponential":
+        assert abs(last[5] - g["last"][1]) / g["last"][1] < 1e-12
+    else:
+        # ThreadPoolRKEmbeddedTest.cpp:347-351 tolerance (test.tol)
+        assert np.all(np.abs(last[[0, 3, 6, 9, 12]] - g["last"]) < 1e-10)
+
+
+def test_host_driven_matches_oracle_with_libm_pow(exe):
+    """integrate_host_driven runs the reference loop literally (host
+    controller, C library pow, device norm in the canonical order): bit-equal
+    to the oracle configured the same way, and to the single-threaded
+    DOPRI5 Van der Pol golden (n_var = 2)."""
+    import ode_prototyping_amd as oh
+    out = oh.RawOutput()
+    acc, rej, rhs = oh.integrate_host_driven("DOPRI5", np.array([2.0, 0.0]), 0.0, 2.0, 1e-10,
+                                             1e-10, oh.VanDerPolODE(), out, exe)
+    g = GOLD["single_threaded"]["DOPRI5_VanDerPol"]
+    assert len(out.times()) == g["count"]
+    mid = g["count"] // 2
+    assert out.times()[mid] == g["t_mid"]
+    assert list(out.states()[mid]) == g["mid"]
+    assert list(out.states()[-1]) == g["last"]
+    from ode_prototyping_amd import scenarios
+    x0, m = scenarios.uniform_cube(96, seed=4)
+    out = oh.RawOutput()
+    acc, rej, rhs = oh.integrate_host_driven("DOP853", x0, 0.0, 10.0, 1e-10, 1e-10,
+                                             oh.NBodyODE(1e-3, m, "exact"), out, exe,
+                                             max_accepted=15)
+    ref = O.integrate("DOP853", O.RHS_NBODY, x0, 0.0, 10.0, 1e-10, 1e-10, softening=1e-3,
+                      masses=m, reduce=O.HIP_ORDER, pow_mode=O.POW_LIBM, max_accepted=15)
+    assert (acc, rej, rhs) == (ref.accepted, ref.rejected, ref.rhs_evals)
+    assert np.array_equal(bits(out.times()), bits(ref.times))
+    for k in range(len(ref.times)):
+        assert np.array_equal(bits(out.states()[k]), bits(ref.state(k)))
+
+
+def test_fsal_and_reject_reuse_same_results(exe):
+    """FSAL (DOPRI5's last stage is the next first stage) and keeping k1
+    across a rejection change nothing but the RHS count."""
+    import ode_prototyping_amd as oh
+    from ode_prototyping_amd import scenarios
+    x0, m = scenarios.uniform_cube(64, seed=0)
+    ode = oh.NBodyODE(1e-3, m, "exact")
+    a = device_solve(exe, "DOPRI5", x0, 0.0, 1.0, 1e-10, ode, fsal=False)
+    b = device_solve(exe, "DOPRI5", x0, 0.0, 1.0, 1e-10, ode, fsal=True)
+    assert np.array_equal(bits(a.x_final), bits(b.x_final))
+    assert np.array_equal(bits(a.times), bits(b.times))
+    assert (a.accepted, a.rejected) == (b.accepted, b.rejected)
+    assert a.rhs_evals == 2 + 7 * (a.accepted + a.rejected)
+    # first attempt evaluates all 7 stages; every later one reuses k1
+    assert b.rhs_evals == 2 + 7 + 6 * (b.accepted + b.rejected - 1)
+
+
+def test_graph_replay_equals_direct_launch(exe):
+    import ode_prototyping_amd as oh
+    from ode_prototyping_amd import scenarios
+    x0, m = scenarios.plummer(200, seed=3)
+    ode = oh.NBodyODE(1e-2, m, "exact")
+    a = device_solve(exe, "DOP853", x0, 0.0, 10.0, 1e-10, ode, max_accepted=25,
+                     attempts_per_launch=-1)
+    b = device_solve(exe, "DOP853", x0, 0.0, 10.0, 1e-10, ode, max_accepted=25,
+                     attempts_per_launch=3)
+    c = device_solve(exe, "DOP853", x0, 0.0, 10.0, 1e-10, ode, max_accepted=25,
+                     attempts_per_launch=3)
+    for r in (b, c):
+        assert np.array_equal(bits(a.x_final), bits(r.x_final))
+        assert np.array_equal(bits(a.times), bits(r.times))
+
+
+def test_status_codes(exe):
+    import ode_prototyping_amd as oh
+    from ode_prototyping_amd import scenarios
+    x0, m = scenarios.uniform_cube(32, seed=6)
+    ode = oh.NBodyODE(1e-2, m, "exact")
+    r = device_solve(exe, "DOPRI5", x0, 0.0, 10.0, 1e-10, ode, max_accepted=5)
+    assert r.status == "max_accepted" and r.accepted == 5
+    r = device_solve(exe, "DOPRI5", x0, 0.0, 10.0, 1e-10, ode, max_attempts=3)
+    assert r.status == "max_attempts" and r.accepted + r.rejected == 3
+    r = device_solve(exe, "DOPRI5", x0, 1.0, 1.0, 1e-10, ode)
+    assert r.status == "reached_end" and r.attempts == 0
+    bad = x0.copy()
+    bad[5] = np.nan
+    r = device_solve(exe, "DOPRI5", bad, 0.0, 1.0, 1e-10, ode, max_attempts=10)
+    assert r.status == "nonfinite"
+    # a huge tolerance with a short interval ends at tf with few attempts
+    r = device_solve(exe, "DOPRI5", x0, 0.0, 1e-3, 1e-2, ode)
+    assert r.status == "reached_end" and r.t == 1e-3
+
+
+def test_fast_variant_parity_within_1e12(exe):
+    """The FP64 fast RHS (not bit-identical) after a fixed step count: the
+    relative state error against the exact oracle is <= 1e-12 and the
+    accept/reject sequence matches wherever |err - 1| > 1e-12."""
+    import ode_prototyping_amd as oh
+    from ode_prototyping_amd import scenarios
+    n, steps = 1024, 30
+    x0, m = scenarios.plummer(n, seed=5)
+    ref = oracle_solve("DOP853", O.RHS_NBODY, x0, 0.0, 10.0, 1e-10, 1e-2, m,
+                       max_accepted=steps)
+    res = device_solve(exe, "DOP853", x0, 0.0, 10.0, 1e-10, oh.NBodyODE(1e-2, m, "fast"),
+                       max_accepted=steps, trace_cap=200)
+    rel = np.max(np.abs(res.x_final - ref.x_final)) / np.max(np.abs(ref.x_final))
+    assert rel <= 1e-12, rel
+    k = min(len(ref.trace_acc), len(res.trace))
+    decided = np.abs(ref.trace_err[:k] - 1.0) > 1e-12
+    assert np.array_equal(res.trace[:k, 2].astype(int)[decided], ref.trace_acc[:k][decided])
+
+
+def test_energy_drift_matches_oracle(exe):
+    """Energy and momentum drift of the plain device path equal the oracle's
+    (identical states), and momentum is conserved to rounding."""
+    import ode_prototyping_amd as oh
+    from ode_prototyping_amd import scenarios
+    n, steps = 256, 30
+    x0, m = scenarios.plummer(n, seed=8)
+    ref = oracle_solve("DOP853", O.RHS_NBODY, x0, 0.0, 10.0, 1e-10, 1e-2, m,
+                       max_accepted=steps)
+    res = device_solve(exe, "DOP853", x0, 0.0, 10.0, 1e-10, oh.NBodyODE(1e-2, m, "exact"),
+                       max_accepted=steps)
+    e0 = sum(O.energy(x0, n, 1e-2, m))
+    e1 = sum(O.energy(res.x_final, n, 1e-2, m))
+    e1r = sum(O.energy(ref.x_final, n, 1e-2, m))
+    assert e1 == e1r
+    assert abs((e1 - e0) / e0) < 1e-9
+    p1 = O.momentum(res.x_final, n, m)
+    assert np.all(np.abs(p1) < 1e-14)

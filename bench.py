@@ -82,7 +82,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1, help="untimed solves")
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--accepted", type=int, default=200)
-    ap.add_argument("--variant", default="fast", choices=["fast", "exact", "mixed"])
+    ap.add_argument("--variant", default="exact", choices=["fast", "exact", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     args = ap.parse_args()
@@ -171,7 +171,7 @@ def main():
 
     if not args.no_extras:
         extras = {}
-        for var in ("exact", "mixed"):
+        for var in ("exact", "fast", "mixed"):
             if var == args.variant:
                 continue
             o2 = oh.NBodyODE(softening, m, var)

@@ -85,6 +85,9 @@ def main():
     ap.add_argument("--variant", default="exact", choices=["fast", "exact", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--profile-rhs", action="store_true",
+                    help="only run each RHS variant 3x and a 64 MiB calibration copy "
+                         "(the workload of the committed counter profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): this build "
@@ -92,6 +95,21 @@ def main():
         sys.exit(2)
 
     import ode_prototyping_amd as oh
+
+    if args.profile_rhs:
+        x0, m = oh.scenarios.plummer(args.n, seed=0)
+        exe = oh.HipExecutor(0)
+        xs, ks = oh.HipState(exe, data=x0), oh.HipState(exe, 6 * args.n)
+        for var in ("exact", "fast", "mixed"):
+            o = oh.NBodyODE(1e-2, m, var)
+            for _ in range(3):
+                o(exe, xs, ks)
+        a, b = oh.HipState(exe, 8 << 20), oh.HipState(exe, 8 << 20)
+        for _ in range(3):
+            b.assign(a)
+        exe.synchronize()
+        log("profile-rhs done")
+        return
 
     n, steps, warmup = args.n, args.steps, args.warmup
     softening, tol, tableau = 1e-2, 1e-10, "DOP853"
@@ -137,6 +155,15 @@ def main():
     log(f"rhs kernel ({args.variant}): {rhs_ms:.4f} ms/launch, {achieved:.2f} TFLOP/s; "
         f"RHS share of solve {rhs_total_ms / ms_per_step:.3f}")
 
+    # sustained FP64 peak of this device measured live (FMA probe)
+    sus_tflops, sus_mhz = exe.probe_fp64()
+    log(f"sustained FP64 FMA: {sus_tflops:.1f} TFLOP/s at {sus_mhz:.0f} MHz")
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"nbody_rhs_n{n}_traffic.json")
+    if os.path.exists(tfile):
+        tk = json.load(open(tfile))["kernels"].get(args.variant)
+        traffic = tk["hbm_bytes_per_launch"] if tk else None
+
     out = {
         "metric": "particle-steps/s at rtol 1e-10 + RHS evals/solve: chosen vs plain FP64, 1 GPU",
         "value": value,
@@ -163,8 +190,10 @@ def main():
         "roofline": {"bound": "valu", "pipe": "FP64 vector ALU (no dense contraction)",
                      "kernel": f"k_nbody_{args.variant}", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "kernel_ms": rhs_ms,
+                     "sustained_fp64_tflops": sus_tflops, "clock_mhz": sus_mhz,
+                     "frac_of_sustained": achieved / sus_tflops,
                      "algorithmic": f"{FLOP_PER_INTERACTION} FLOP x N(N-1) = "
                                     f"{FLOP_PER_INTERACTION * interactions:.4g} FLOP/launch"},
     }

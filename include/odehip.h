@@ -91,6 +91,11 @@ const char* odehip_version(void);
  * event, waits for it and writes the elapsed milliseconds. */
 int odehip_timer_start(odehip_context* ctx);
 int odehip_timer_stop(odehip_context* ctx, double* elapsed_ms);
+/* Diagnostic (no reference counterpart): sustained FP64 FMA throughput of
+ * the device and its shader clock under that load, from a dependent-free FMA
+ * kernel that fills every SIMD (s_memtime / s_memrealtime).  Writes TFLOP/s
+ * and MHz. */
+int odehip_probe_fp64(odehip_context* ctx, double* tflops, double* clock_mhz);
 
 /* ----------------------------------------------------------------- memory
  * Backing store of HipState (replaces src/state/CudaState.cuh:15-50). */

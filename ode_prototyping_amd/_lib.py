@@ -54,6 +54,7 @@ SIGNATURES = {
     "odehip_synchronize": (C.c_int, [_ctxp]),
     "odehip_timer_start": (C.c_int, [_ctxp]),
     "odehip_timer_stop": (C.c_int, [_ctxp, _dp]),
+    "odehip_probe_fp64": (C.c_int, [_ctxp, _dp, _dp]),
     "odehip_alloc": (C.c_int, [_ctxp, C.c_size_t, C.POINTER(C.c_void_p)]),
     "odehip_free": (C.c_int, [_ctxp, C.c_void_p]),
     "odehip_copy_h2d": (C.c_int, [_ctxp, C.c_void_p, _dp, C.c_size_t]),

@@ -68,6 +68,12 @@ class HipExecutor:
         self.check(lib().odehip_timer_stop(self._h, C.byref(ms)), "timer_stop")
         return ms.value
 
+    def probe_fp64(self):
+        """Sustained FP64 FMA TFLOP/s and shader clock (MHz) of the device."""
+        tf, mhz = C.c_double(), C.c_double()
+        self.check(lib().odehip_probe_fp64(self._h, C.byref(tf), C.byref(mhz)), "probe_fp64")
+        return tf.value, mhz.value
+
     def close(self):
         if getattr(self, "_h", None):
             lib().odehip_destroy(self._h)

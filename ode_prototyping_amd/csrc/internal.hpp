@@ -51,6 +51,8 @@ struct NBodyArgs {
 
 // ---- launchers (nbody_kernels.hip) ----
 hipError_t launch_nbody_rhs(const NBodyArgs& a, int variant, hipStream_t s);
+hipError_t launch_probe_fp64(double* out, long long* clk, int blocks, int iters,
+                             hipStream_t s);
 hipError_t launch_momentum_terms(const double* x, int n, const double* mass,
                                  double* out, hipStream_t s);
 hipError_t launch_nbody_energy(const double* x, int n, double eps2,

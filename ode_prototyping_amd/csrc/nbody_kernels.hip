@@ -72,11 +72,86 @@ __device__ inline void copy_velocities(const double* __restrict__ y,
 }
 
 // ------------------------------------------------------------------ exact --
-constexpr int kExactWaves = 8;
-constexpr int kExactJW = 4;  // partners per wave per tile
+constexpr int kExactWaves = 16;
+constexpr int kExactJW = 2;  // partners per wave per tile
 constexpr int kExactTJ = kExactWaves * kExactJW;
 
+// One pair term exactly as the reference writes it (NBodyODE.hpp:89-98):
+// IEEE correctly rounded sqrt and divisions via the compiler's expansions.
 template <bool kMass>
+__device__ inline void exact_term_ieee(double dx, double dy, double dz,
+                                       double mj, double softening_sq,
+                                       double& ax, double& ay, double& az) {
+  const double dist_sq = dx * dx + dy * dy + dz * dz + softening_sq;
+  const double dist = __builtin_sqrt(dist_sq);
+  const double denominator = dist * dist_sq;
+  ax = dx / denominator;
+  ay = dy / denominator;
+  az = dz / denominator;
+  if (kMass) {
+    ax *= mj;
+    ay *= mj;
+    az *= mj;
+  }
+}
+
+// The same term, same bits, cheaper: valid when the three displacement
+// magnitudes lie in [2^-300, 2^100] and softening_sq in [2^-200, 2^100]
+// (then dist_sq in [2^-200, 2^202] and the denominator in [2^-300, 2^304]).
+//  * sqrt: the compiler's correctly rounded sqrt is a rescale around a
+//    Goldschmidt refinement of v_rsq_f64 (y = rsq(s); g = s y; h = y/2;
+//    r = 1/2 - h g; g += g r; h += h r; twice g += (s - g^2) h); the rescale
+//    is the identity for s >= 2^-767, so the refinement alone gives the same
+//    bits.
+//  * division: the hardware sequence for n/d is d' = div_scale(d), r =
+//    rcp(d') refined by two Newton steps, q0 = n' r, rem = n' - d' q0,
+//    q = div_fmas(rem, r, q0), div_fixup.  With n and d in the range above
+//    div_scale leaves both operands unscaled (no VCC), div_fmas is a plain
+//    fma and div_fixup returns q unchanged -- so the refined reciprocal
+//    depends on d only and the three divisions by the same denominator share
+//    it: 1 rcp + 4 fma, then 3 ops per numerator instead of 11.
+template <bool kMass>
+__device__ inline void exact_term_shared_rcp(double dx, double dy, double dz,
+                                             double mj, double softening_sq,
+                                             double& ax, double& ay,
+                                             double& az) {
+  const double dist_sq = dx * dx + dy * dy + dz * dz + softening_sq;
+  const double y = __builtin_amdgcn_rsq(dist_sq);
+  double g = dist_sq * y;
+  double h = y * 0.5;
+  const double r0 = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r0, g);
+  double d = __builtin_fma(-g, g, dist_sq);
+  h = __builtin_fma(h, r0, h);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, dist_sq);
+  const double dist = __builtin_fma(d, h, g);
+  const double den = dist * dist_sq;
+  double r = __builtin_amdgcn_rcp(den);
+  double e = __builtin_fma(-den, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-den, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  const double qx = dx * r, qy = dy * r, qz = dz * r;
+  ax = __builtin_fma(__builtin_fma(-den, qx, dx), r, qx);
+  ay = __builtin_fma(__builtin_fma(-den, qy, dy), r, qy);
+  az = __builtin_fma(__builtin_fma(-den, qz, dz), r, qz);
+  if (kMass) {
+    ax *= mj;
+    ay *= mj;
+    az *= mj;
+  }
+}
+
+__device__ inline bool in_shared_rcp_range(double dx, double dy, double dz) {
+  const double lo = __builtin_fmin(__builtin_fmin(__builtin_fabs(dx), __builtin_fabs(dy)),
+                                   __builtin_fabs(dz));
+  const double hi = __builtin_fmax(__builtin_fmax(__builtin_fabs(dx), __builtin_fabs(dy)),
+                                   __builtin_fabs(dz));
+  return (lo >= 0x1p-300) & (hi <= 0x1p100);
+}
+
+template <bool kMass, bool kSharedRcp>
 __global__ __launch_bounds__(64 * kExactWaves) void k_nbody_exact(NBodyArgs a) {
   if (prologue_skip(a)) return;
   __shared__ double terms[2][kExactTJ][3][64];
@@ -96,46 +171,80 @@ __global__ __launch_bounds__(64 * kExactWaves) void k_nbody_exact(NBodyArgs a) {
   const double zi = valid_i ? y[3 * i + 2] : 0.0;
   double acc = 0.0;
   const int n_tiles = (n + kExactTJ - 1) / kExactTJ;
-  for (int t = 0; t <= n_tiles; ++t) {
-    if (t < n_tiles) {
-      const int jb = t * kExactTJ + wave * kExactJW;
+  // Partner coordinates of the next tile are loaded one tile ahead (scalar
+  // loads), so their latency hides behind the current tile's arithmetic.
+  double px[kExactJW], py[kExactJW], pz[kExactJW], pm[kExactJW];
+  auto load_partners = [&](int t) {
+    const int jb = t * kExactTJ + wave * kExactJW;
+    if (jb + kExactJW <= n) {
+      const uniform_ptr pp = yu + 3 * jb;
 #pragma unroll
       for (int jj = 0; jj < kExactJW; ++jj) {
-        const int j = jb + jj;
-        double ax = 0.0, ay = 0.0, az = 0.0;
-        if (j < n) {
-          const double dx = yu[3 * j] - xi;
-          const double dy = yu[3 * j + 1] - yi;
-          const double dz = yu[3 * j + 2] - zi;
-          const double dist_sq = dx * dx + dy * dy + dz * dz + softening_sq;
-          const double dist = __builtin_sqrt(dist_sq);
-          const double denominator = dist * dist_sq;
-          ax = dx / denominator;
-          ay = dy / denominator;
-          az = dz / denominator;
-          if (kMass) {
-            const double mj = massu[j];
-            ax *= mj;
-            ay *= mj;
-            az *= mj;
-          }
-        }
-        terms[t & 1][wave * kExactJW + jj][0][lane] = ax;
-        terms[t & 1][wave * kExactJW + jj][1][lane] = ay;
-        terms[t & 1][wave * kExactJW + jj][2][lane] = az;
+        px[jj] = pp[3 * jj];
+        py[jj] = pp[3 * jj + 1];
+        pz[jj] = pp[3 * jj + 2];
+        pm[jj] = kMass ? massu[jb + jj] : 1.0;
+      }
+    } else {
+#pragma unroll
+      for (int jj = 0; jj < kExactJW; ++jj) {
+        // partners past the end are clamped to a valid index: their terms
+        // are computed but never folded
+        const int j = min(jb + jj, n - 1);
+        px[jj] = yu[3 * j];
+        py[jj] = yu[3 * j + 1];
+        pz[jj] = yu[3 * j + 2];
+        pm[jj] = kMass ? massu[j] : 1.0;
       }
     }
+  };
+  load_partners(0);
+  for (int t = 0; t <= n_tiles; ++t) {
+    // Fold the previous tile first: the in-order adds of the three folding
+    // waves then overlap the other waves' arithmetic on the same SIMD.
     if (t > 0 && wave < 3) {
       const int base = (t - 1) * kExactTJ;
       const int cnt = min(kExactTJ, n - base);
       const double* src = &terms[(t - 1) & 1][0][wave][lane];
-      if (base + cnt <= i0 || base >= i0 + 64) {
-        for (int m = 0; m < cnt; ++m) acc += src[m * 192];
+      if (cnt == kExactTJ && (base + kExactTJ <= i0 || base >= i0 + 64)) {
+#pragma unroll
+        for (int m = 0; m < kExactTJ; ++m) acc += src[m * 192];
       } else {
         for (int m = 0; m < cnt; ++m) {
           const double v = src[m * 192];
           if (base + m != i) acc += v;
         }
+      }
+    }
+    if (t < n_tiles) {
+      double dx[kExactJW], dy[kExactJW], dz[kExactJW], mj[kExactJW];
+      bool ok = true;
+#pragma unroll
+      for (int jj = 0; jj < kExactJW; ++jj) {
+        dx[jj] = px[jj] - xi;
+        dy[jj] = py[jj] - yi;
+        dz[jj] = pz[jj] - zi;
+        mj[jj] = pm[jj];
+        if (kSharedRcp) ok = ok & in_shared_rcp_range(dx[jj], dy[jj], dz[jj]);
+      }
+      if (t + 1 < n_tiles) load_partners(t + 1);
+      double ax[kExactJW], ay[kExactJW], az[kExactJW];
+      if (kSharedRcp && __all(ok)) {
+#pragma unroll
+        for (int jj = 0; jj < kExactJW; ++jj)
+          exact_term_shared_rcp<kMass>(dx[jj], dy[jj], dz[jj], mj[jj], softening_sq,
+                                       ax[jj], ay[jj], az[jj]);
+      } else {
+#pragma unroll
+        for (int jj = 0; jj < kExactJW; ++jj)
+          exact_term_ieee<kMass>(dx[jj], dy[jj], dz[jj], mj[jj], softening_sq, ax[jj],
+                                 ay[jj], az[jj]);
+      }
+#pragma unroll
+      for (int jj = 0; jj < kExactJW; ++jj) {
+        terms[t & 1][wave * kExactJW + jj][0][lane] = ax[jj];
+        terms[t & 1][wave * kExactJW + jj][1][lane] = ay[jj];
+        terms[t & 1][wave * kExactJW + jj][2][lane] = az[jj];
       }
     }
     __syncthreads();
@@ -189,14 +298,16 @@ __global__ __launch_bounds__(64 * kFastWaves) void k_nbody_fast(NBodyArgs a) {
   const int jbeg = min(n, wave * seg);
   const int jend = min(n, jbeg + seg);
   double ax = 0.0, ay = 0.0, az = 0.0;
+  // 8 independent partner chains per step (wave-uniform scalar loads)
+  constexpr int U = 8;
   int j = jbeg;
-  for (; j + 4 <= jend; j += 4) {
+  for (; j + U <= jend; j += U) {
+    const uniform_ptr pp = yu + 3 * j;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int jj = j + u;
-      const double mj = kMass ? mass[jj] : 1.0;
-      fast_term<kMass, kGuard>(yu[3 * jj], yu[3 * jj + 1], yu[3 * jj + 2], mj, jj,
-                               i, xi, yi, zi, eps2, ax, ay, az);
+    for (int u = 0; u < U; ++u) {
+      const double mj = kMass ? mass[j + u] : 1.0;
+      fast_term<kMass, kGuard>(pp[3 * u], pp[3 * u + 1], pp[3 * u + 2], mj, j + u, i, xi,
+                               yi, zi, eps2, ax, ay, az);
     }
   }
   for (; j < jend; ++j) {
@@ -264,13 +375,13 @@ __global__ __launch_bounds__(64 * kMixedWaves) void k_nbody_mixed(NBodyArgs a) {
   while (j < jend) {
     const int jstop = min(jend, j + kMixedFlush);
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    for (; j + 4 <= jstop; j += 4) {
+    for (; j + 8 <= jstop; j += 8) {
+      const uniform_ptr pp = yu + 3 * j;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int jj = j + u;
-        const float mj = kMass ? static_cast<float>(mass[jj]) : 1.0f;
-        mixed_term<kMass, kGuard>(yu[3 * jj], yu[3 * jj + 1], yu[3 * jj + 2], mj,
-                                  jj, i, xi, yi, zi, eps2f, fx, fy, fz);
+      for (int u = 0; u < 8; ++u) {
+        const float mj = kMass ? static_cast<float>(mass[j + u]) : 1.0f;
+        mixed_term<kMass, kGuard>(pp[3 * u], pp[3 * u + 1], pp[3 * u + 2], mj, j + u, i,
+                                  xi, yi, zi, eps2f, fx, fy, fz);
       }
     }
     for (; j < jstop; ++j) {
@@ -332,7 +443,39 @@ __global__ void k_momentum_terms(const double* __restrict__ y, int n,
   for (int c = 0; c < 3; ++c) out[static_cast<long>(c) * n + i] = mi * v[3 * i + c];
 }
 
+// Eight independent FMA chains per lane; lane 0 of block 0 records the
+// shader-clock and 100 MHz real-time counters around the loop.
+__global__ __launch_bounds__(1024) void k_probe_fp64(double* out, long long* clk,
+                                                     int iters) {
+  double a[8];
+  for (int u = 0; u < 8; ++u) a[u] = threadIdx.x * 1e-9 + u;
+  const double b = 0.999999, c = 1e-7;
+  long long t0 = 0, r0 = 0;
+  const bool rec = threadIdx.x == 0 && blockIdx.x == 0;
+  if (rec) {
+    t0 = __builtin_amdgcn_s_memtime();
+    r0 = __builtin_amdgcn_s_memrealtime();
+  }
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] = __builtin_fma(a[u], b, c);
+  }
+  if (rec) {
+    clk[0] = __builtin_amdgcn_s_memtime() - t0;
+    clk[1] = __builtin_amdgcn_s_memrealtime() - r0;
+  }
+  double s = 0.0;
+  for (int u = 0; u < 8; ++u) s += a[u];
+  out[static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x] = s;
+}
+
 }  // namespace
+
+hipError_t launch_probe_fp64(double* out, long long* clk, int blocks, int iters,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_probe_fp64, dim3(blocks), dim3(1024), 0, s, out, clk, iters);
+  return hipGetLastError();
+}
 
 hipError_t launch_momentum_terms(const double* x, int n, const double* mass,
                                  double* out, hipStream_t s) {
@@ -348,12 +491,22 @@ hipError_t launch_nbody_rhs(const NBodyArgs& a, int variant, hipStream_t s) {
   const bool mass = a.mass != nullptr;
   const bool guard = !(a.eps2 >= 1e-150);
   switch (variant) {
-    case ODEHIP_NBODY_EXACT:
-      if (mass)
-        hipLaunchKernelGGL(k_nbody_exact<true>, dim3(blocks), dim3(64 * kExactWaves), 0, s, a);
-      else
-        hipLaunchKernelGGL(k_nbody_exact<false>, dim3(blocks), dim3(64 * kExactWaves), 0, s, a);
+    case ODEHIP_NBODY_EXACT: {
+      // the shared-reciprocal form needs softening^2 in [2^-200, 2^100]
+      const bool shared = a.eps2 >= 0x1p-200 && a.eps2 <= 0x1p100;
+      if (mass) {
+        if (shared)
+          hipLaunchKernelGGL((k_nbody_exact<true, true>), dim3(blocks), dim3(64 * kExactWaves), 0, s, a);
+        else
+          hipLaunchKernelGGL((k_nbody_exact<true, false>), dim3(blocks), dim3(64 * kExactWaves), 0, s, a);
+      } else {
+        if (shared)
+          hipLaunchKernelGGL((k_nbody_exact<false, true>), dim3(blocks), dim3(64 * kExactWaves), 0, s, a);
+        else
+          hipLaunchKernelGGL((k_nbody_exact<false, false>), dim3(blocks), dim3(64 * kExactWaves), 0, s, a);
+      }
       break;
+    }
     case ODEHIP_NBODY_FAST:
       if (mass) {
         if (guard)

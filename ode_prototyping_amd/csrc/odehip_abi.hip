@@ -284,6 +284,35 @@ int odehip_timer_stop(odehip_context* ctx, double* elapsed_ms) {
   return ODEHIP_OK;
 }
 
+int odehip_probe_fp64(odehip_context* ctx, double* tflops, double* clock_mhz) {
+  if (!ctx || !tflops || !clock_mhz) return ODEHIP_ERR_INVALID;
+  const int blocks = 1024, threads = 1024, iters = 20000;
+  const size_t need = static_cast<size_t>(blocks) * threads + 2;
+  int rc = ensure_scratch(ctx, need);
+  if (rc) return rc;
+  long long* clk = reinterpret_cast<long long*>(ctx->scratch + static_cast<size_t>(blocks) * threads);
+  double best = 0.0, mhz = 0.0;
+  for (int rep = 0; rep < 3; ++rep) {
+    HIP_TRY(ctx, hipEventRecord(ctx->t_start, ctx->stream));
+    HIP_TRY(ctx, launch_probe_fp64(ctx->scratch, clk, blocks, iters, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->t_stop, ctx->stream));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->t_stop));
+    float ms = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->t_start, ctx->t_stop));
+    long long h[2];
+    HIP_TRY(ctx, hipMemcpy(h, clk, sizeof h, hipMemcpyDeviceToHost));
+    const double fl = 2.0 * blocks * threads * static_cast<double>(iters) * 8.0;
+    const double tf = fl / (ms * 1e-3) / 1e12;
+    if (tf > best) {
+      best = tf;
+      mhz = h[1] > 0 ? static_cast<double>(h[0]) / (static_cast<double>(h[1]) / 100.0) : 0.0;
+    }
+  }
+  *tflops = best;
+  *clock_mhz = mhz;
+  return ODEHIP_OK;
+}
+
 // ------------------------------------------------------------------ memory --
 int odehip_alloc(odehip_context* ctx, size_t n, double** out) {
   if (!ctx || !out) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");

@@ -52,6 +52,23 @@ __device__ inline double fold_partials(const double* p, int n_groups) {
   return r;
 }
 
+// The same fold, called by every thread of a 64-thread block: the partials
+// are first staged through LDS by all lanes (coalesced), then lane 0 adds
+// them in order from LDS.  Returns the sum in lane 0.
+constexpr int kFoldLds = 4096;
+__device__ inline double fold_partials_block(const double* p, int n_groups) {
+  __shared__ double buf[kFoldLds];
+  if (n_groups > kFoldLds) return threadIdx.x == 0 ? fold_partials(p, n_groups) : 0.0;
+  for (int g = threadIdx.x; g < n_groups; g += blockDim.x) buf[g] = p[g];
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll 8
+    for (int g = 0; g < n_groups; ++g) r = r + buf[g];
+  }
+  return r;
+}
+
 // ----------------------------------------------------------- simple RHS ----
 __global__ void k_exp_rhs(const double* y, const double* xb0, const double* xb1,
                           const SolverState* st, double* k, const double* kfsal,
@@ -177,10 +194,10 @@ __global__ __launch_bounds__(kBlock) void k_final_dev(
 // device, plus the termination tests and counters.
 __global__ void k_controller(SolverState* st, const double* partials,
                              size_t n_var, double* times, double* trace) {
-  if (threadIdx.x != 0) return;
   if (st->done) return;
+  const double sum = fold_partials_block(partials, st->n_groups);
+  if (threadIdx.x != 0) return;
   const Tableau& T = c_tab[st->tableau];
-  const double sum = fold_partials(partials, st->n_groups);
   const double scaled_error =
       std::sqrt(sum / static_cast<double>(static_cast<long>(n_var)));
   const long attempt = st->attempts;
@@ -302,24 +319,27 @@ __global__ __launch_bounds__(kBlock) void k_norm_partials(
 }
 
 __global__ void k_fold_sum(const double* partials, int n_groups, double* out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  *out = fold_partials(partials, n_groups);
+  const double s = fold_partials_block(partials, n_groups);
+  if (threadIdx.x == 0) *out = s;
 }
 
 __global__ void k_fold_norm(const double* partials, int n_groups, size_t n_var,
                             double* out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double s = fold_partials(partials, n_groups);
+  const double s = fold_partials_block(partials, n_groups);
+  if (threadIdx.x != 0) return;
   *out = std::sqrt(s / static_cast<double>(static_cast<long>(n_var)));
 }
 
 // estimate_initial_step (RKEmbeddedParallel.hpp:183-185): d0, d1, dt0.
 __global__ void k_init_step_a(SolverState* st, const double* pa,
                               const double* pb, size_t n_var) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double sa = fold_partials_block(pa, st->n_groups);
+  __syncthreads();
+  const double sb = fold_partials_block(pb, st->n_groups);
+  if (threadIdx.x != 0) return;
   const double nv = static_cast<double>(static_cast<long>(n_var));
-  const double d0 = std::sqrt(fold_partials(pa, st->n_groups) / nv);
-  const double d1 = std::sqrt(fold_partials(pb, st->n_groups) / nv);
+  const double d0 = std::sqrt(sa / nv);
+  const double d1 = std::sqrt(sb / nv);
   const double dt0 = (d0 < 1.0e-5 or d1 < 1.0e-5) ? 1.0e-6 : 0.01 * (d0 / d1);
   st->dt0 = dt0;
   st->d1 = d1;
@@ -334,12 +354,13 @@ __global__ void k_euler(const SolverState* st, double* x1, const double* x0,
 
 // (:193-199): d2, dt1, the initial step.
 __global__ void k_init_step_b(SolverState* st, const double* pa, size_t n_var) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double sa = fold_partials_block(pa, st->n_groups);
+  if (threadIdx.x != 0) return;
   const Tableau& T = c_tab[st->tableau];
   const double nv = static_cast<double>(static_cast<long>(n_var));
   const double dt0 = st->dt0;
   const double d1 = st->d1;
-  const double d2 = std::sqrt(fold_partials(pa, st->n_groups) / nv) / dt0;
+  const double d2 = std::sqrt(sa / nv) / dt0;
   const int p = T.p;
   const double m = (d1 < d2) ? d2 : d1;
   const double hi = 1.0e-6, lo = dt0 * 1.0e-3;

@@ -190,3 +190,26 @@ def test_energy_and_momentum(exe):
                                                out.ctypes.data_as(C.POINTER(C.c_double))), "p")
     p = O.momentum(x, n, m)
     assert np.all(np.abs(out - p) <= 1e-17)
+
+
+@pytest.mark.parametrize("eps", [1e-3, 1e-70, 0.0])
+def test_exact_rhs_extreme_inputs_bit_identical(exe, eps):
+    """Inputs that leave the exact kernel's shared-reciprocal range: zero
+    displacement components (coplanar particles), separations below 2^-300
+    and above 2^100, and softenings outside [2^-100, 2^50].  Every term must
+    still match the IEEE expression bit for bit (NaN where the reference
+    also produces one)."""
+    rng = np.random.default_rng(42)
+    n = 200
+    pos = rng.uniform(-1, 1, (n, 3))
+    pos[:40, 2] = 0.0                              # coplanar block: dz == 0
+    pos[40:60] = pos[40] + rng.uniform(-1, 1, (20, 3)) * 1e-95   # tiny separations
+    pos[60:80] *= 1e35                             # huge separations
+    pos[80:90, 0] = pos[0, 0]                      # shared x coordinate
+    x = np.concatenate([pos.ravel(), rng.normal(size=3 * n)])
+    m = rng.uniform(0.5, 1.5, n) / n
+    got = gpu_rhs(exe, x, eps, m, "exact")
+    want = O.rhs(O.RHS_NBODY, x, eps, m)
+    finite = np.isfinite(want)
+    assert np.array_equal(np.isfinite(got), finite)
+    assert np.array_equal(got[finite].view(np.uint64), want[finite].view(np.uint64))

@@ -150,6 +150,18 @@ int odehip_rk_update_error_target(odehip_context* ctx, double* sc,
  * (as CudaExecutor::transform_reduce does, CudaExecutor.cuh:117-119). */
 int odehip_rk_norm(odehip_context* ctx, const double* v, const double* sc,
                    size_t n_var, double* host_result);
+/* detail::compute_error_target_kernel (RKEmbeddedParallel.hpp:151-157):
+ * sc = |x0| * rtol + atol. */
+int odehip_rk_compute_error_target(odehip_context* ctx, double* sc,
+                                   const double* x0, const double* atol,
+                                   const double* rtol, size_t n_var);
+/* detail::euler_step_kernel (RKEmbeddedParallel.hpp:159-163):
+ * x1 = x0 + f0 * dt0. */
+int odehip_rk_euler_step(odehip_context* ctx, double* x1, const double* x0,
+                         const double* f0, double dt0, size_t n_var);
+/* detail::difference_kernel (RKEmbeddedParallel.hpp:165-168): df -= f0. */
+int odehip_rk_difference(odehip_context* ctx, double* df, const double* f0,
+                         size_t n_var);
 /* detail::estimate_initial_step (RKEmbeddedParallel.hpp:170-200) on the
  * device; writes the step to *host_dt. */
 int odehip_rk_initial_step(odehip_context* ctx, int tableau, int rhs_kind,

@@ -74,6 +74,11 @@ SIGNATURES = {
     "odehip_rk_update_error_target": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_void_p,
                                                 C.c_void_p, C.c_void_p, C.c_size_t]),
     "odehip_rk_norm": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_size_t, _dp]),
+    "odehip_rk_compute_error_target": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                 C.c_void_p, C.c_size_t]),
+    "odehip_rk_euler_step": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                       C.c_size_t]),
+    "odehip_rk_difference": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_size_t]),
     "odehip_rk_initial_step": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_size_t, C.c_double, C.c_void_p,
                                          C.c_int, _dp]),

@@ -377,16 +377,24 @@ def integrate_host_driven(tableau, x0, t0, tf, atol, rtol, ode, output, exe, *,
     ks = HipState(exe, S * n)
     rhs_evals = 0
 
-    # estimate_initial_step (:170-200); its kernels via executor-level calls
-    sc = HipState(exe, n)
-    dt0_out = C.c_double()
-    m = ode.masses_device(exe) if isinstance(ode, NBodyODE) else None
-    exe.check(lib().odehip_rk_initial_step(
-        exe.handle, TABLEAUS[tableau], ode.rhs_kind, x0s.ptr, at.ptr, rt.ptr, n,
-        getattr(ode, "softening", 0.0), m.ptr if m else None,
-        getattr(ode, "_variant_id", 0), C.byref(dt0_out)), "rk_initial_step")
+    # estimate_initial_step (:170-200), its kernels as executor-level calls
+    L = lib()
+    sc, f0, x1, df = (HipState(exe, n) for _ in range(4))
+    exe.check(L.odehip_rk_compute_error_target(exe.handle, sc.ptr, x0s.ptr, at.ptr, rt.ptr, n),
+              "compute_error_target")
+    ode(exe, x0s, f0)
+    d0 = exe.rk_norm(x0s, sc)
+    d1 = exe.rk_norm(f0, sc)
+    dt0 = 1.0e-6 if (d0 < 1.0e-5 or d1 < 1.0e-5) else 0.01 * (d0 / d1)
+    exe.check(L.odehip_rk_euler_step(exe.handle, x1.ptr, x0s.ptr, f0.ptr, dt0, n), "euler_step")
+    ode(exe, x1, df)
+    exe.check(L.odehip_rk_difference(exe.handle, df.ptr, f0.ptr, n), "difference")
+    d2 = exe.rk_norm(df, sc) / dt0
+    dmax = d2 if d1 < d2 else d1
+    dt1 = (max(1.0e-6, dt0 * 1.0e-3) if dmax <= 1.0e-15
+           else _host_pow(0.01 / dmax, 1.0 / (1.0 + p)))
+    dt = min(100.0 * dt0, dt1)
     rhs_evals += 2
-    dt = dt0_out.value
     t = t0
     x = x0s.copy()
     temp = HipState(exe, n)

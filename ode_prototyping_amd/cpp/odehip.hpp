@@ -1,0 +1,377 @@
+// C++20 host-side mirror of the reference's integration interfaces over the
+// libodehip C ABI (include/odehip.h).  A reference user switches the
+// execution policy and state container and keeps the rest:
+//
+//   reference (CUDA):                         this library (HIP, gfx950):
+//   CudaExecutor exe;                         odehip::HipExecutor exe;
+//   CudaState<double, N> x0{data};            odehip::HipState<double, N> x0{data};
+//   RKEmbeddedParallel<BTDOPRI5>::integrate(  odehip::RKEmbeddedParallel<odehip::BTDOPRI5>::integrate(
+//       x0, t0, tf, atol, rtol,                   x0, t0, tf, atol, rtol,
+//       NBodyODE<double, N>{eps}, out, exe);      odehip::NBodyODE<double, N>{eps}, out, exe);
+//
+// Mirrored interfaces (reference file:line):
+//   HipState<T, N>         src/state/CudaState.cuh:10-64 (ODEState concept,
+//                          src/state/ODEState.hpp:25-43)
+//   HipExecutor            src/par_exec/CudaExecutor.cuh:80-126
+//   NBodyODE / VanDerPolODE / ExponentialODE   src/ode/*.hpp
+//   RawOutput              src/output/RawOutput.hpp:8-27
+//   RKEmbeddedParallel<BT>::integrate   src/integrator/RKEmbeddedParallel/
+//                          RKEmbeddedParallel.hpp:13-81 (host-driven: every
+//                          reference kernel launch is one C-ABI call, the
+//                          controller runs on the host with std::pow, exactly
+//                          the CUDA executor's control flow);
+//                          integrate_on_device: the same solve with the attempt
+//                          loop and controller resident on the GPU.
+//   fill                   src/state/StateUtils.hpp:5-16
+// Errors: the reference prints CUDA errors and continues
+// (src/cuda_util/CudaErrorCheck.cuh:8-16); here every failing C-ABI call
+// throws odehip::Error with the library's message.
+#pragma once
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstddef>
+#include <span>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "odehip.h"
+
+namespace odehip {
+
+struct Error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// HipExecutor{} wraps the process-wide context (one HIP stream) that HipState
+// also allocates and copies on, as CudaExecutor{} and CudaState share the CUDA
+// runtime's implicit context and legacy stream (CudaExecutor.cuh:80-126,
+// CudaState.cuh:15-50): kernels and state copies are ordered.  HipExecutor(d)
+// creates a separate context (own stream) on device d for callers that order
+// their own work.
+class HipExecutor {
+ public:
+  HipExecutor() : ctx_{shared_context()}, owned_{false} {}
+  explicit HipExecutor(int device) : owned_{true} {
+    if (odehip_create(device, &ctx_) != ODEHIP_OK)
+      throw Error("odehip_create failed: no usable HIP device");
+  }
+  HipExecutor(const HipExecutor&) = delete;
+  HipExecutor& operator=(const HipExecutor&) = delete;
+  ~HipExecutor() {
+    if (owned_) odehip_destroy(ctx_);
+  }
+
+  odehip_context* context() const { return ctx_; }
+  void check(int rc, const char* what) const {
+    if (rc != ODEHIP_OK)
+      throw Error(std::string(what) + ": " + odehip_last_error(ctx_));
+  }
+  void synchronize() const { check(odehip_synchronize(ctx_), "synchronize"); }
+
+  static odehip_context* shared_context() {
+    static odehip_context* ctx = [] {
+      odehip_context* c = nullptr;
+      if (odehip_create(0, &c) != ODEHIP_OK)
+        throw Error("odehip_create failed: no usable HIP device");
+      return c;
+    }();
+    return ctx;
+  }
+
+ private:
+  odehip_context* ctx_ = nullptr;
+  bool owned_ = false;
+};
+
+// Owning device buffer of N values of T (T = double); CudaState semantics:
+// zero-initialised default, construction from a host span, deep copy on copy
+// construction/assignment, copy_to_span to the host.
+template <typename T, std::size_t N>
+class HipState {
+  static_assert(std::is_same_v<T, double>, "libodehip states are FP64");
+
+ public:
+  static constexpr auto size() { return N; }
+
+  HipState() { alloc(); }
+  explicit HipState(std::span<const T, N> host) {
+    alloc();
+    exe().check(odehip_copy_h2d(exe().context(), ptr_, host.data(), N), "copy_h2d");
+  }
+  HipState(const HipState& other) {
+    alloc();
+    exe().check(odehip_copy_d2d(exe().context(), ptr_, other.ptr_, N), "copy_d2d");
+  }
+  HipState(HipState&& other) noexcept : ptr_{other.ptr_} { other.ptr_ = nullptr; }
+  HipState& operator=(const HipState& other) {
+    if (this != &other)
+      exe().check(odehip_copy_d2d(exe().context(), ptr_, other.ptr_, N), "copy_d2d");
+    return *this;
+  }
+  HipState& operator=(HipState&& other) noexcept {
+    std::swap(ptr_, other.ptr_);
+    return *this;
+  }
+  ~HipState() {
+    if (ptr_) odehip_free(exe().context(), ptr_);
+  }
+
+  T* data() { return ptr_; }
+  const T* data() const { return ptr_; }
+
+  void copy_to_span(std::span<T, N> dest) const {
+    exe().check(odehip_copy_d2h(exe().context(), dest.data(), ptr_, N), "copy_d2h");
+  }
+  friend auto span(HipState& s) { return std::span<T, N>{s.data(), N}; }
+  friend auto span(const HipState& s) { return std::span<const T, N>{s.data(), N}; }
+
+ private:
+  struct Ctx {
+    odehip_context* context() const { return HipExecutor::shared_context(); }
+    void check(int rc, const char* what) const {
+      if (rc != ODEHIP_OK)
+        throw Error(std::string(what) + ": " + odehip_last_error(context()));
+    }
+  };
+  static Ctx exe() { return {}; }
+  void alloc() {
+    exe().check(odehip_alloc(exe().context(), N, &ptr_), "alloc");
+  }
+  T* ptr_ = nullptr;
+};
+
+template <typename R>
+HipState(R&&) -> HipState<typename std::remove_reference_t<R>::value_type,
+                          std::tuple_size_v<std::remove_reference_t<R>>>;
+
+// StateUtils fill (src/state/StateUtils.hpp:13-16).
+template <typename T, std::size_t N>
+void fill(HipExecutor& exe, HipState<T, N>& s, T value) {
+  exe.check(odehip_fill(exe.context(), s.data(), value, N), "fill");
+}
+
+// ------------------------------------------------------------------ tableaux
+// Tag types carrying the library's tableau id and (p, pt, S), named as the
+// reference's butcher_tableaus/*.hpp structs.
+template <int Id, int P, int PT, int S>
+struct TableauTag {
+  static constexpr int id = Id;
+  static constexpr int p = P;
+  static constexpr int pt = PT;
+  static constexpr int n_stages = S;
+};
+using BTHE21 = TableauTag<ODEHIP_TABLEAU_HE21, 2, 1, 2>;
+using BTRKF45 = TableauTag<ODEHIP_TABLEAU_RKF45, 4, 5, 6>;
+using BTDOPRI5 = TableauTag<ODEHIP_TABLEAU_DOPRI5, 5, 4, 7>;
+using BTDVERK = TableauTag<ODEHIP_TABLEAU_DVERK, 6, 5, 8>;
+using BTRKF78 = TableauTag<ODEHIP_TABLEAU_RKF78, 7, 8, 13>;
+using BTDOP853 = TableauTag<ODEHIP_TABLEAU_DOP853, 8, 5, 12>;
+
+// ---------------------------------------------------------------------- ODEs
+enum class NBodyArithmetic { exact = ODEHIP_NBODY_EXACT, fast = ODEHIP_NBODY_FAST,
+                             mixed = ODEHIP_NBODY_MIXED };
+
+// src/ode/NBodyODE.hpp: softening as the reference's aggregate member; masses
+// (device pointer, N entries) and arithmetic are extensions.
+template <typename ValueType, std::size_t n_var>
+struct NBodyODE {
+  static constexpr auto n_particles = n_var / 6;
+  double softening{};
+  const double* masses = nullptr;
+  NBodyArithmetic arithmetic = NBodyArithmetic::exact;
+
+  void operator()(HipExecutor& exe, std::span<const ValueType, n_var> x,
+                  std::span<ValueType, n_var> dxdt) const {
+    exe.check(odehip_nbody_rhs(exe.context(), x.data(), dxdt.data(), n_particles,
+                               softening, masses, static_cast<int>(arithmetic)),
+              "nbody_rhs");
+  }
+  // NBodyODE::acc (NBodyODE.hpp:41-46)
+  void acc(HipExecutor& exe, std::span<const ValueType, n_var> pos_vel,
+           std::span<ValueType, n_var / 2> a) const {
+    exe.check(odehip_nbody_acc(exe.context(), pos_vel.data(), a.data(), n_particles,
+                               softening, masses, static_cast<int>(arithmetic)),
+              "nbody_acc");
+  }
+  odehip_ode describe() const {
+    return {ODEHIP_RHS_NBODY, static_cast<int>(arithmetic), n_var, softening, masses};
+  }
+};
+
+template <typename ValueType>
+struct VanDerPolODE {
+  static constexpr auto n_var = 2;
+  void operator()(HipExecutor& exe, std::span<const ValueType, 2> x,
+                  std::span<ValueType, 2> dxdt) const {
+    exe.check(odehip_ode_rhs(exe.context(), ODEHIP_RHS_VANDERPOL, x.data(), dxdt.data(), 2),
+              "vdp_rhs");
+  }
+  odehip_ode describe() const { return {ODEHIP_RHS_VANDERPOL, 0, 2, 0.0, nullptr}; }
+};
+
+template <typename ValueType, std::size_t n_var>
+struct ExponentialODE {
+  void operator()(HipExecutor& exe, std::span<const ValueType, n_var> x,
+                  std::span<ValueType, n_var> dxdt) const {
+    exe.check(odehip_ode_rhs(exe.context(), ODEHIP_RHS_EXPONENTIAL, x.data(), dxdt.data(),
+                             n_var),
+              "exp_rhs");
+  }
+  odehip_ode describe() const { return {ODEHIP_RHS_EXPONENTIAL, 0, n_var, 0.0, nullptr}; }
+};
+
+// -------------------------------------------------------------------- output
+// src/output/RawOutput.hpp:8-27 (host copies of every saved state).
+template <std::size_t N>
+class RawOutput {
+ public:
+  template <typename StateType>
+  void save_state(double t, const StateType& x) {
+    times_.push_back(t);
+    states_.emplace_back();
+    x.copy_to_span(std::span<double, N>{states_.back()});
+  }
+  const auto& times() const { return times_; }
+  const auto& states() const { return states_; }
+
+ private:
+  std::vector<double> times_{};
+  std::vector<std::array<double, N>> states_{};
+};
+
+// ---------------------------------------------------------------- integrator
+struct DeviceSolveStats {
+  long accepted = 0, rejected = 0, rhs_evals = 0;
+  int status = 0;
+  double t = 0.0;
+};
+
+template <typename BT>
+struct RKEmbeddedParallel {
+  // RKEmbeddedParallel.hpp:13-81, driven from the host: one C-ABI call per
+  // reference kernel launch; the error norm is read back each attempt
+  // (as CudaExecutor::transform_reduce does) and the controller runs here.
+  template <typename ODE, typename Output, std::size_t N>
+  static void integrate(HipState<double, N> x0, double t0, double tf,
+                        HipState<double, N> atol, HipState<double, N> rtol, ODE ode,
+                        Output& output, HipExecutor& exe) {
+    constexpr auto n_var = N;
+    constexpr auto max_step_scale = 6.0;
+    constexpr auto min_step_scale = 0.33;
+    constexpr auto q = BT::p < BT::pt ? BT::p : BT::pt;
+    static const auto safety_factor = std::pow(0.38, (1.0 / (1.0 + q)));
+    HipState<double, BT::n_stages * n_var> ks{};
+    auto* ctx = exe.context();
+    double dt = estimate_initial_step(exe, x0, atol, rtol, ode);
+    double t = t0;
+    auto x = x0;
+    HipState<double, N> temp_state{}, error_estimate{}, error_target{};
+    output.save_state(t, x);
+    while (t < tf) {
+      ode(exe, span(std::as_const(x0)), std::span<double, n_var>{ks.data(), n_var});
+      for (int stage = 1; stage < BT::n_stages; ++stage) {
+        exe.check(odehip_rk_stage(ctx, BT::id, stage, dt, temp_state.data(), ks.data(),
+                                  x0.data(), n_var),
+                  "rk_stage");
+        ode(exe, span(std::as_const(temp_state)),
+            std::span<double, n_var>{ks.data() + stage * n_var, n_var});
+      }
+      exe.check(odehip_rk_update_state_and_error(ctx, BT::id, dt, x.data(),
+                                                 error_estimate.data(), ks.data(),
+                                                 x0.data(), n_var),
+                "rk_update_state_and_error");
+      exe.check(odehip_rk_update_error_target(ctx, error_target.data(), atol.data(),
+                                              rtol.data(), x.data(), x0.data(), n_var),
+                "rk_update_error_target");
+      double scaled_error = 0.0;
+      exe.check(odehip_rk_norm(ctx, error_estimate.data(), error_target.data(), n_var,
+                               &scaled_error),
+                "rk_norm");
+      if (!std::isfinite(scaled_error))
+        throw Error("non-finite error norm (the reference would loop forever)");
+      if (scaled_error <= 1.0) {
+        t += dt;
+        x0 = x;
+        output.save_state(t, x);
+      }
+      auto dtnew = dt * safety_factor / std::pow(scaled_error, 1.0 / (1.0 + q));
+      if (std::abs(dtnew) > max_step_scale * std::abs(dt)) {
+        dt *= max_step_scale;
+      } else if (std::abs(dtnew) < min_step_scale * std::abs(dt)) {
+        dt *= min_step_scale;
+      } else if (dtnew / (tf - t0) < 1.0e-12) {
+        dt = (tf - t0) * 1.0e-12;
+      } else {
+        dt = dtnew;
+      }
+      if (t + dt > tf) dt = tf - t;
+    }
+  }
+
+  // The same solve with the attempt loop, error norm, accept/reject and
+  // controller on the device (odehip_rk_solve); the final state replaces x,
+  // and the accepted times are returned.  max_accepted = 0: run to tf.
+  template <typename ODE, std::size_t N>
+  static DeviceSolveStats integrate_on_device(HipState<double, N>& x, double t0, double tf,
+                                              const HipState<double, N>& atol,
+                                              const HipState<double, N>& rtol,
+                                              const ODE& ode, HipExecutor& exe,
+                                              std::vector<double>* times = nullptr,
+                                              long max_accepted = 0, bool fsal = false) {
+    odehip_ode o = ode.describe();
+    odehip_solve_options opt{};
+    opt.t0 = t0;
+    opt.tf = tf;
+    opt.atol = atol.data();
+    opt.rtol = rtol.data();
+    opt.max_accepted = max_accepted;
+    opt.fsal = fsal ? 1 : 0;
+    if (times) {
+      times->assign(max_accepted > 0 ? max_accepted + 1 : (1 << 20), 0.0);
+      opt.times = times->data();
+      opt.times_cap = static_cast<long>(times->size());
+    }
+    odehip_solve_stats st{};
+    exe.check(odehip_rk_solve(exe.context(), BT::id, &o, &opt, x.data(), &st), "rk_solve");
+    if (times) times->resize(st.n_times);
+    return {st.accepted, st.rejected, st.rhs_evals, st.status, st.t};
+  }
+
+ private:
+  // detail::estimate_initial_step (RKEmbeddedParallel.hpp:170-200).
+  template <typename ODE, std::size_t N>
+  static double estimate_initial_step(HipExecutor& exe, const HipState<double, N>& x0,
+                                      const HipState<double, N>& atol,
+                                      const HipState<double, N>& rtol, const ODE& ode) {
+    auto* ctx = exe.context();
+    HipState<double, N> error_target{}, f0{}, x1{}, df{};
+    exe.check(odehip_rk_compute_error_target(ctx, error_target.data(), x0.data(),
+                                             atol.data(), rtol.data(), N),
+              "compute_error_target");
+    ode(exe, span(x0), span(f0));
+    auto norm = [&](const HipState<double, N>& v) {
+      double r = 0.0;
+      exe.check(odehip_rk_norm(ctx, v.data(), error_target.data(), N, &r), "rk_norm");
+      return r;
+    };
+    auto d0 = norm(x0);
+    auto d1 = norm(f0);
+    auto dt0 = (d0 < 1.0e-5 or d1 < 1.0e-5) ? 1.0e-6 : 0.01 * (d0 / d1);
+    exe.check(odehip_rk_euler_step(ctx, x1.data(), x0.data(), f0.data(), dt0, N),
+              "euler_step");
+    ode(exe, span(std::as_const(x1)), span(df));
+    exe.check(odehip_rk_difference(ctx, df.data(), f0.data(), N), "difference");
+    auto d2 = norm(df) / dt0;
+    constexpr auto p = BT::p;
+    auto dt1 = (std::max(d1, d2) <= 1.0e-15)
+                   ? std::max(1.0e-6, dt0 * 1.0e-3)
+                   : std::pow(0.01 / std::max(d1, d2), (1.0 / (1.0 + p)));
+    return std::min(100.0 * dt0, dt1);
+  }
+};
+
+}  // namespace odehip

@@ -73,6 +73,12 @@ hipError_t launch_rk_update_host_dt(int tab, double dt, double* x, double* err,
 hipError_t launch_rk_target(double* sc, const double* atol, const double* rtol,
                             const double* x, const double* x0, size_t n_var,
                             hipStream_t s);
+hipError_t launch_init_target(double* sc, const double* x0, const double* atol,
+                              const double* rtol, size_t n_var, hipStream_t s);
+hipError_t launch_euler_host_dt(double* x1, const double* x0, const double* f0,
+                                double dt0, size_t n_var, hipStream_t s);
+hipError_t launch_difference(double* df, const double* f0, size_t n_var,
+                             hipStream_t s);
 hipError_t launch_norm_partials(const double* v, const double* sub,
                                 const double* sc, size_t n_var,
                                 double* partials, hipStream_t s);

@@ -456,6 +456,28 @@ int odehip_rk_norm(odehip_context* ctx, const double* v, const double* sc,
   return ODEHIP_OK;
 }
 
+int odehip_rk_compute_error_target(odehip_context* ctx, double* sc,
+                                   const double* x0, const double* atol,
+                                   const double* rtol, size_t n_var) {
+  if (!ctx || !sc || !x0 || !atol || !rtol) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  HIP_TRY(ctx, launch_init_target(sc, x0, atol, rtol, n_var, ctx->stream));
+  return ODEHIP_OK;
+}
+
+int odehip_rk_euler_step(odehip_context* ctx, double* x1, const double* x0,
+                         const double* f0, double dt0, size_t n_var) {
+  if (!ctx || !x1 || !x0 || !f0) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  HIP_TRY(ctx, launch_euler_host_dt(x1, x0, f0, dt0, n_var, ctx->stream));
+  return ODEHIP_OK;
+}
+
+int odehip_rk_difference(odehip_context* ctx, double* df, const double* f0,
+                         size_t n_var) {
+  if (!ctx || !df || !f0) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  HIP_TRY(ctx, launch_difference(df, f0, n_var, ctx->stream));
+  return ODEHIP_OK;
+}
+
 int odehip_rk_initial_step(odehip_context* ctx, int tableau, int rhs_kind,
                            const double* x0, const double* atol,
                            const double* rtol, size_t n_var, double softening,

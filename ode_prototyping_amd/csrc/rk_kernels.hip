@@ -345,6 +345,18 @@ __global__ void k_init_step_a(SolverState* st, const double* pa,
   st->d1 = d1;
 }
 
+// Host-dt forms of the initial-step kernels (:151-168).
+__global__ void k_euler_host_dt(double* x1, const double* x0, const double* f0,
+                                double dt0, size_t n_var) {
+  const size_t i = gid();
+  if (i < n_var) x1[i] = x0[i] + f0[i] * dt0;
+}
+
+__global__ void k_difference(double* df, const double* f0, size_t n_var) {
+  const size_t i = gid();
+  if (i < n_var) df[i] -= f0[i];
+}
+
 // euler_step_kernel (:159-163).
 __global__ void k_euler(const SolverState* st, double* x1, const double* x0,
                         const double* f0, size_t n_var) {
@@ -441,6 +453,27 @@ hipError_t launch_rk_target(double* sc, const double* atol, const double* rtol,
                             hipStream_t s) {
   hipLaunchKernelGGL(k_target, dim3(blocks_for(n_var)), dim3(kBlock), 0, s, sc,
                      atol, rtol, x, x0, n_var);
+  return hipGetLastError();
+}
+
+hipError_t launch_init_target(double* sc, const double* x0, const double* atol,
+                              const double* rtol, size_t n_var, hipStream_t s) {
+  hipLaunchKernelGGL(k_init_target, dim3(blocks_for(n_var)), dim3(kBlock), 0, s, sc,
+                     x0, atol, rtol, n_var);
+  return hipGetLastError();
+}
+
+hipError_t launch_euler_host_dt(double* x1, const double* x0, const double* f0,
+                                double dt0, size_t n_var, hipStream_t s) {
+  hipLaunchKernelGGL(k_euler_host_dt, dim3(blocks_for(n_var)), dim3(kBlock), 0, s,
+                     x1, x0, f0, dt0, n_var);
+  return hipGetLastError();
+}
+
+hipError_t launch_difference(double* df, const double* f0, size_t n_var,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_difference, dim3(blocks_for(n_var)), dim3(kBlock), 0, s, df,
+                     f0, n_var);
   return hipGetLastError();
 }
 

@@ -1,0 +1,315 @@
+// The C++ host mirror (ode_prototyping_amd/cpp/odehip.hpp) driven the way the
+// reference's own suites drive the CUDA policy: src/test/CudaRKEmbeddedTest.cu,
+// CudaStateTest.cu and CudaExecutorTest.cu, with the golden values of
+// src/test/SingleThreadedRKEmbeddedTest.cpp (copied as data; file:line cited
+// per case).  GTest is not available, so a minimal EXPECT harness stands in;
+// the program prints one line per check and exits non-zero on any failure.
+//
+// Expected agreement: the host-driven integrate uses the host's std::pow and
+// the library's canonical reduction order, which for a 2-component state is
+// the single-threaded order -> Van der Pol goldens hold to EXPECT_DOUBLE_EQ;
+// larger states get the thread-pool suite's tolerances
+// (ThreadPoolRKEmbeddedTest.cpp:314-351, :362-400).
+#include <cmath>
+#include <cstdio>
+#include <memory>
+#include <numbers>
+#include <numeric>
+#include <random>
+#include <string>
+
+#include "odehip.hpp"
+
+using namespace odehip;
+
+static int g_failures = 0;
+static int g_checks = 0;
+
+static bool ulp_eq(double a, double b) {
+  if (a == b) return true;
+  const double d = std::abs(a - b);
+  const double u = std::nextafter(std::max(std::abs(a), std::abs(b)), INFINITY) -
+                   std::max(std::abs(a), std::abs(b));
+  return d <= 4 * u;
+}
+
+#define EXPECT_EQ(a, b) check((a) == (b), #a " == " #b, __LINE__, double(a), double(b))
+#define EXPECT_DOUBLE_EQ(a, b) check(ulp_eq((a), (b)), "DOUBLE_EQ " #a ", " #b, __LINE__, a, b)
+#define EXPECT_NEAR(a, b, tol) \
+  check(std::abs((a) - (b)) <= (tol), "NEAR " #a ", " #b, __LINE__, a, b)
+
+static void check(bool ok, const char* what, int line, double a, double b) {
+  ++g_checks;
+  if (!ok) {
+    ++g_failures;
+    std::printf("FAIL line %d: %s (%.17g vs %.17g)\n", line, what, a, b);
+  }
+}
+
+// --------------------------------------------------------------- fixtures
+// VanDerPolTest.hpp:10-24
+struct VanDerPolCase {
+  static constexpr std::size_t n_var = 2;
+  HipState<double, 2> x0{std::array{2.0, 0.0}};
+  double t0 = 0.0, tf = 2.0, tol = 1.0e-10;
+  HipState<double, 2> atol{std::array{1.0e-10, 1.0e-10}};
+  HipState<double, 2> rtol = atol;
+  VanDerPolODE<double> ode{};
+  RawOutput<2> output{};
+};
+
+// ExponentialTest.hpp:12-36
+struct ExponentialCase {
+  static constexpr std::size_t n_var = 10;
+  static std::array<double, 10> iota() {
+    std::array<double, 10> a{};
+    std::iota(a.begin(), a.end(), 0.0);
+    return a;
+  }
+  static std::array<double, 10> tols() {
+    std::array<double, 10> a{};
+    a.fill(1.0e-6);
+    return a;
+  }
+  HipState<double, 10> x0{iota()};
+  double t0 = 0.0, tf = 10.0;
+  HipState<double, 10> atol{tols()};
+  HipState<double, 10> rtol = atol;
+  ExponentialODE<double, 10> ode{};
+  RawOutput<10> output{};
+};
+
+// NBodyTest.hpp:13-37
+struct NBodyCase {
+  static constexpr std::size_t n_var = 30;
+  static constexpr std::array<double, 30> x0_data{
+      1.657666,  0.0,       0.0, 0.439775,  -0.169717, 0.0,
+      -1.268608, -0.267651, 0.0, -1.268608, 0.267651,  0.0,
+      0.439775,  0.169717,  0.0, 0.0,       -0.593786, 0.0,
+      1.822785,  0.128248,  0.0, 1.271564,  0.168645,  0.0,
+      -1.271564, 0.168645,  0.0, -1.822785, 0.128248,  0.0};
+  static std::array<double, 30> tols() {
+    std::array<double, 30> a{};
+    a.fill(1.0e-10);
+    return a;
+  }
+  HipState<double, 30> x0{x0_data};
+  double t0 = 0.0, tf = 6.3, tol = 1.0e-10;
+  HipState<double, 30> atol{tols()};
+  HipState<double, 30> rtol = atol;
+  NBodyODE<double, 30> ode{};
+  RawOutput<30> output{};
+};
+
+// SpinningCubeNBodyTest.hpp:53-113 (N = 1024)
+struct SpinningCubeCase {
+  static constexpr int N = 1024;
+  static constexpr std::size_t n_var = 6 * N;
+  static constexpr double L = 1.0;
+  double t0 = 0.0;
+  double tf = std::sqrt(L * L * L / N);
+  double omega = 2.0 * std::sqrt(std::numbers::pi / 3.0) / std::sqrt(L * L * L / N);
+  double softening = 4.0 * L / std::sqrt(N);
+  std::unique_ptr<HipState<double, n_var>> x0, atol, rtol;
+  NBodyODE<double, n_var> ode{};
+  RawOutput<n_var> output{};
+  SpinningCubeCase() {
+    auto gen = std::mt19937{0};
+    auto dist = std::uniform_real_distribution<double>(0.0, L);
+    auto data = std::make_unique<std::array<double, n_var>>();
+    for (std::size_t i = 0; i < n_var / 2; ++i) (*data)[i] = dist(gen);
+    for (int i = 0; i < N; ++i) {
+      const double x = (*data)[3 * i] - L / 2.0;
+      const double y = (*data)[3 * i + 1] - L / 2.0;
+      (*data)[n_var / 2 + 3 * i] = -omega * y;
+      (*data)[n_var / 2 + 3 * i + 1] = omega * x;
+      (*data)[n_var / 2 + 3 * i + 2] = 0.0;
+    }
+    x0 = std::make_unique<HipState<double, n_var>>(*data);
+    auto tol = std::make_unique<std::array<double, n_var>>();
+    tol->fill(softening);
+    atol = std::make_unique<HipState<double, n_var>>(*tol);
+    rtol = std::make_unique<HipState<double, n_var>>(*tol);
+    ode.softening = softening;
+  }
+};
+
+// ------------------------------------------------------------------- tests
+struct VdpGolden {
+  const char* name;
+  std::size_t count;
+  double t_mid, mid0, mid1, last0, last1;
+};
+
+template <typename BT>
+static void van_der_pol_consistency(const VdpGolden& g) {
+  HipExecutor executor{};
+  VanDerPolCase test;
+  RKEmbeddedParallel<BT>::integrate(test.x0, test.t0, test.tf, test.atol, test.rtol, test.ode,
+                                    test.output, executor);
+  const auto& times = test.output.times();
+  const auto& states = test.output.states();
+  EXPECT_EQ(g.count, times.size());
+  EXPECT_DOUBLE_EQ(0.0, times.front());
+  EXPECT_DOUBLE_EQ(g.t_mid, times[times.size() / 2]);
+  EXPECT_DOUBLE_EQ(2.0, times.back());
+  EXPECT_DOUBLE_EQ(2.0, states.front()[0]);
+  EXPECT_DOUBLE_EQ(0.0, states.front()[1]);
+  EXPECT_DOUBLE_EQ(g.mid0, states[states.size() / 2][0]);
+  EXPECT_DOUBLE_EQ(g.mid1, states[states.size() / 2][1]);
+  EXPECT_DOUBLE_EQ(g.last0, states.back()[0]);
+  EXPECT_DOUBLE_EQ(g.last1, states.back()[1]);
+  std::printf("ok %s\n", g.name);
+}
+
+static void dopri5_exponential() {
+  // SingleThreadedRKEmbeddedTest.cpp:162-192, thread-pool-style tolerance
+  HipExecutor executor{};
+  ExponentialCase test;
+  RKEmbeddedParallel<BTDOPRI5>::integrate(test.x0, test.t0, test.tf, test.atol, test.rtol,
+                                          test.ode, test.output, executor);
+  const auto& t = test.output.times();
+  const auto& s = test.output.states();
+  EXPECT_EQ(45u, t.size());
+  EXPECT_DOUBLE_EQ(10.0, t.back());
+  EXPECT_NEAR(4.9896555947535841, t[t.size() / 2], 1e-10);
+  EXPECT_NEAR(110132.46804595254, s.back()[5], 110132.0 * 1e-12);
+  EXPECT_NEAR(198238.44248271457, s.back()[9], 198238.0 * 1e-12);
+  std::printf("ok DOPRI5_Exponential\n");
+}
+
+static void rkf78_nbody() {
+  // SingleThreadedRKEmbeddedTest.cpp:306-340 goldens with the tolerances of
+  // ThreadPoolRKEmbeddedTest.cpp:314-351
+  HipExecutor executor{};
+  NBodyCase test;
+  RKEmbeddedParallel<BTRKF78>::integrate(test.x0, test.t0, test.tf, test.atol, test.rtol,
+                                         test.ode, test.output, executor);
+  const auto& t = test.output.times();
+  const auto& s = test.output.states();
+  const double midpoint_tol = 1.657666 * test.tol * t.size() / 2;
+  EXPECT_EQ(172u, t.size());
+  EXPECT_NEAR(3.1699354066499978, t[t.size() / 2], midpoint_tol);
+  EXPECT_DOUBLE_EQ(6.3, t.back());
+  EXPECT_DOUBLE_EQ(1.657666, s.front()[0]);
+  EXPECT_NEAR(-1.657049151812777, s[s.size() / 2][0], midpoint_tol);
+  EXPECT_NEAR(1.6573213121279662, s.back()[0], test.tol);
+  EXPECT_NEAR(0.46802604979077689, s.back()[3], test.tol);
+  EXPECT_NEAR(-1.2481480178176445, s.back()[6], test.tol);
+  EXPECT_NEAR(-1.2883624086182086, s.back()[9], test.tol);
+  EXPECT_NEAR(0.41116306451711232, s.back()[12], test.tol);
+  std::printf("ok RKF78_NBody\n");
+}
+
+static void rkf78_spinning_cube() {
+  // SingleThreadedRKEmbeddedTest.cpp:342-382 goldens; tolerance 10 eps as
+  // ThreadPoolRKEmbeddedTest.cpp:362
+  HipExecutor executor{};
+  SpinningCubeCase test;
+  RKEmbeddedParallel<BTRKF78>::integrate(*test.x0, test.t0, test.tf, *test.atol, *test.rtol,
+                                         test.ode, test.output, executor);
+  const auto& t = test.output.times();
+  const auto& s = test.output.states();
+  const double rounding_error = 10.0 * 2.220446049250313e-16;
+  constexpr int n = SpinningCubeCase::N;
+  EXPECT_EQ(4u, t.size());
+  EXPECT_NEAR(0.0099564200160493655, t[t.size() / 2], rounding_error);
+  EXPECT_DOUBLE_EQ(0.03125, t.back());
+  EXPECT_DOUBLE_EQ(0.59284461651668263, s.front()[0]);
+  EXPECT_DOUBLE_EQ(0.73169375852499918, s.front()[n / 4]);
+  EXPECT_NEAR(0.16121502269618729, s.back()[0], rounding_error);
+  EXPECT_NEAR(0.71404096847054921, s.back()[n / 4], rounding_error);
+  EXPECT_NEAR(0.59500181799963803, s.back()[n / 2], rounding_error);
+  EXPECT_NEAR(0.42875743462583493, s.back()[3 * n / 4], rounding_error);
+  std::printf("ok RKF78_SpinningCube1024\n");
+}
+
+static void device_resident_van_der_pol() {
+  // HE21 has 183011 accepted steps (SingleThreadedRKEmbeddedTest.cpp:28-49):
+  // run it, and DOPRI5, with the controller on the device.
+  HipExecutor executor{};
+  {
+    VanDerPolCase test;
+    auto x = test.x0;
+    std::vector<double> times;
+    auto st = RKEmbeddedParallel<BTHE21>::integrate_on_device(x, test.t0, test.tf, test.atol,
+                                                              test.rtol, test.ode, executor,
+                                                              &times);
+    std::array<double, 2> last{};
+    x.copy_to_span(last);
+    EXPECT_EQ(183012u, times.size());
+    EXPECT_EQ(183011, st.accepted);
+    EXPECT_NEAR(0.9274958393895415, times[times.size() / 2], 1e-12);
+    EXPECT_NEAR(0.32331666707211976, last[0], 1e-12);
+    EXPECT_NEAR(-1.8329745679629066, last[1], 1e-12);
+  }
+  {
+    VanDerPolCase test;
+    auto x = test.x0;
+    std::vector<double> times;
+    RKEmbeddedParallel<BTDOPRI5>::integrate_on_device(x, test.t0, test.tf, test.atol,
+                                                      test.rtol, test.ode, executor, &times);
+    std::array<double, 2> last{};
+    x.copy_to_span(last);
+    EXPECT_EQ(76u, times.size());
+    EXPECT_DOUBLE_EQ(0.67149697578171041, times[times.size() / 2]);
+    EXPECT_DOUBLE_EQ(0.32331666704309109, last[0]);
+    EXPECT_DOUBLE_EQ(-1.8329745679718159, last[1]);
+  }
+  std::printf("ok device_resident_VanDerPol\n");
+}
+
+static void state_semantics() {
+  // CudaStateTest.cu:18-92
+  constexpr std::size_t N = 10;
+  std::array<double, N> data{};
+  std::iota(data.begin(), data.end(), 0.0);
+  HipState<double, N> zero{};
+  std::array<double, N> out{};
+  zero.copy_to_span(out);
+  for (auto v : out) EXPECT_DOUBLE_EQ(0.0, v);
+  HipState<double, N> a{data};
+  a.copy_to_span(out);
+  for (std::size_t i = 0; i < N; ++i) EXPECT_DOUBLE_EQ(data[i], out[i]);
+  auto b = a;  // deep copy
+  HipExecutor executor{};
+  fill(executor, a, 3.0);
+  b.copy_to_span(out);
+  for (std::size_t i = 0; i < N; ++i) EXPECT_DOUBLE_EQ(data[i], out[i]);
+  a.copy_to_span(out);
+  for (auto v : out) EXPECT_DOUBLE_EQ(3.0, v);
+  // CudaExecutorTest.cu:52-61 inner product of iota and iota+1 = 330, as the
+  // square of a scaled RMS norm
+  std::array<double, N> v{}, one{};
+  for (std::size_t i = 0; i < N; ++i) {
+    v[i] = std::sqrt(static_cast<double>(i) * (i + 1));
+    one[i] = 1.0;
+  }
+  HipState<double, N> vs{v}, ones{one};
+  double norm = 0.0;
+  executor.check(odehip_rk_norm(executor.context(), vs.data(), ones.data(), N, &norm), "norm");
+  EXPECT_NEAR(330.0, norm * norm * N, 1e-10);
+  std::printf("ok state_semantics\n");
+}
+
+int main() {
+  state_semantics();
+  van_der_pol_consistency<BTDOPRI5>({"DOPRI5_VanDerPol", 76, 0.67149697578171041,
+                                     1.7382670271053748, -0.62280413467104634,
+                                     0.32331666704309109, -1.8329745679718159});
+  van_der_pol_consistency<BTRKF45>({"RKF45_VanDerPol", 83, 0.65364467640860291,
+                                    1.7493089062893852, -0.61420762201954171,
+                                    0.32331666497810646, -1.8329745707118268});
+  van_der_pol_consistency<BTDVERK>({"DVERK_VanDerPol", 42, 0.66331345368999683,
+                                    1.7433476890385384, -0.61887401617955773,
+                                    0.32331666651502594, -1.8329745684699728});
+  van_der_pol_consistency<BTRKF78>({"RKF78_VanDerPol", 15, 0.72814921744061389,
+                                    1.7022210141241134, -0.64964031512382325,
+                                    0.3233166666554449, -1.8329745686289092});
+  dopri5_exponential();
+  rkf78_nbody();
+  rkf78_spinning_cube();
+  device_resident_van_der_pol();
+  std::printf("%d checks, %d failures\n", g_checks, g_failures);
+  return g_failures == 0 ? 0 : 1;
+}

@@ -54,6 +54,20 @@ extern "C" {
                                 FP32, accumulation in FP64 */
 
 /* Solve exit status (the reference only ever exits with REACHED_END). */
+/* Step-size controllers.  REFERENCE is RKEmbeddedParallel.hpp:64-79
+   (dt * (0.38 / err)^(1/(1+q)), clamps [0.33, 6], floor, clip to tf).  PI
+   multiplies by (0.38/err)^(0.3/k) (err_prev/err)^(0.4/k), k = 1 + q
+   (Gustafsson's PI.3.4, err_prev of the last accepted step), with the same
+   clamps.  LEARNED asks a trained network for
+   the ratio dt_new / dt and falls back to PI (see odehip_mlp_*).  Every
+   controller accepts a step exactly when the embedded error norm is <= 1. */
+#define ODEHIP_CONTROLLER_REFERENCE 0
+#define ODEHIP_CONTROLLER_PI 1
+#define ODEHIP_CONTROLLER_LEARNED 2
+
+/* A trained step-size network (odehip_mlp_create); opaque. */
+struct odehip_mlp;
+
 #define ODEHIP_STATUS_RUNNING (-1)
 #define ODEHIP_STATUS_REACHED_END 0  /* t >= tf */
 #define ODEHIP_STATUS_MAX_ACCEPTED 1 /* accepted-step budget reached */
@@ -201,6 +215,8 @@ typedef struct {
   double* trace;         /* host, trace_cap x 3: per attempt (err, dt,
                             accepted); may be NULL */
   long trace_cap;
+  int controller;        /* ODEHIP_CONTROLLER_*; 0 = the reference's */
+  const struct odehip_mlp* mlp; /* trained network (LEARNED only) */
 } odehip_solve_options;
 
 typedef struct {
@@ -209,6 +225,7 @@ typedef struct {
   long n_trace;          /* entries written to trace */
   int status;            /* ODEHIP_STATUS_* */
   double t, dt, dt_initial, last_err;
+  long learned_steps;    /* steps sized by the learned controller */
 } odehip_solve_stats;
 
 /* x: device, n_var; in: x0, out: the state at the final time. */
@@ -225,6 +242,84 @@ int odehip_nbody_energy(odehip_context* ctx, const double* x, size_t n,
                         double* host_out);
 int odehip_nbody_momentum(odehip_context* ctx, const double* x, size_t n,
                           const double* masses, double* host_out);
+
+/* ------------------------------------------------------------- ensembles
+ * Many independent N-body systems of different sizes solved together, each
+ * with its own step size, accept/reject sequence, counters and termination
+ * (the reference integrates one system per integrate() call,
+ * RKEmbeddedParallel.hpp:13-81; this is that call for every system at once).
+ * States are packed: system k occupies 6 n_k doubles [pos | vel] starting at
+ * 6 * (n_0 + ... + n_{k-1}); masses (optional) are packed the same way with
+ * n_k entries per system.  Each system's result is bit-identical to
+ * odehip_rk_solve on that system alone with the same options (same right-hand
+ * side variant, FSAL setting and controller).
+ *
+ * The attempt of every active system advances in lockstep rounds: one
+ * launch per stage covers all systems, a round ends with a controller pass
+ * (one thread per system), and K rounds are captured in a HIP graph that is
+ * replayed until every system has finished.  The right-hand side runs one
+ * wavefront per tile of 64 (or 128) target particles; which systems use
+ * 128-target tiles, the waves per workgroup and the XCD placement of tiles
+ * are chosen by an autotuner that times the candidates on the ensemble. */
+#define ODEHIP_ENS_TILE_CONFIGS 12
+#define ODEHIP_ENS_TILE_AUTO (-1)
+
+typedef struct {
+  double t0, tf;
+  double atol, rtol;     /* scalar tolerances, every system and component */
+  long max_accepted;     /* per system; 0 = none */
+  long max_attempts;     /* per system; 0 = none */
+  int fsal;              /* as odehip_solve_options.fsal */
+  int rounds_per_launch; /* lockstep rounds per graph replay; 0 = auto */
+  int tile_config;       /* ODEHIP_ENS_TILE_AUTO or 0..ODEHIP_ENS_TILE_CONFIGS-1 */
+  int controller;        /* ODEHIP_CONTROLLER_* */
+  const struct odehip_mlp* mlp; /* trained network (LEARNED only) */
+  double* trace;         /* host, n_systems x trace_cap x 3: per attempt
+                            (err, dt, accepted); may be NULL */
+  long trace_cap;
+  double* tune_ms;       /* host, ODEHIP_ENS_TILE_CONFIGS entries: ms per
+                            right-hand side of every candidate (autotune
+                            only; untried entries 0); may be NULL */
+} odehip_ensemble_options;
+
+typedef struct {
+  long accepted, rejected, attempts, rhs_evals, n_trace, learned_steps;
+  int status;            /* ODEHIP_STATUS_* */
+  int pad;
+  double t, dt, dt_initial, last_err;
+} odehip_system_stats;
+
+typedef struct {
+  long rounds;           /* lockstep rounds launched */
+  long accepted, rejected, attempts, rhs_evals, learned_steps; /* sums */
+  double particle_steps; /* sum over systems of n_k * accepted_k */
+  double interactions;   /* sum n_k (n_k - 1): pair terms per ensemble RHS */
+  int tile_config;       /* the configuration used */
+  int n_unfinished;      /* systems still running at return (0 on success) */
+  double rhs_ms;         /* autotuner's time of one RHS in that configuration
+                            (0 when not tuned) */
+} odehip_ensemble_stats;
+
+/* n_particles: host, n_systems entries (>= 1).  masses, x0, x_out: device,
+ * packed (masses may be NULL: unit masses).  per_system: host array of
+ * n_systems or NULL.  Only ODEHIP_RHS_NBODY systems. */
+int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
+                          const int* n_particles, int nbody_variant,
+                          double softening, const double* masses,
+                          const odehip_ensemble_options* opt,
+                          const double* x0, double* x_out,
+                          odehip_ensemble_stats* stats,
+                          odehip_system_stats* per_system);
+
+/* One batched evaluation of every system's right-hand side (NBodyODE
+ * operator(), NBodyODE.hpp:31-39, per system): dxdt = f(x), both packed and
+ * on the device.  tile_config as above (AUTO: the tuned or default one);
+ * ms_out (host, may be NULL) receives the kernel time. */
+int odehip_ensemble_rhs(odehip_context* ctx, int n_systems,
+                        const int* n_particles, int nbody_variant,
+                        double softening, const double* masses,
+                        int tile_config, const double* x, double* dxdt,
+                        double* ms_out);
 
 /* ------------------------------------------------------ initial conditions
  * Host-side generators of the synthetic systems (seeded mt19937_64 with an

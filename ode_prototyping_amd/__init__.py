@@ -10,7 +10,7 @@ and the integrator (:func:`integrate`, ``RKEmbeddedParallel<BT>::integrate``,
 ``src/integrator/RKEmbeddedParallel/RKEmbeddedParallel.hpp:13-81``).  All
 numerical work runs in libodehip.so through its C ABI (include/odehip.h).
 """
-from ._lib import (NBODY_VARIANTS, REDUCE_GROUP, STAGES, STATUS, TABLEAUS, lib)
+from ._lib import (CONTROLLERS, NBODY_VARIANTS, REDUCE_GROUP, STAGES, STATUS, TABLEAUS, lib)
 from .core import (ExponentialODE, HipExecutor, HipState, MostRecentOutput, NBodyODE,
                    RawOutput, SolveResult, VanDerPolODE, integrate, integrate_host_driven)
 from . import scenarios

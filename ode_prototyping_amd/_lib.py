@@ -17,6 +17,7 @@ TABLEAUS = {"HE21": 0, "RKF45": 1, "DOPRI5": 2, "DVERK": 3, "RKF78": 4, "DOP853"
 STAGES = {"HE21": 2, "RKF45": 6, "DOPRI5": 7, "DVERK": 8, "RKF78": 13, "DOP853": 12}
 RHS_NBODY, RHS_EXPONENTIAL, RHS_VANDERPOL = 0, 1, 2
 NBODY_VARIANTS = {"exact": 0, "fast": 1, "mixed": 2}
+CONTROLLERS = {"reference": 0, "pi": 1, "learned": 2}
 STATUS = {-1: "running", 0: "reached_end", 1: "max_accepted", 2: "nonfinite", 3: "max_attempts"}
 REDUCE_GROUP = 256
 
@@ -34,15 +35,44 @@ class SolveOptions(C.Structure):
                 ("rtol", C.c_void_p), ("max_accepted", C.c_long), ("max_attempts", C.c_long),
                 ("fsal", C.c_int), ("attempts_per_launch", C.c_int),
                 ("times", C.c_void_p), ("times_cap", C.c_long),
-                ("trace", C.c_void_p), ("trace_cap", C.c_long)]
+                ("trace", C.c_void_p), ("trace_cap", C.c_long),
+                ("controller", C.c_int), ("mlp", C.c_void_p)]
 
 
 class SolveStats(C.Structure):
     _fields_ = [("accepted", C.c_long), ("rejected", C.c_long), ("attempts", C.c_long),
                 ("rhs_evals", C.c_long), ("n_times", C.c_long), ("n_trace", C.c_long),
                 ("status", C.c_int), ("t", C.c_double), ("dt", C.c_double),
+                ("dt_initial", C.c_double), ("last_err", C.c_double),
+                ("learned_steps", C.c_long)]
+
+
+class EnsembleOptions(C.Structure):
+    _fields_ = [("t0", C.c_double), ("tf", C.c_double), ("atol", C.c_double),
+                ("rtol", C.c_double), ("max_accepted", C.c_long), ("max_attempts", C.c_long),
+                ("fsal", C.c_int), ("rounds_per_launch", C.c_int), ("tile_config", C.c_int),
+                ("controller", C.c_int), ("mlp", C.c_void_p), ("trace", C.c_void_p),
+                ("trace_cap", C.c_long), ("tune_ms", C.c_void_p)]
+
+
+class SystemStats(C.Structure):
+    _fields_ = [("accepted", C.c_long), ("rejected", C.c_long), ("attempts", C.c_long),
+                ("rhs_evals", C.c_long), ("n_trace", C.c_long), ("learned_steps", C.c_long),
+                ("status", C.c_int), ("pad", C.c_int), ("t", C.c_double), ("dt", C.c_double),
                 ("dt_initial", C.c_double), ("last_err", C.c_double)]
 
+
+class EnsembleStats(C.Structure):
+    _fields_ = [("rounds", C.c_long), ("accepted", C.c_long), ("rejected", C.c_long),
+                ("attempts", C.c_long), ("rhs_evals", C.c_long), ("learned_steps", C.c_long),
+                ("particle_steps", C.c_double), ("interactions", C.c_double),
+                ("tile_config", C.c_int), ("n_unfinished", C.c_int), ("rhs_ms", C.c_double)]
+
+
+ENS_TILE_CONFIGS = 12
+ENS_TILE_AUTO = -1
+
+_ip = C.POINTER(C.c_int)
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -84,6 +114,12 @@ SIGNATURES = {
                                          C.c_int, _dp]),
     "odehip_rk_solve": (C.c_int, [_ctxp, C.c_int, C.POINTER(OdeStruct),
                                   C.POINTER(SolveOptions), C.c_void_p, C.POINTER(SolveStats)]),
+    "odehip_ensemble_solve": (C.c_int, [_ctxp, C.c_int, C.c_int, _ip, C.c_int, C.c_double,
+                                        C.c_void_p, C.POINTER(EnsembleOptions), C.c_void_p,
+                                        C.c_void_p, C.POINTER(EnsembleStats),
+                                        C.POINTER(SystemStats)]),
+    "odehip_ensemble_rhs": (C.c_int, [_ctxp, C.c_int, _ip, C.c_int, C.c_double, C.c_void_p,
+                                      C.c_int, C.c_void_p, C.c_void_p, _dp]),
     "odehip_nbody_energy": (C.c_int, [_ctxp, C.c_void_p, C.c_size_t, C.c_double, C.c_void_p,
                                       _dp]),
     "odehip_nbody_momentum": (C.c_int, [_ctxp, C.c_void_p, C.c_size_t, C.c_void_p, _dp]),

@@ -25,7 +25,7 @@ import math
 import numpy as np
 
 from . import _lib
-from ._lib import (NBODY_VARIANTS, RHS_EXPONENTIAL, RHS_NBODY, RHS_VANDERPOL, STAGES,
+from ._lib import (CONTROLLERS, NBODY_VARIANTS, RHS_EXPONENTIAL, RHS_NBODY, RHS_VANDERPOL, STAGES,
                    STATUS, TABLEAUS, dptr, lib)
 
 
@@ -301,7 +301,7 @@ def _as_state(exe, v, n, name):
 
 def integrate(tableau, x0, t0, tf, atol, rtol, ode, output=None, exe=None, *,
               max_accepted=0, max_attempts=0, fsal=False, attempts_per_launch=0,
-              times_cap=None, trace_cap=0, x_out=None, fetch=True):
+              times_cap=None, trace_cap=0, x_out=None, fetch=True, controller="reference"):
     """RKEmbeddedParallel<BT>::integrate with the attempt loop on the device.
 
     ``x0``: HipState or host array (n_var); ``atol``/``rtol``: scalars, host
@@ -316,6 +316,8 @@ def integrate(tableau, x0, t0, tf, atol, rtol, ode, output=None, exe=None, *,
     """
     if tableau not in TABLEAUS:
         raise ValueError(f"unknown tableau {tableau}")
+    if controller not in CONTROLLERS:
+        raise ValueError(f"unknown controller {controller}")
     exe = exe or HipExecutor()
     x0_host = None if isinstance(x0, HipState) else np.ascontiguousarray(x0, np.float64)
     n = x0.size if isinstance(x0, HipState) else x0_host.size
@@ -335,7 +337,8 @@ def integrate(tableau, x0, t0, tf, atol, rtol, ode, output=None, exe=None, *,
                             int(max_accepted), int(max_attempts), 1 if fsal else 0,
                             int(attempts_per_launch),
                             times.ctypes.data if times_cap > 0 else None, int(times_cap),
-                            trace.ctypes.data if trace_cap > 0 else None, int(trace_cap))
+                            trace.ctypes.data if trace_cap > 0 else None, int(trace_cap),
+                            CONTROLLERS[controller], None)
     stats = _lib.SolveStats()
     if output is not None:
         output.save_state(t0, work)

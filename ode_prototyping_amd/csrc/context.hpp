@@ -1,0 +1,90 @@
+// The library context (odehip_context of odehip.h) and the error helpers
+// shared by the C-ABI translation units.
+#pragma once
+
+#include <cstring>
+#include <string>
+
+#include "internal.hpp"
+
+namespace odehip {
+struct EnsembleWorkspace;
+void destroy_ensemble_workspace(EnsembleWorkspace* w);
+}  // namespace odehip
+
+struct GraphKey {
+  int tableau = -1, rhs_kind = -1, variant = -1, k = 0, fsal = -1;
+  size_t n_var = 0;
+  double eps2 = 0.0;
+  const void* mass = nullptr;
+  const void* atol = nullptr;
+  const void* rtol = nullptr;
+  const void* ws = nullptr;
+  bool operator==(const GraphKey& o) const {
+    return tableau == o.tableau && rhs_kind == o.rhs_kind &&
+           variant == o.variant && k == o.k && fsal == o.fsal &&
+           n_var == o.n_var && std::memcmp(&eps2, &o.eps2, sizeof eps2) == 0 &&
+           mass == o.mass && atol == o.atol && rtol == o.rtol && ws == o.ws;
+  }
+};
+
+struct odehip_context {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;  // current stream (own or external)
+  std::string err;
+  // solve workspace
+  double* ws = nullptr;
+  size_t ws_cap = 0;
+  odehip::SolverState* st = nullptr;
+  // small scratch for reductions
+  double* scratch = nullptr;
+  size_t scratch_cap = 0;
+  // pinned host staging
+  int* h_done = nullptr;           // 2 slots
+  odehip::SolverState* h_st = nullptr;
+  double* h_scalar = nullptr;      // 8 doubles
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipEvent_t t_start = nullptr, t_stop = nullptr;  // timing events
+  // graph cache
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  GraphKey key;
+  // ensemble solves (ensemble_abi.hip)
+  odehip::EnsembleWorkspace* ens = nullptr;
+};
+
+inline int fail(odehip_context* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+inline int hip_fail(odehip_context* ctx, hipError_t e, const char* where) {
+  return fail(ctx, ODEHIP_ERR_HIP,
+              std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                                   \
+  do {                                                       \
+    hipError_t _e = (expr);                                  \
+    if (_e != hipSuccess) return hip_fail((ctx), _e, #expr); \
+  } while (0)
+
+// Work stream for graph capture: the context's own stream, ordered after the
+// caller's stream when an external (possibly legacy default) stream is set.
+inline hipStream_t begin_work(odehip_context* ctx) {
+  if (ctx->stream != ctx->own) {
+    (void)hipEventRecord(ctx->ev_in, ctx->stream);
+    (void)hipStreamWaitEvent(ctx->own, ctx->ev_in, 0);
+  }
+  return ctx->own;
+}
+
+inline void end_work(odehip_context* ctx) {
+  if (ctx->stream != ctx->own) {
+    (void)hipEventRecord(ctx->ev_out, ctx->own);
+    (void)hipStreamWaitEvent(ctx->stream, ctx->ev_out, 0);
+  }
+}
+

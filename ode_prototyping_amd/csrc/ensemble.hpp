@@ -1,0 +1,84 @@
+// Internal layout of an ensemble solve: many independent N-body systems of
+// different sizes packed into one set of device vectors.
+#pragma once
+
+#include "internal.hpp"
+
+namespace odehip {
+
+// One system of the ensemble (device table).
+struct EnsSys {
+  long eoff;     // first state element (6 x the particles of earlier systems)
+  int n;         // particles
+  int gbase;     // first reduction group of this system
+  int n_groups;  // its groups of ODEHIP_REDUCE_GROUP elements
+  int pad;
+};
+
+// One right-hand-side work item: the targets [i0, i0 + 64 tpl) of system
+// `sys` (tpl = 2 for systems with n >= tpl2_min_n, else 1).  sys < 0 marks
+// an empty slot.
+struct EnsTile {
+  int sys;
+  int i0;
+};
+
+// Device buffers and parameters of one ensemble solve.  State-sized vectors
+// hold all systems packed (nv elements); ks holds S of them, nv apart.
+struct EnsBufs {
+  const EnsSys* sys;
+  const int* group_sys;  // group -> system
+  const EnsTile* tiles;  // RHS work items in dispatch order
+  int n_tiles;           // slots: RHS workgroups x waves per workgroup
+  SolverState* st;       // one per system
+  int* n_active;         // systems still running
+  double* xb0;
+  double* xb1;
+  double* ks;
+  double* ytmp;
+  double* kfsal;
+  double* sc0;
+  double* partA;
+  double* partB;
+  double* trace;  // n_sys x trace_cap x 3, or null
+  long trace_cap;
+  const double* mass;  // packed, or null
+  const void* mlp;     // learned controller's device network, or null
+  double atol, rtol;
+  size_t nv;
+  int n_sys;
+  int n_groups;
+  int tableau;
+  int variant;
+  int wpb;         // waves per RHS workgroup
+  int tpl2_min_n;  // systems with n >= this use 128-target tiles
+  double eps2;
+};
+
+// Autotuner candidates: waves per workgroup, the size from which systems use
+// 128-target tiles, and whether tiles of one system stay on one XCD.
+struct EnsTileConfig {
+  int wpb;
+  int tpl2_min_n;
+  int xcd;
+};
+constexpr int kEnsNoTpl2 = 1 << 30;
+constexpr EnsTileConfig kEnsTileConfigs[ODEHIP_ENS_TILE_CONFIGS] = {
+    {4, kEnsNoTpl2, 0}, {4, kEnsNoTpl2, 1}, {4, 1024, 0}, {4, 1024, 1},
+    {4, 256, 0},        {4, 256, 1},        {1, kEnsNoTpl2, 0}, {1, kEnsNoTpl2, 1},
+    {1, 1024, 0},       {1, 1024, 1},       {1, 256, 0},  {1, 256, 1}};
+constexpr int kEnsDefaultTileConfig = 1;
+
+// y == null: each system's current x0 buffer (solve); else a packed state.
+// st == null in b: plain evaluation of every system.
+hipError_t ens_launch_rhs(const EnsBufs& b, const double* y, double* k,
+                          int stage0, hipStream_t s);
+hipError_t ens_enqueue_init(const EnsBufs& b, const double* x_in, double t0,
+                            double tf, long max_acc, long max_att, int fsal,
+                            int controller, hipStream_t s);
+hipError_t ens_enqueue_initial_step(const EnsBufs& b, hipStream_t s);
+hipError_t ens_enqueue_round(const EnsBufs& b, hipStream_t s);
+hipError_t ens_enqueue_copy_out(const EnsBufs& b, double* x_out,
+                                hipStream_t s);
+
+}  // namespace odehip

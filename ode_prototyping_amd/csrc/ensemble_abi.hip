@@ -1,0 +1,519 @@
+// C ABI of the ensemble solve (declarations in odehip.h): layout and tile
+// tables, the tile autotuner, and the lockstep-round driver.
+//
+// The driver mirrors odehip_rk_solve: initial step for every system, then K
+// lockstep rounds (one attempt of each active system per round) captured in
+// a HIP graph and replayed until the device-side count of running systems
+// reaches zero; the host polls that count one replay behind.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "context.hpp"
+#include "ensemble.hpp"
+
+namespace odehip {
+
+struct EnsembleWorkspace {
+  char* dev = nullptr;  // device arena
+  size_t cap = 0;       // bytes
+  int* h_active = nullptr;  // pinned, 2 slots
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  // cached graph of K rounds and the arguments it was captured with
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  EnsBufs graph_bufs{};
+  int graph_k = 0;
+  // tables currently on the device
+  std::vector<int> table_sizes;
+  int table_cfg = -2;
+  const void* table_base = nullptr;
+  // autotuner cache
+  std::vector<int> tuned_sizes;
+  int tuned_variant = -1;
+  bool tuned_mass = false;
+  double tuned_eps2 = -1.0;
+  int tuned_cfg = -1;
+  double tuned_ms[ODEHIP_ENS_TILE_CONFIGS] = {};
+};
+
+void destroy_ensemble_workspace(EnsembleWorkspace* w) {
+  if (!w) return;
+  if (w->exec) (void)hipGraphExecDestroy(w->exec);
+  if (w->graph) (void)hipGraphDestroy(w->graph);
+  if (w->dev) (void)hipFree(w->dev);
+  if (w->h_active) (void)hipHostFree(w->h_active);
+  for (auto& e : w->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (w->t0) (void)hipEventDestroy(w->t0);
+  if (w->t1) (void)hipEventDestroy(w->t1);
+  delete w;
+}
+
+}  // namespace odehip
+
+using namespace odehip;
+
+namespace {
+
+// Host-side layout of an ensemble.
+struct Layout {
+  std::vector<EnsSys> sys;
+  std::vector<int> group_sys;
+  std::vector<int> sizes;
+  size_t nv = 0;
+  int n_groups = 0;
+  double interactions = 0.0;
+  double work = 0.0;  // sum n^2
+};
+
+int make_layout(odehip_context* ctx, int n_systems, const int* n_particles,
+                Layout* L) {
+  if (n_systems <= 0) return fail(ctx, ODEHIP_ERR_INVALID, "n_systems must be positive");
+  if (!n_particles) return fail(ctx, ODEHIP_ERR_INVALID, "n_particles is null");
+  L->sys.resize(n_systems);
+  L->sizes.assign(n_particles, n_particles + n_systems);
+  long eoff = 0;
+  long groups = 0;
+  for (int s = 0; s < n_systems; ++s) {
+    const int n = n_particles[s];
+    if (n < 1 || n > (1 << 24))
+      return fail(ctx, ODEHIP_ERR_INVALID, "system sizes must be in [1, 2^24]");
+    const int ng = n_groups_for(6 * static_cast<size_t>(n));
+    L->sys[s] = EnsSys{eoff, n, static_cast<int>(groups), ng, 0};
+    eoff += 6L * n;
+    groups += ng;
+    L->interactions += static_cast<double>(n) * (n - 1);
+    L->work += static_cast<double>(n) * n;
+  }
+  if (groups > (1L << 30)) return fail(ctx, ODEHIP_ERR_INVALID, "ensemble too large");
+  L->nv = static_cast<size_t>(eoff);
+  L->n_groups = static_cast<int>(groups);
+  L->group_sys.resize(groups);
+  for (int s = 0; s < n_systems; ++s)
+    for (int g = 0; g < L->sys[s].n_groups; ++g) L->group_sys[L->sys[s].gbase + g] = s;
+  return ODEHIP_OK;
+}
+
+// Right-hand-side work items for one tile configuration: tiles of the
+// largest systems first.  With xcd placement the systems are dealt to eight
+// queues of about equal sum n^2 and workgroup b takes its tiles from queue
+// b mod 8, the XCD that the hardware dispatches workgroup b to, so all tiles
+// of one system share one XCD's L2.
+std::vector<EnsTile> make_tiles(const Layout& L, const EnsTileConfig& cfg) {
+  const int n_sys = static_cast<int>(L.sys.size());
+  std::vector<int> order(n_sys);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return L.sys[a].n > L.sys[b].n; });
+  auto tiles_of = [&](int s, std::vector<EnsTile>& out) {
+    const int n = L.sys[s].n;
+    const int step = 64 * (n >= cfg.tpl2_min_n ? 2 : 1);
+    for (int i0 = 0; i0 < n; i0 += step) out.push_back(EnsTile{s, i0});
+  };
+  std::vector<EnsTile> tiles;
+  const EnsTile empty{-1, 0};
+  if (!cfg.xcd) {
+    for (int s : order) tiles_of(s, tiles);
+    while (tiles.size() % cfg.wpb) tiles.push_back(empty);
+    return tiles;
+  }
+  constexpr int kXcds = 8;
+  std::vector<std::vector<EnsTile>> q(kXcds);
+  double load[kXcds] = {};
+  for (int s : order) {
+    const int x = static_cast<int>(std::min_element(load, load + kXcds) - load);
+    load[x] += static_cast<double>(L.sys[s].n) * L.sys[s].n;
+    tiles_of(s, q[x]);
+  }
+  size_t rounds = 0;
+  for (auto& v : q) rounds = std::max(rounds, (v.size() + cfg.wpb - 1) / cfg.wpb);
+  tiles.reserve(rounds * kXcds * cfg.wpb);
+  for (size_t r = 0; r < rounds; ++r)
+    for (int x = 0; x < kXcds; ++x)
+      for (int w = 0; w < cfg.wpb; ++w) {
+        const size_t idx = r * cfg.wpb + w;
+        tiles.push_back(idx < q[x].size() ? q[x][idx] : empty);
+      }
+  return tiles;
+}
+
+size_t max_tiles(const Layout& L) {
+  size_t m = 0;
+  for (const auto& cfg : kEnsTileConfigs) m = std::max(m, make_tiles(L, cfg).size());
+  return m;
+}
+
+// Device arena: tables, per-system state, vectors.
+struct Plan {
+  size_t sys, group_sys, tiles, st, n_active, xb0, xb1, ks, ytmp, kfsal, sc0,
+      partA, partB, trace, total;
+};
+
+Plan plan(const Layout& L, size_t tiles_cap, int S, long trace_cap,
+          bool with_state) {
+  Plan p{};
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += (bytes + 255) & ~static_cast<size_t>(255);
+    return at;
+  };
+  const size_t ns = L.sys.size();
+  p.sys = take(ns * sizeof(EnsSys));
+  p.group_sys = take(L.group_sys.size() * sizeof(int));
+  p.tiles = take(tiles_cap * sizeof(EnsTile));
+  p.st = take(ns * sizeof(SolverState));
+  p.n_active = take(sizeof(int));
+  const size_t v = L.nv * sizeof(double);
+  if (with_state) {
+    p.xb0 = take(v);
+    p.xb1 = take(v);
+    p.ks = take(static_cast<size_t>(S) * v);
+    p.ytmp = take(v);
+    p.kfsal = take(v);
+    p.sc0 = take(v);
+    p.partA = take(L.group_sys.size() * sizeof(double));
+    p.partB = take(L.group_sys.size() * sizeof(double));
+    p.trace = take(ns * static_cast<size_t>(trace_cap) * 3 * sizeof(double));
+  }
+  p.total = off;
+  return p;
+}
+
+int workspace(odehip_context* ctx, size_t bytes, EnsembleWorkspace** out) {
+  if (!ctx->ens) {
+    auto* w = new (std::nothrow) EnsembleWorkspace();
+    if (!w) return fail(ctx, ODEHIP_ERR_NOMEM, "ensemble workspace");
+    ctx->ens = w;
+    HIP_TRY(ctx, hipHostMalloc(reinterpret_cast<void**>(&w->h_active), 2 * sizeof(int)));
+    for (auto& e : w->ev) HIP_TRY(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIP_TRY(ctx, hipEventCreate(&w->t0));
+    HIP_TRY(ctx, hipEventCreate(&w->t1));
+  }
+  EnsembleWorkspace* w = ctx->ens;
+  if (w->cap < bytes) {
+    HIP_TRY(ctx, hipDeviceSynchronize());
+    if (w->exec) (void)hipGraphExecDestroy(w->exec);
+    if (w->graph) (void)hipGraphDestroy(w->graph);
+    w->exec = nullptr;
+    w->graph = nullptr;
+    if (w->dev) HIP_TRY(ctx, hipFree(w->dev));
+    w->dev = nullptr;
+    w->cap = 0;
+    w->table_cfg = -2;
+    HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&w->dev), bytes));
+    w->cap = bytes;
+  }
+  *out = w;
+  return ODEHIP_OK;
+}
+
+// Uploads the system/group tables and the tile list of `cfg` (skipped when
+// the device already holds them).
+int upload_tables(odehip_context* ctx, EnsembleWorkspace* w, const Plan& p,
+                  const Layout& L, int cfg, int* n_tiles, hipStream_t s) {
+  const std::vector<EnsTile> tiles = make_tiles(L, kEnsTileConfigs[cfg]);
+  *n_tiles = static_cast<int>(tiles.size());
+  if (w->table_cfg == cfg && w->table_base == w->dev && w->table_sizes == L.sizes)
+    return ODEHIP_OK;
+  HIP_TRY(ctx, hipStreamSynchronize(s));  // no kernel may still read them
+  HIP_TRY(ctx, hipMemcpy(w->dev + p.sys, L.sys.data(), L.sys.size() * sizeof(EnsSys),
+                         hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(w->dev + p.group_sys, L.group_sys.data(),
+                         L.group_sys.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(w->dev + p.tiles, tiles.data(), tiles.size() * sizeof(EnsTile),
+                         hipMemcpyHostToDevice));
+  w->table_cfg = cfg;
+  w->table_base = w->dev;
+  w->table_sizes = L.sizes;
+  return ODEHIP_OK;
+}
+
+EnsBufs make_bufs(EnsembleWorkspace* w, const Plan& p, const Layout& L,
+                  bool with_state) {
+  EnsBufs b;
+  std::memset(&b, 0, sizeof b);  // padding too: the graph cache compares bytes
+  char* d = w->dev;
+  b.sys = reinterpret_cast<const EnsSys*>(d + p.sys);
+  b.group_sys = reinterpret_cast<const int*>(d + p.group_sys);
+  b.tiles = reinterpret_cast<const EnsTile*>(d + p.tiles);
+  b.n_active = reinterpret_cast<int*>(d + p.n_active);
+  if (with_state) {
+    b.st = reinterpret_cast<SolverState*>(d + p.st);
+    b.xb0 = reinterpret_cast<double*>(d + p.xb0);
+    b.xb1 = reinterpret_cast<double*>(d + p.xb1);
+    b.ks = reinterpret_cast<double*>(d + p.ks);
+    b.ytmp = reinterpret_cast<double*>(d + p.ytmp);
+    b.kfsal = reinterpret_cast<double*>(d + p.kfsal);
+    b.sc0 = reinterpret_cast<double*>(d + p.sc0);
+    b.partA = reinterpret_cast<double*>(d + p.partA);
+    b.partB = reinterpret_cast<double*>(d + p.partB);
+  }
+  b.nv = L.nv;
+  b.n_sys = static_cast<int>(L.sys.size());
+  b.n_groups = L.n_groups;
+  return b;
+}
+
+void set_tile_cfg(EnsBufs* b, int cfg, int n_tiles) {
+  b->wpb = kEnsTileConfigs[cfg].wpb;
+  b->tpl2_min_n = kEnsTileConfigs[cfg].tpl2_min_n;
+  b->n_tiles = n_tiles;
+}
+
+bool tune_cache_hit(const EnsembleWorkspace* w, const Layout& L, int variant,
+                    bool mass, double eps2) {
+  return w->tuned_cfg >= 0 && w->tuned_variant == variant && w->tuned_mass == mass &&
+         std::memcmp(&w->tuned_eps2, &eps2, sizeof eps2) == 0 && w->tuned_sizes == L.sizes;
+}
+
+// Times one right-hand side of the state `y` into `k` for every candidate
+// configuration (one warm-up, best of two) and keeps the fastest.
+int autotune(odehip_context* ctx, EnsembleWorkspace* w, const Plan& p,
+             const Layout& L, EnsBufs b, const double* y, double* k,
+             hipStream_t s) {
+  double best = 1e300;
+  int best_cfg = kEnsDefaultTileConfig;
+  for (int c = 0; c < ODEHIP_ENS_TILE_CONFIGS; ++c) {
+    int n_tiles = 0;
+    int rc = upload_tables(ctx, w, p, L, c, &n_tiles, s);
+    if (rc) return rc;
+    set_tile_cfg(&b, c, n_tiles);
+    HIP_TRY(ctx, ens_launch_rhs(b, y, k, 0, s));
+    double ms_best = 1e300;
+    for (int r = 0; r < 2; ++r) {
+      HIP_TRY(ctx, hipEventRecord(w->t0, s));
+      HIP_TRY(ctx, ens_launch_rhs(b, y, k, 0, s));
+      HIP_TRY(ctx, hipEventRecord(w->t1, s));
+      HIP_TRY(ctx, hipEventSynchronize(w->t1));
+      float ms = 0.0f;
+      HIP_TRY(ctx, hipEventElapsedTime(&ms, w->t0, w->t1));
+      ms_best = std::min(ms_best, static_cast<double>(ms));
+    }
+    w->tuned_ms[c] = ms_best;
+    if (ms_best < best) {
+      best = ms_best;
+      best_cfg = c;
+    }
+  }
+  w->tuned_cfg = best_cfg;
+  return ODEHIP_OK;
+}
+
+int check_common(odehip_context* ctx, int nbody_variant, double softening) {
+  if (nbody_variant < 0 || nbody_variant > 2)
+    return fail(ctx, ODEHIP_ERR_INVALID, "bad nbody_variant");
+  if (!(softening >= 0.0)) return fail(ctx, ODEHIP_ERR_INVALID, "bad softening");
+  return ODEHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int odehip_ensemble_rhs(odehip_context* ctx, int n_systems,
+                        const int* n_particles, int nbody_variant,
+                        double softening, const double* masses,
+                        int tile_config, const double* x, double* dxdt,
+                        double* ms_out) {
+  if (!ctx) return ODEHIP_ERR_INVALID;
+  if (!x || !dxdt) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  int rc = check_common(ctx, nbody_variant, softening);
+  if (rc) return rc;
+  if (tile_config < ODEHIP_ENS_TILE_AUTO || tile_config >= ODEHIP_ENS_TILE_CONFIGS)
+    return fail(ctx, ODEHIP_ERR_INVALID, "bad tile_config");
+  Layout L;
+  if ((rc = make_layout(ctx, n_systems, n_particles, &L))) return rc;
+  const size_t tcap = max_tiles(L);
+  const Plan p = plan(L, tcap, 0, 0, false);
+  EnsembleWorkspace* w = nullptr;
+  if ((rc = workspace(ctx, p.total, &w))) return rc;
+  const double eps2 = softening * softening;
+  int cfg = tile_config;
+  if (cfg == ODEHIP_ENS_TILE_AUTO)
+    cfg = tune_cache_hit(w, L, nbody_variant, masses != nullptr, eps2) ? w->tuned_cfg
+                                                                      : kEnsDefaultTileConfig;
+  hipStream_t s = begin_work(ctx);
+  int n_tiles = 0;
+  if ((rc = upload_tables(ctx, w, p, L, cfg, &n_tiles, s))) return rc;
+  EnsBufs b = make_bufs(w, p, L, false);
+  b.mass = masses;
+  b.variant = nbody_variant;
+  b.eps2 = eps2;
+  set_tile_cfg(&b, cfg, n_tiles);
+  HIP_TRY(ctx, hipEventRecord(w->t0, s));
+  HIP_TRY(ctx, ens_launch_rhs(b, x, dxdt, 0, s));
+  HIP_TRY(ctx, hipEventRecord(w->t1, s));
+  if (ms_out) {
+    HIP_TRY(ctx, hipEventSynchronize(w->t1));
+    float ms = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, w->t0, w->t1));
+    *ms_out = ms;
+  }
+  end_work(ctx);
+  return ODEHIP_OK;
+}
+
+int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
+                          const int* n_particles, int nbody_variant,
+                          double softening, const double* masses,
+                          const odehip_ensemble_options* opt,
+                          const double* x0, double* x_out,
+                          odehip_ensemble_stats* stats,
+                          odehip_system_stats* per_system) {
+  if (!ctx) return ODEHIP_ERR_INVALID;
+  if (!opt || !x0 || !x_out) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  if (tableau < 0 || tableau >= ODEHIP_NUM_TABLEAUS)
+    return fail(ctx, ODEHIP_ERR_INVALID, "bad tableau");
+  int rc = check_common(ctx, nbody_variant, softening);
+  if (rc) return rc;
+  if (opt->max_accepted < 0 || opt->max_attempts < 0 || opt->trace_cap < 0)
+    return fail(ctx, ODEHIP_ERR_INVALID, "negative budget or capacity");
+  if (opt->tile_config < ODEHIP_ENS_TILE_AUTO || opt->tile_config >= ODEHIP_ENS_TILE_CONFIGS)
+    return fail(ctx, ODEHIP_ERR_INVALID, "bad tile_config");
+  if (opt->controller < ODEHIP_CONTROLLER_REFERENCE ||
+      opt->controller > ODEHIP_CONTROLLER_LEARNED)
+    return fail(ctx, ODEHIP_ERR_INVALID, "bad controller");
+  if (opt->controller == ODEHIP_CONTROLLER_LEARNED)
+    return fail(ctx, ODEHIP_ERR_UNSUPPORTED, "learned controller not available");
+  Layout L;
+  if ((rc = make_layout(ctx, n_systems, n_particles, &L))) return rc;
+  const int S = kTableaus[tableau].S;
+  const long trace_cap = opt->trace ? opt->trace_cap : 0;
+  const size_t tcap = max_tiles(L);
+  const Plan p = plan(L, tcap, S, trace_cap, true);
+  EnsembleWorkspace* w = nullptr;
+  if ((rc = workspace(ctx, p.total, &w))) return rc;
+  const double eps2 = softening * softening;
+  hipStream_t s = begin_work(ctx);
+
+  EnsBufs b = make_bufs(w, p, L, true);
+  b.trace = trace_cap > 0 ? reinterpret_cast<double*>(w->dev + p.trace) : nullptr;
+  b.trace_cap = trace_cap;
+  b.mass = masses;
+  b.atol = opt->atol;
+  b.rtol = opt->rtol;
+  b.tableau = tableau;
+  b.variant = nbody_variant;
+  b.eps2 = eps2;
+
+  // tile configuration
+  int cfg = opt->tile_config;
+  if (cfg == ODEHIP_ENS_TILE_AUTO) {
+    if (!tune_cache_hit(w, L, nbody_variant, masses != nullptr, eps2)) {
+      std::fill(w->tuned_ms, w->tuned_ms + ODEHIP_ENS_TILE_CONFIGS, 0.0);
+      EnsBufs tb = b;
+      tb.st = nullptr;  // plain evaluations of x0 into the k1 slot
+      if ((rc = autotune(ctx, w, p, L, tb, x0, b.ks, s))) return rc;
+      w->tuned_variant = nbody_variant;
+      w->tuned_mass = masses != nullptr;
+      w->tuned_eps2 = eps2;
+      w->tuned_sizes = L.sizes;
+    }
+    cfg = w->tuned_cfg;
+  }
+  const bool tuned = tune_cache_hit(w, L, nbody_variant, masses != nullptr, eps2);
+  if (opt->tune_ms) {
+    for (int c = 0; c < ODEHIP_ENS_TILE_CONFIGS; ++c)
+      opt->tune_ms[c] = tuned ? w->tuned_ms[c] : 0.0;
+  }
+  int n_tiles = 0;
+  if ((rc = upload_tables(ctx, w, p, L, cfg, &n_tiles, s))) return rc;
+  set_tile_cfg(&b, cfg, n_tiles);
+
+  HIP_TRY(ctx, ens_enqueue_init(b, x0, opt->t0, opt->tf, opt->max_accepted,
+                                opt->max_attempts, opt->fsal ? 1 : 0, opt->controller, s));
+  HIP_TRY(ctx, ens_enqueue_initial_step(b, s));
+
+  int K = opt->rounds_per_launch;
+  if (K <= 0) {
+    const double work = static_cast<double>(S) * L.work;
+    K = work > 1.0e10 ? 2 : (work > 1.0e9 ? 4 : (work > 1.0e8 ? 8 : 32));
+  }
+  if (!w->exec || w->graph_k != K || std::memcmp(&w->graph_bufs, &b, sizeof b) != 0) {
+    if (w->exec) (void)hipGraphExecDestroy(w->exec);
+    if (w->graph) (void)hipGraphDestroy(w->graph);
+    w->exec = nullptr;
+    w->graph = nullptr;
+    HIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t ce = hipSuccess;
+    for (int r = 0; r < K && ce == hipSuccess; ++r) ce = ens_enqueue_round(b, s);
+    hipGraph_t g = nullptr;
+    hipError_t ee = hipStreamEndCapture(s, &g);
+    if (ce != hipSuccess) return hip_fail(ctx, ce, "capture round");
+    if (ee != hipSuccess) return hip_fail(ctx, ee, "hipStreamEndCapture");
+    w->graph = g;
+    HIP_TRY(ctx, hipGraphInstantiate(&w->exec, g, nullptr, nullptr, 0));
+    w->graph_bufs = b;
+    w->graph_k = K;
+  }
+
+  const long cap_rounds = opt->max_attempts > 0 ? opt->max_attempts : (1L << 40);
+  long launched = 0;
+  int slot = 0;
+  bool have_prev = false;
+  for (;;) {
+    HIP_TRY(ctx, hipGraphLaunch(w->exec, s));
+    launched += K;
+    HIP_TRY(ctx, hipMemcpyAsync(&w->h_active[slot], b.n_active, sizeof(int),
+                                hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipEventRecord(w->ev[slot], s));
+    if (have_prev) {
+      HIP_TRY(ctx, hipEventSynchronize(w->ev[slot ^ 1]));
+      if (w->h_active[slot ^ 1] == 0) break;
+    }
+    have_prev = true;
+    slot ^= 1;
+    if (launched >= cap_rounds + 2L * K) break;
+  }
+  HIP_TRY(ctx, ens_enqueue_copy_out(b, x_out, s));
+  std::vector<SolverState> hs(L.sys.size());
+  HIP_TRY(ctx, hipMemcpyAsync(hs.data(), b.st, hs.size() * sizeof(SolverState),
+                              hipMemcpyDeviceToHost, s));
+  HIP_TRY(ctx, hipStreamSynchronize(s));
+  if (opt->trace && trace_cap > 0)
+    HIP_TRY(ctx, hipMemcpy(opt->trace, b.trace,
+                           hs.size() * static_cast<size_t>(trace_cap) * 3 * sizeof(double),
+                           hipMemcpyDeviceToHost));
+  end_work(ctx);
+  odehip_ensemble_stats agg{};
+  agg.tile_config = cfg;
+  agg.interactions = L.interactions;
+  agg.rhs_ms = tuned ? w->tuned_ms[cfg] : 0.0;
+  for (size_t k = 0; k < hs.size(); ++k) {
+    const SolverState& h = hs[k];
+    agg.rounds = std::max(agg.rounds, h.attempts);
+    agg.accepted += h.accepted;
+    agg.rejected += h.rejected;
+    agg.attempts += h.attempts;
+    agg.rhs_evals += h.rhs_evals;
+    agg.learned_steps += h.learned_steps;
+    agg.particle_steps += static_cast<double>(L.sys[k].n) * h.accepted;
+    if (h.status == ODEHIP_STATUS_RUNNING) agg.n_unfinished += 1;
+    if (per_system) {
+      odehip_system_stats& o = per_system[k];
+      o.accepted = h.accepted;
+      o.rejected = h.rejected;
+      o.attempts = h.attempts;
+      o.rhs_evals = h.rhs_evals;
+      o.n_trace = std::min(h.attempts, trace_cap);
+      o.learned_steps = h.learned_steps;
+      o.status = h.status;
+      o.pad = 0;
+      o.t = h.t;
+      o.dt = h.dt;
+      o.dt_initial = h.dt_initial;
+      o.last_err = h.err;
+    }
+  }
+  if (stats) *stats = agg;
+  if (agg.n_unfinished)
+    return fail(ctx, ODEHIP_ERR_INVALID, "ensemble did not finish within the attempt cap");
+  return ODEHIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,587 @@
+// Ensemble kernels: the embedded RK attempt of many independent N-body
+// systems advanced together, one launch per stage for all of them.
+//
+// Layout: the systems' states are packed (system k at element offset
+// eoff_k = 6 (n_0 + ... + n_{k-1})).  The element-wise kernels run one
+// 256-thread workgroup per reduction group; a group never straddles two
+// systems (groups are counted from each system's first element), so a
+// system's error norm is reduced in exactly the order of a single-system
+// solve.  Per-element arithmetic, the group reduction and the controller are
+// the functions the single-system kernels call (rk_device.hpp).
+//
+// The right-hand side runs one wavefront per tile of 64 (or 128) targets of
+// one system: the wave walks that system's partners j = 0 .. n-1 in order,
+// partner coordinates arriving through scalar loads (wave-uniform), and
+// accumulates every target's acceleration in registers -- for the exact
+// variant exactly the increasing-j order of the reference's single-threaded
+// pair loop, so no cross-wave fold is needed.  Tiles are listed largest
+// system first, so the long tiles start early and the short ones fill the
+// tail; the list is built on the host per tile configuration.
+#include "ensemble.hpp"
+#include "nbody_terms.hpp"
+#include "rk_device.hpp"
+
+namespace odehip {
+namespace {
+
+__constant__ Tableau e_tab[ODEHIP_NUM_TABLEAUS] = {
+    kTableaus[0], kTableaus[1], kTableaus[2],
+    kTableaus[3], kTableaus[4], kTableaus[5]};
+
+constexpr int kBlock = ODEHIP_REDUCE_GROUP;
+
+typedef const __attribute__((address_space(4))) double* uniform_ptr;
+__device__ inline uniform_ptr as_uniform(const double* p) {
+  return (uniform_ptr)(p);
+}
+
+// ------------------------------------------------------------- RHS tiles --
+
+// dxdt[0:3n] = x[3n:6n] for the tile's targets (NBodyODE.hpp:51-56).
+template <int TPL>
+__device__ inline void tile_velocities(const double* __restrict__ y,
+                                       double* __restrict__ k, int n,
+                                       int i0) {
+  const int lane = threadIdx.x & 63;
+  const long lim = 3L * n;
+  const long base = 3L * i0;
+#pragma unroll
+  for (int c = 0; c < 3 * TPL; ++c) {
+    const long e = base + 64 * c + lane;
+    if (e < lim) k[e] = y[lim + e];
+  }
+}
+
+template <int TPL>
+__device__ inline void tile_store(double* __restrict__ k, int n, int i0,
+                                  const double* ax, const double* ay,
+                                  const double* az) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int t = 0; t < TPL; ++t) {
+    const int i = i0 + 64 * t + lane;
+    if (i < n) {
+      k[3L * n + 3 * i] = ax[t];
+      k[3L * n + 3 * i + 1] = ay[t];
+      k[3L * n + 3 * i + 2] = az[t];
+    }
+  }
+}
+
+// exact: every target sums its partners' terms in increasing j, skipping
+// j == i (NBodyODE.hpp:89-98 with the order of the single-threaded loop).
+// Terms of U partners are computed together; the shared-reciprocal form is
+// used when every lane's displacements are in its range (the diagonal
+// partner's term is discarded and does not count).
+template <bool kMass, bool kSharedRcp, int TPL>
+__device__ inline void exact_tile(const double* __restrict__ y,
+                                  const double* __restrict__ mass,
+                                  double* __restrict__ k, int n, int i0,
+                                  double eps2) {
+  constexpr int U = TPL == 1 ? 4 : 2;
+  const int lane = threadIdx.x & 63;
+  const uniform_ptr yu = as_uniform(y);
+  const uniform_ptr mu = as_uniform(mass);
+  int ic[TPL];
+  double xi[TPL], yi[TPL], zi[TPL], ax[TPL], ay[TPL], az[TPL];
+#pragma unroll
+  for (int t = 0; t < TPL; ++t) {
+    const int i = i0 + 64 * t + lane;
+    ic[t] = i < n ? i : n - 1;
+    xi[t] = y[3 * ic[t]];
+    yi[t] = y[3 * ic[t] + 1];
+    zi[t] = y[3 * ic[t] + 2];
+    ax[t] = ay[t] = az[t] = 0.0;
+  }
+  double px[U], py[U], pz[U], pm[U];
+  auto load = [&](int j) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      px[u] = yu[3 * (j + u)];
+      py[u] = yu[3 * (j + u) + 1];
+      pz[u] = yu[3 * (j + u) + 2];
+      pm[u] = kMass ? mu[j + u] : 1.0;
+    }
+  };
+  int j = 0;
+  if (U <= n) load(0);
+  for (; j + U <= n; j += U) {
+    double cx[U], cy[U], cz[U], cm[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      cx[u] = px[u];
+      cy[u] = py[u];
+      cz[u] = pz[u];
+      cm[u] = pm[u];
+    }
+    if (j + 2 * U <= n) load(j + U);  // next partners, one group ahead
+    double dx[U][TPL], dy[U][TPL], dz[U][TPL];
+    bool ok = true;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int t = 0; t < TPL; ++t) {
+        dx[u][t] = cx[u] - xi[t];
+        dy[u][t] = cy[u] - yi[t];
+        dz[u][t] = cz[u] - zi[t];
+        if (kSharedRcp)
+          ok = ok & (in_shared_rcp_range(dx[u][t], dy[u][t], dz[u][t]) |
+                     (j + u == ic[t]));
+      }
+    }
+    double tx[U][TPL], ty[U][TPL], tz[U][TPL];
+    if (kSharedRcp && __all(ok)) {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int t = 0; t < TPL; ++t)
+          exact_term_shared_rcp<kMass>(dx[u][t], dy[u][t], dz[u][t], cm[u], eps2,
+                                       tx[u][t], ty[u][t], tz[u][t]);
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int t = 0; t < TPL; ++t)
+          exact_term_ieee<kMass>(dx[u][t], dy[u][t], dz[u][t], cm[u], eps2,
+                                 tx[u][t], ty[u][t], tz[u][t]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int t = 0; t < TPL; ++t) {
+        const bool self = (j + u == ic[t]);
+        ax[t] = self ? ax[t] : ax[t] + tx[u][t];
+        ay[t] = self ? ay[t] : ay[t] + ty[u][t];
+        az[t] = self ? az[t] : az[t] + tz[u][t];
+      }
+    }
+  }
+  for (; j < n; ++j) {
+    const double cx = yu[3 * j], cy = yu[3 * j + 1], cz = yu[3 * j + 2];
+    const double cm = kMass ? mu[j] : 1.0;
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+      double tx, ty, tz;
+      exact_term_ieee<kMass>(cx - xi[t], cy - yi[t], cz - zi[t], cm, eps2, tx, ty,
+                             tz);
+      const bool self = (j == ic[t]);
+      ax[t] = self ? ax[t] : ax[t] + tx;
+      ay[t] = self ? ay[t] : ay[t] + ty;
+      az[t] = self ? az[t] : az[t] + tz;
+    }
+  }
+  tile_store<TPL>(k, n, i0, ax, ay, az);
+  tile_velocities<TPL>(y, k, n, i0);
+}
+
+// fast: refined hardware rsqrt, FMA accumulation in increasing j.
+template <bool kMass, bool kGuard, int TPL>
+__device__ inline void fast_tile(const double* __restrict__ y,
+                                 const double* __restrict__ mass,
+                                 double* __restrict__ k, int n, int i0,
+                                 double eps2) {
+  constexpr int U = TPL == 1 ? 8 : 4;
+  const int lane = threadIdx.x & 63;
+  const uniform_ptr yu = as_uniform(y);
+  const uniform_ptr mu = as_uniform(mass);
+  int ic[TPL];
+  double xi[TPL], yi[TPL], zi[TPL], ax[TPL], ay[TPL], az[TPL];
+#pragma unroll
+  for (int t = 0; t < TPL; ++t) {
+    const int i = i0 + 64 * t + lane;
+    ic[t] = i < n ? i : n - 1;
+    xi[t] = y[3 * ic[t]];
+    yi[t] = y[3 * ic[t] + 1];
+    zi[t] = y[3 * ic[t] + 2];
+    ax[t] = ay[t] = az[t] = 0.0;
+  }
+  int j = 0;
+  for (; j + U <= n; j += U) {
+    const uniform_ptr pp = yu + 3 * j;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double mj = kMass ? mu[j + u] : 1.0;
+#pragma unroll
+      for (int t = 0; t < TPL; ++t)
+        fast_term<kMass, kGuard>(pp[3 * u], pp[3 * u + 1], pp[3 * u + 2], mj, j + u,
+                                 ic[t], xi[t], yi[t], zi[t], eps2, ax[t], ay[t],
+                                 az[t]);
+    }
+  }
+  for (; j < n; ++j) {
+    const double mj = kMass ? mu[j] : 1.0;
+#pragma unroll
+    for (int t = 0; t < TPL; ++t)
+      fast_term<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j, ic[t],
+                               xi[t], yi[t], zi[t], eps2, ax[t], ay[t], az[t]);
+  }
+  tile_store<TPL>(k, n, i0, ax, ay, az);
+  tile_velocities<TPL>(y, k, n, i0);
+}
+
+// mixed: FP64 displacement rounded to FP32, FP32 pair force, partial sums
+// in FP32 flushed into FP64 every 32 partners.
+template <bool kMass, bool kGuard, int TPL>
+__device__ inline void mixed_tile(const double* __restrict__ y,
+                                  const double* __restrict__ mass,
+                                  double* __restrict__ k, int n, int i0,
+                                  double eps2) {
+  constexpr int kFlush = 32;
+  constexpr int U = 8;
+  const int lane = threadIdx.x & 63;
+  const uniform_ptr yu = as_uniform(y);
+  const uniform_ptr mu = as_uniform(mass);
+  const float eps2f = static_cast<float>(eps2);
+  int ic[TPL];
+  double xi[TPL], yi[TPL], zi[TPL], ax[TPL], ay[TPL], az[TPL];
+#pragma unroll
+  for (int t = 0; t < TPL; ++t) {
+    const int i = i0 + 64 * t + lane;
+    ic[t] = i < n ? i : n - 1;
+    xi[t] = y[3 * ic[t]];
+    yi[t] = y[3 * ic[t] + 1];
+    zi[t] = y[3 * ic[t] + 2];
+    ax[t] = ay[t] = az[t] = 0.0;
+  }
+  int j = 0;
+  while (j < n) {
+    const int jstop = min(n, j + kFlush);
+    float fx[TPL], fy[TPL], fz[TPL];
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) fx[t] = fy[t] = fz[t] = 0.0f;
+    for (; j + U <= jstop; j += U) {
+      const uniform_ptr pp = yu + 3 * j;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const float mj = kMass ? static_cast<float>(mu[j + u]) : 1.0f;
+#pragma unroll
+        for (int t = 0; t < TPL; ++t)
+          mixed_term<kMass, kGuard>(pp[3 * u], pp[3 * u + 1], pp[3 * u + 2], mj, j + u,
+                                    ic[t], xi[t], yi[t], zi[t], eps2f, fx[t], fy[t],
+                                    fz[t]);
+      }
+    }
+    for (; j < jstop; ++j) {
+      const float mj = kMass ? static_cast<float>(mu[j]) : 1.0f;
+#pragma unroll
+      for (int t = 0; t < TPL; ++t)
+        mixed_term<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j, ic[t],
+                                  xi[t], yi[t], zi[t], eps2f, fx[t], fy[t], fz[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+      ax[t] += static_cast<double>(fx[t]);
+      ay[t] += static_cast<double>(fy[t]);
+      az[t] += static_cast<double>(fz[t]);
+    }
+  }
+  tile_store<TPL>(k, n, i0, ax, ay, az);
+  tile_velocities<TPL>(y, k, n, i0);
+}
+
+// One wave per tile slot.  kSpecial: shared reciprocal (exact) or the
+// diagonal guard (fast, mixed).
+template <int kVariant, bool kMass, bool kSpecial, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_ens_rhs(EnsBufs b,
+                                                      const double* y_in,
+                                                      double* k_out,
+                                                      int stage0) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int slot = blockIdx.x * WPB + wave;
+  if (slot >= b.n_tiles) return;
+  const EnsTile tile = b.tiles[slot];
+  if (tile.sys < 0) return;
+  const SolverState* st = b.st ? b.st + tile.sys : nullptr;
+  if (st && st->done) return;
+  const EnsSys sy = b.sys[tile.sys];
+  const int n = sy.n;
+  const int tpl = n >= b.tpl2_min_n ? 2 : 1;
+  double* k = k_out + sy.eoff;
+  if (st && stage0 && st->k1_valid) {
+    // k1 of this attempt is the FSAL copy (or kept after a rejection)
+    if (st->k1_valid == 1) {
+      const int lane = threadIdx.x & 63;
+      const long lim = 3L * n;
+      const double* src = b.kfsal + sy.eoff;
+      for (int c = 0; c < 3 * tpl; ++c) {
+        const long e = 3L * tile.i0 + 64 * c + lane;
+        if (e < lim) {
+          k[e] = src[e];
+          k[lim + e] = src[lim + e];
+        }
+      }
+    }
+    return;
+  }
+  const double* y = (y_in ? y_in : (st->cur ? b.xb1 : b.xb0)) + sy.eoff;
+  const double* m = kMass ? b.mass + sy.eoff / 6 : nullptr;
+  if (kVariant == ODEHIP_NBODY_EXACT) {
+    if (tpl == 2)
+      exact_tile<kMass, kSpecial, 2>(y, m, k, n, tile.i0, b.eps2);
+    else
+      exact_tile<kMass, kSpecial, 1>(y, m, k, n, tile.i0, b.eps2);
+  } else if (kVariant == ODEHIP_NBODY_FAST) {
+    if (tpl == 2)
+      fast_tile<kMass, kSpecial, 2>(y, m, k, n, tile.i0, b.eps2);
+    else
+      fast_tile<kMass, kSpecial, 1>(y, m, k, n, tile.i0, b.eps2);
+  } else {
+    if (tpl == 2)
+      mixed_tile<kMass, kSpecial, 2>(y, m, k, n, tile.i0, b.eps2);
+    else
+      mixed_tile<kMass, kSpecial, 1>(y, m, k, n, tile.i0, b.eps2);
+  }
+}
+
+// ------------------------------------------------------ element kernels --
+
+struct GroupPos {
+  int sys;
+  bool valid;
+  size_t e;  // packed element index
+};
+
+__device__ inline GroupPos group_pos(const EnsBufs& b, const EnsSys& sy, int s) {
+  const long l = static_cast<long>(blockIdx.x - sy.gbase) * kBlock + threadIdx.x;
+  GroupPos p;
+  p.sys = s;
+  p.valid = l < 6L * sy.n;
+  p.e = static_cast<size_t>(sy.eoff + l);
+  return p;
+}
+
+// rk_stage_kernel (RKEmbeddedParallel.hpp:91-101) for every active system.
+__global__ __launch_bounds__(kBlock) void k_ens_stage(EnsBufs b, int stage) {
+  const int s = b.group_sys[blockIdx.x];
+  const SolverState* st = b.st + s;
+  if (st->done) return;
+  const EnsSys sy = b.sys[s];
+  const GroupPos p = group_pos(b, sy, s);
+  if (!p.valid) return;
+  const double* x0 = st->cur ? b.xb1 : b.xb0;
+  b.ytmp[p.e] = stage_value(e_tab[b.tableau], stage, st->dt, b.ks, x0, p.e, b.nv);
+}
+
+// The fused final pass of an attempt (RKEmbeddedParallel.hpp:47-55,
+// 103-137): new state, embedded error, error target, the scaled squared
+// error reduced into this group's partial, and the FSAL copy.
+__global__ __launch_bounds__(kBlock) void k_ens_final(EnsBufs b) {
+  const int s = b.group_sys[blockIdx.x];
+  const SolverState* st = b.st + s;
+  if (st->done) return;  // uniform over the workgroup
+  const EnsSys sy = b.sys[s];
+  const GroupPos p = group_pos(b, sy, s);
+  const Tableau& T = e_tab[b.tableau];
+  const int cur = st->cur;
+  const double* x0 = cur ? b.xb1 : b.xb0;
+  double* x = cur ? b.xb0 : b.xb1;
+  double term = 0.0;
+  if (p.valid) {
+    double xv, ev;
+    update_values(T, st->dt, b.ks, x0, p.e, b.nv, xv, ev);
+    x[p.e] = xv;
+    const double sc = b.atol + b.rtol * max_abs(xv, x0[p.e]);
+    const double scaled = ev / sc;
+    term = scaled * scaled;
+    if (st->fsal && T.fsal) b.kfsal[p.e] = b.ks[(T.S - 1) * b.nv + p.e];
+  }
+  const double g = group_reduce(term);
+  if (threadIdx.x == 0) b.partA[blockIdx.x] = g;
+}
+
+// Accept/reject and step size, one thread per system.
+__global__ void k_ens_controller(EnsBufs b) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= b.n_sys) return;
+  SolverState* st = b.st + s;
+  if (st->done) return;
+  const EnsSys sy = b.sys[s];
+  const double sum = fold_partials(b.partA + sy.gbase, sy.n_groups);
+  double* trace = b.trace ? b.trace + static_cast<size_t>(s) * b.trace_cap * 3 : nullptr;
+  ctrl_attempt(st, e_tab[st->tableau], sum, 6 * static_cast<size_t>(sy.n), nullptr,
+               trace);
+  if (st->done) atomicSub(b.n_active, 1);
+}
+
+__global__ void k_ens_init(EnsBufs b, double t0, double tf, long max_acc,
+                           long max_att, int fsal, int controller) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) *b.n_active = (t0 < tf) ? b.n_sys : 0;
+  if (s >= b.n_sys) return;
+  ctrl_init(b.st + s, e_tab[b.tableau], t0, tf, max_acc, max_att, fsal, b.tableau,
+            b.sys[s].n_groups, 0, b.trace ? b.trace_cap : 0, controller);
+}
+
+// compute_error_target_kernel (RKEmbeddedParallel.hpp:151-157).
+__global__ void k_ens_init_target(EnsBufs b) {
+  const size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < b.nv) b.sc0[e] = std::fabs(b.xb0[e]) * b.rtol + b.atol;
+}
+
+// Group partials of sum(((v - sub) / sc)^2) (rk_norm / difference,
+// RKEmbeddedParallel.hpp:132-149, 165-168).
+__global__ __launch_bounds__(kBlock) void k_ens_norm_partials(
+    EnsBufs b, const double* v, const double* sub, double* partials) {
+  const int s = b.group_sys[blockIdx.x];
+  if (b.st[s].done) return;
+  const EnsSys sy = b.sys[s];
+  const GroupPos p = group_pos(b, sy, s);
+  double term = 0.0;
+  if (p.valid) {
+    const double d = sub ? (v[p.e] - sub[p.e]) : v[p.e];
+    const double q = d / b.sc0[p.e];
+    term = q * q;
+  }
+  const double g = group_reduce(term);
+  if (threadIdx.x == 0) partials[blockIdx.x] = g;
+}
+
+__global__ void k_ens_init_step_a(EnsBufs b) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= b.n_sys) return;
+  SolverState* st = b.st + s;
+  if (st->done) return;
+  const EnsSys sy = b.sys[s];
+  const double sa = fold_partials(b.partA + sy.gbase, sy.n_groups);
+  const double sb = fold_partials(b.partB + sy.gbase, sy.n_groups);
+  ctrl_init_step_a(st, sa, sb, 6 * static_cast<size_t>(sy.n));
+}
+
+// euler_step_kernel (RKEmbeddedParallel.hpp:159-163).
+__global__ __launch_bounds__(kBlock) void k_ens_euler(EnsBufs b) {
+  const int s = b.group_sys[blockIdx.x];
+  const SolverState* st = b.st + s;
+  if (st->done) return;
+  const EnsSys sy = b.sys[s];
+  const GroupPos p = group_pos(b, sy, s);
+  if (p.valid) b.ytmp[p.e] = b.xb0[p.e] + b.ks[p.e] * st->dt0;
+}
+
+__global__ void k_ens_init_step_b(EnsBufs b) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= b.n_sys) return;
+  SolverState* st = b.st + s;
+  if (st->done) return;
+  const EnsSys sy = b.sys[s];
+  const double sa = fold_partials(b.partA + sy.gbase, sy.n_groups);
+  ctrl_init_step_b(st, e_tab[st->tableau], sa, 6 * static_cast<size_t>(sy.n));
+}
+
+__global__ __launch_bounds__(kBlock) void k_ens_copy_out(EnsBufs b,
+                                                         double* out) {
+  const int s = b.group_sys[blockIdx.x];
+  const EnsSys sy = b.sys[s];
+  const GroupPos p = group_pos(b, sy, s);
+  if (!p.valid) return;
+  out[p.e] = (b.st[s].cur ? b.xb1 : b.xb0)[p.e];
+}
+
+template <int kVariant, bool kMass, bool kSpecial>
+hipError_t launch_rhs_v(const EnsBufs& b, const double* y, double* k,
+                        int stage0, hipStream_t s) {
+  const unsigned blocks = static_cast<unsigned>(b.n_tiles / b.wpb);
+  if (blocks == 0) return hipSuccess;
+  if (b.wpb == 4) {
+    hipLaunchKernelGGL((k_ens_rhs<kVariant, kMass, kSpecial, 4>), dim3(blocks),
+                       dim3(256), 0, s, b, y, k, stage0);
+  } else if (b.wpb == 1) {
+    hipLaunchKernelGGL((k_ens_rhs<kVariant, kMass, kSpecial, 1>), dim3(blocks),
+                       dim3(64), 0, s, b, y, k, stage0);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int kVariant>
+hipError_t launch_rhs_m(const EnsBufs& b, const double* y, double* k,
+                        int stage0, hipStream_t s, bool special) {
+  if (b.mass)
+    return special ? launch_rhs_v<kVariant, true, true>(b, y, k, stage0, s)
+                   : launch_rhs_v<kVariant, true, false>(b, y, k, stage0, s);
+  return special ? launch_rhs_v<kVariant, false, true>(b, y, k, stage0, s)
+                 : launch_rhs_v<kVariant, false, false>(b, y, k, stage0, s);
+}
+
+inline unsigned groups_grid(const EnsBufs& b) {
+  return static_cast<unsigned>(b.n_groups);
+}
+inline unsigned sys_grid(const EnsBufs& b) {
+  return static_cast<unsigned>((b.n_sys + 63) / 64);
+}
+
+}  // namespace
+
+hipError_t ens_launch_rhs(const EnsBufs& b, const double* y, double* k,
+                          int stage0, hipStream_t s) {
+  switch (b.variant) {
+    case ODEHIP_NBODY_EXACT: {
+      const bool shared = b.eps2 >= 0x1p-200 && b.eps2 <= 0x1p100;
+      return launch_rhs_m<ODEHIP_NBODY_EXACT>(b, y, k, stage0, s, shared);
+    }
+    case ODEHIP_NBODY_FAST:
+      return launch_rhs_m<ODEHIP_NBODY_FAST>(b, y, k, stage0, s, !(b.eps2 >= 1e-150));
+    case ODEHIP_NBODY_MIXED:
+      return launch_rhs_m<ODEHIP_NBODY_MIXED>(b, y, k, stage0, s,
+                                              !(static_cast<float>(b.eps2) >= 1e-12f));
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t ens_enqueue_init(const EnsBufs& b, const double* x_in, double t0,
+                            double tf, long max_acc, long max_att, int fsal,
+                            int controller, hipStream_t s) {
+  hipError_t e = hipMemcpyAsync(b.xb0, x_in, b.nv * sizeof(double),
+                                hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_ens_init, dim3(sys_grid(b)), dim3(64), 0, s, b, t0, tf,
+                     max_acc, max_att, fsal, controller);
+  return hipGetLastError();
+}
+
+// estimate_initial_step (RKEmbeddedParallel.hpp:170-200) for every system.
+hipError_t ens_enqueue_initial_step(const EnsBufs& b, hipStream_t s) {
+  double* f0 = b.ks;
+  double* f1 = b.ks + b.nv;
+  const unsigned eblocks = static_cast<unsigned>((b.nv + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_ens_init_target, dim3(eblocks), dim3(kBlock), 0, s, b);
+  hipError_t e = ens_launch_rhs(b, b.xb0, f0, 0, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_ens_norm_partials, dim3(groups_grid(b)), dim3(kBlock), 0, s,
+                     b, b.xb0, nullptr, b.partA);
+  hipLaunchKernelGGL(k_ens_norm_partials, dim3(groups_grid(b)), dim3(kBlock), 0, s,
+                     b, f0, nullptr, b.partB);
+  hipLaunchKernelGGL(k_ens_init_step_a, dim3(sys_grid(b)), dim3(64), 0, s, b);
+  hipLaunchKernelGGL(k_ens_euler, dim3(groups_grid(b)), dim3(kBlock), 0, s, b);
+  if ((e = ens_launch_rhs(b, b.ytmp, f1, 0, s)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_ens_norm_partials, dim3(groups_grid(b)), dim3(kBlock), 0, s,
+                     b, f1, f0, b.partA);
+  hipLaunchKernelGGL(k_ens_init_step_b, dim3(sys_grid(b)), dim3(64), 0, s, b);
+  return hipGetLastError();
+}
+
+// One lockstep round: an attempt of every active system.
+hipError_t ens_enqueue_round(const EnsBufs& b, hipStream_t s) {
+  const Tableau& T = kTableaus[b.tableau];
+  hipError_t e = ens_launch_rhs(b, nullptr, b.ks, 1, s);
+  if (e != hipSuccess) return e;
+  for (int stage = 1; stage < T.S; ++stage) {
+    hipLaunchKernelGGL(k_ens_stage, dim3(groups_grid(b)), dim3(kBlock), 0, s, b,
+                       stage);
+    if ((e = ens_launch_rhs(b, b.ytmp, b.ks + stage * b.nv, 0, s)) != hipSuccess)
+      return e;
+  }
+  hipLaunchKernelGGL(k_ens_final, dim3(groups_grid(b)), dim3(kBlock), 0, s, b);
+  hipLaunchKernelGGL(k_ens_controller, dim3(sys_grid(b)), dim3(64), 0, s, b);
+  return hipGetLastError();
+}
+
+hipError_t ens_enqueue_copy_out(const EnsBufs& b, double* x_out,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(k_ens_copy_out, dim3(groups_grid(b)), dim3(kBlock), 0, s, b,
+                     x_out);
+  return hipGetLastError();
+}
+
+}  // namespace odehip

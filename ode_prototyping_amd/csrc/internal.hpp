@@ -21,9 +21,12 @@ struct SolverState {
   double safety;      // 0.38^(1/(1+q))
   double inv_q1;      // 1/(1+q)
   double dt0, d1;     // initial-step scratch
+  double err_prev;    // error of the last accepted step (PI controller)
+  double hist_err[3], hist_dt[3], hist_acc[3];  // last attempts, newest first
   long accepted, rejected, attempts, rhs_evals;
   long max_accepted, max_attempts;
   long times_cap, trace_cap;
+  long learned_steps;  // steps sized by the learned controller
   int cur;       // which ping-pong buffer holds x0
   int done;      // nonzero: every kernel returns immediately
   int status;    // ODEHIP_STATUS_*
@@ -31,7 +34,8 @@ struct SolverState {
   int fsal;      // FSAL / k1 reuse enabled
   int tableau;
   int n_groups;  // reduction groups of the error norm
-  int pad;
+  int controller;      // ODEHIP_CONTROLLER_*
+  int rejects_in_row;  // consecutive rejections up to now
 };
 
 // Arguments of the N-body RHS kernels.  Exactly one of y / (xb0, xb1, st)
@@ -113,6 +117,7 @@ struct SolveBufs {
   int n_particles;
   double eps2;
   const double* mass;
+  int controller;
 };
 
 hipError_t enqueue_solve_init(const SolveBufs& b, const double* x_in,

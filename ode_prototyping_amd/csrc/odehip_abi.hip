@@ -15,68 +15,11 @@
 #include <new>
 #include <string>
 
-#include "internal.hpp"
+#include "context.hpp"
 
 using namespace odehip;
 
-struct GraphKey {
-  int tableau = -1, rhs_kind = -1, variant = -1, k = 0, fsal = -1;
-  size_t n_var = 0;
-  double eps2 = 0.0;
-  const void* mass = nullptr;
-  const void* atol = nullptr;
-  const void* rtol = nullptr;
-  const void* ws = nullptr;
-  bool operator==(const GraphKey& o) const {
-    return tableau == o.tableau && rhs_kind == o.rhs_kind &&
-           variant == o.variant && k == o.k && fsal == o.fsal &&
-           n_var == o.n_var && std::memcmp(&eps2, &o.eps2, sizeof eps2) == 0 &&
-           mass == o.mass && atol == o.atol && rtol == o.rtol && ws == o.ws;
-  }
-};
-
-struct odehip_context {
-  int device = 0;
-  hipStream_t own = nullptr;
-  hipStream_t stream = nullptr;  // current stream (own or external)
-  std::string err;
-  // solve workspace
-  double* ws = nullptr;
-  size_t ws_cap = 0;
-  SolverState* st = nullptr;
-  // small scratch for reductions
-  double* scratch = nullptr;
-  size_t scratch_cap = 0;
-  // pinned host staging
-  int* h_done = nullptr;           // 2 slots
-  SolverState* h_st = nullptr;
-  double* h_scalar = nullptr;      // 8 doubles
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  hipEvent_t ev_in = nullptr, ev_out = nullptr;
-  hipEvent_t t_start = nullptr, t_stop = nullptr;  // timing events
-  // graph cache
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  GraphKey key;
-};
-
 namespace {
-
-int fail(odehip_context* ctx, int code, const std::string& msg) {
-  if (ctx) ctx->err = msg;
-  return code;
-}
-
-int hip_fail(odehip_context* ctx, hipError_t e, const char* where) {
-  return fail(ctx, ODEHIP_ERR_HIP,
-              std::string(where) + ": " + hipGetErrorString(e));
-}
-
-#define HIP_TRY(ctx, expr)                                   \
-  do {                                                       \
-    hipError_t _e = (expr);                                  \
-    if (_e != hipSuccess) return hip_fail((ctx), _e, #expr); \
-  } while (0)
 
 int ensure_scratch(odehip_context* ctx, size_t n) {
   if (ctx->scratch_cap >= n) return ODEHIP_OK;
@@ -178,23 +121,6 @@ int prepare_bufs(odehip_context* ctx, int tableau, const odehip_ode* ode,
   return ODEHIP_OK;
 }
 
-// Work stream for graph capture: the context's own stream, ordered after the
-// caller's stream when an external (possibly legacy default) stream is set.
-hipStream_t begin_work(odehip_context* ctx) {
-  if (ctx->stream != ctx->own) {
-    (void)hipEventRecord(ctx->ev_in, ctx->stream);
-    (void)hipStreamWaitEvent(ctx->own, ctx->ev_in, 0);
-  }
-  return ctx->own;
-}
-
-void end_work(odehip_context* ctx) {
-  if (ctx->stream != ctx->own) {
-    (void)hipEventRecord(ctx->ev_out, ctx->own);
-    (void)hipStreamWaitEvent(ctx->stream, ctx->ev_out, 0);
-  }
-}
-
 }  // namespace
 
 extern "C" {
@@ -234,6 +160,7 @@ int odehip_destroy(odehip_context* ctx) {
   if (ctx->own) (void)hipStreamSynchronize(ctx->own);
   if (ctx->exec) (void)hipGraphExecDestroy(ctx->exec);
   if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  if (ctx->ens) destroy_ensemble_workspace(ctx->ens);
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->st) (void)hipFree(ctx->st);
@@ -514,12 +441,19 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
   if (opt->max_accepted < 0 || opt->max_attempts < 0 || opt->times_cap < 0 ||
       opt->trace_cap < 0)
     return fail(ctx, ODEHIP_ERR_INVALID, "negative budget or capacity");
+  if (opt->controller < ODEHIP_CONTROLLER_REFERENCE ||
+      opt->controller > ODEHIP_CONTROLLER_LEARNED)
+    return fail(ctx, ODEHIP_ERR_INVALID, "bad controller");
+  if (opt->controller == ODEHIP_CONTROLLER_LEARNED)
+    return fail(ctx, ODEHIP_ERR_UNSUPPORTED,
+                "the learned controller runs in the ensemble solve");
   const long times_cap = opt->times ? opt->times_cap : 0;
   const long trace_cap = opt->trace ? opt->trace_cap : 0;
   SolveBufs b{};
   rc = prepare_bufs(ctx, tableau, ode, opt->atol, opt->rtol, times_cap,
                     trace_cap, &b);
   if (rc) return rc;
+  b.controller = opt->controller;
   const int S = kTableaus[tableau].S;
   const size_t n = ode->n_var;
 
@@ -623,6 +557,7 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
     stats->dt = hs.dt;
     stats->dt_initial = hs.dt_initial;
     stats->last_err = hs.err;
+    stats->learned_steps = hs.learned_steps;
   }
   if (hs.status == ODEHIP_STATUS_RUNNING)
     return fail(ctx, ODEHIP_ERR_INVALID, "solve did not finish within the attempt cap");

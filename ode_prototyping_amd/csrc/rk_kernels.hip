@@ -7,8 +7,7 @@
 // reference's.
 #include <cmath>
 
-#include "crpow.hpp"
-#include "internal.hpp"
+#include "rk_device.hpp"
 
 namespace odehip {
 namespace {
@@ -21,52 +20,6 @@ constexpr int kBlock = 256;
 
 __device__ inline size_t gid() {
   return static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-}
-
-__device__ inline double max_abs(double a, double b) {
-  // std::max(std::abs(x), std::abs(x0)): the first argument unless it is
-  // less than the second.
-  const double fa = std::fabs(a), fb = std::fabs(b);
-  return (fa < fb) ? fb : fa;
-}
-
-// Canonical group reduction (odehip.h, ODEHIP_REDUCE_GROUP): 256 threads; each
-// 64-lane quad reduced with halving xor strides (lane 0 holds
-// v0 + v32, + (v16 + v48), ...), then (q0 + q1) + (q2 + q3).
-__device__ inline double group_reduce(double v) {
-  __shared__ double q[4];
-#pragma unroll
-  for (int s = 32; s >= 1; s >>= 1) v = v + __shfl_xor(v, s, 64);
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  if (lane == 0) q[wave] = v;
-  __syncthreads();
-  return (q[0] + q[1]) + (q[2] + q[3]);
-}
-
-// Sequential fold of group partials from 0.0 (the last level of the
-// canonical order).
-__device__ inline double fold_partials(const double* p, int n_groups) {
-  double r = 0.0;
-  for (int g = 0; g < n_groups; ++g) r = r + p[g];
-  return r;
-}
-
-// The same fold, called by every thread of a 64-thread block: the partials
-// are first staged through LDS by all lanes (coalesced), then lane 0 adds
-// them in order from LDS.  Returns the sum in lane 0.
-constexpr int kFoldLds = 4096;
-__device__ inline double fold_partials_block(const double* p, int n_groups) {
-  __shared__ double buf[kFoldLds];
-  if (n_groups > kFoldLds) return threadIdx.x == 0 ? fold_partials(p, n_groups) : 0.0;
-  for (int g = threadIdx.x; g < n_groups; g += blockDim.x) buf[g] = p[g];
-  __syncthreads();
-  double r = 0.0;
-  if (threadIdx.x == 0) {
-#pragma unroll 8
-    for (int g = 0; g < n_groups; ++g) r = r + buf[g];
-  }
-  return r;
 }
 
 // ----------------------------------------------------------- simple RHS ----
@@ -103,15 +56,6 @@ __global__ void k_vdp_rhs(const double* y, const double* xb0, const double* xb1,
 }
 
 // ------------------------------------------------------- stage kernels -----
-// detail::rk_stage_kernel (RKEmbeddedParallel.hpp:91-101).
-__device__ inline double stage_value(const Tableau& T, int stage, double dt,
-                                     const double* ks, const double* x0,
-                                     size_t i, size_t n_var) {
-  double tmp = 0.0;
-  for (int j = 0; j < stage; ++j) tmp += T.a[stage - 1][j] * ks[j * n_var + i];
-  return x0[i] + tmp * dt;
-}
-
 __global__ void k_stage_host_dt(int tab, int stage, double dt, double* temp,
                                 const double* ks, const double* x0,
                                 size_t n_var) {
@@ -126,21 +70,6 @@ __global__ void k_stage_dev(const SolverState* st, int tab, int stage,
   const size_t i = gid();
   const double* x0 = st->cur ? xb1 : xb0;
   if (i < n_var) temp[i] = stage_value(c_tab[tab], stage, st->dt, ks, x0, i, n_var);
-}
-
-// detail::update_state_and_error_kernel (RKEmbeddedParallel.hpp:103-123).
-__device__ inline void update_values(const Tableau& T, double dt,
-                                     const double* ks, const double* x0,
-                                     size_t i, size_t n_var, double& x,
-                                     double& err) {
-  double xs = 0.0, es = 0.0;
-  for (int j = 0; j < T.S; ++j) {
-    const double kj = ks[j * n_var + i];
-    xs += T.b[j] * kj;
-    es += T.db[j] * kj;
-  }
-  x = x0[i] + xs * dt;
-  err = es * dt;
 }
 
 __global__ void k_update_host_dt(int tab, double dt, double* x, double* err,
@@ -197,101 +126,17 @@ __global__ void k_controller(SolverState* st, const double* partials,
   if (st->done) return;
   const double sum = fold_partials_block(partials, st->n_groups);
   if (threadIdx.x != 0) return;
-  const Tableau& T = c_tab[st->tableau];
-  const double scaled_error =
-      std::sqrt(sum / static_cast<double>(static_cast<long>(n_var)));
-  const long attempt = st->attempts;
-  st->attempts = attempt + 1;
-  st->rhs_evals += st->k1_valid ? (T.S - 1) : T.S;
-  st->err = scaled_error;
-  double t = st->t;
-  double dt = st->dt;
-  const bool finite = std::isfinite(scaled_error);
-  const bool accept = finite && scaled_error <= 1.0;
-  if (trace && attempt < st->trace_cap) {
-    trace[3 * attempt] = scaled_error;
-    trace[3 * attempt + 1] = dt;
-    trace[3 * attempt + 2] = accept ? 1.0 : 0.0;
-  }
-  if (!finite) {
-    st->rejected += 1;
-    st->status = ODEHIP_STATUS_NONFINITE;
-    st->done = 1;
-    return;
-  }
-  if (accept) {
-    t += dt;
-    st->cur ^= 1;  // x0 = x (the ping-pong buffers swap roles)
-    const long acc = st->accepted + 1;
-    st->accepted = acc;
-    if (times && acc < st->times_cap) times[acc] = t;
-    st->k1_valid = (st->fsal && T.fsal) ? 1 : 0;
-  } else {
-    st->rejected += 1;
-    st->k1_valid = st->fsal ? 2 : 0;
-  }
-  const double max_step_scale = 6.0;
-  const double min_step_scale = 0.33;
-  const double tf = st->tf, t0 = st->t0;
-  const double dtnew = dt * st->safety / cr_pow(scaled_error, st->inv_q1);
-  if (std::fabs(dtnew) > max_step_scale * std::fabs(dt)) {
-    dt *= max_step_scale;
-  } else if (std::fabs(dtnew) < min_step_scale * std::fabs(dt)) {
-    dt *= min_step_scale;
-  } else if (dtnew / (tf - t0) < 1.0e-12) {
-    dt = (tf - t0) * 1.0e-12;
-  } else {
-    dt = dtnew;
-  }
-  if (t + dt > tf) dt = tf - t;
-  st->t = t;
-  st->dt = dt;
-  if (!(t < tf)) {
-    st->status = ODEHIP_STATUS_REACHED_END;
-    st->done = 1;
-  } else if (accept && st->max_accepted > 0 && st->accepted >= st->max_accepted) {
-    st->status = ODEHIP_STATUS_MAX_ACCEPTED;
-    st->done = 1;
-  } else if (st->max_attempts > 0 && st->attempts >= st->max_attempts) {
-    st->status = ODEHIP_STATUS_MAX_ATTEMPTS;
-    st->done = 1;
-  }
+  ctrl_attempt(st, c_tab[st->tableau], sum, n_var, times, trace);
 }
 
 // ------------------------------------------------------ initial step -------
 __global__ void k_init_state(SolverState* st, double t0, double tf,
                              long max_acc, long max_att, int fsal, int tab,
                              int n_groups, double* times, long times_cap,
-                             long trace_cap) {
+                             long trace_cap, int controller) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const Tableau& T = c_tab[tab];
-  const int q = T.p < T.pt ? T.p : T.pt;
-  st->t = t0;
-  st->t0 = t0;
-  st->tf = tf;
-  st->dt = 0.0;
-  st->err = 0.0;
-  st->dt_initial = 0.0;
-  st->inv_q1 = 1.0 / (1.0 + q);
-  st->safety = cr_pow(0.38, (1.0 / (1.0 + q)));
-  st->dt0 = 0.0;
-  st->d1 = 0.0;
-  st->accepted = 0;
-  st->rejected = 0;
-  st->attempts = 0;
-  st->rhs_evals = 0;
-  st->max_accepted = max_acc;
-  st->max_attempts = max_att;
-  st->times_cap = times_cap;
-  st->trace_cap = trace_cap;
-  st->cur = 0;
-  st->fsal = fsal;
-  st->tableau = tab;
-  st->n_groups = n_groups;
-  st->k1_valid = 0;
-  st->status = ODEHIP_STATUS_RUNNING;
-  st->done = (t0 < tf) ? 0 : 1;
-  if (st->done) st->status = ODEHIP_STATUS_REACHED_END;
+  ctrl_init(st, c_tab[tab], t0, tf, max_acc, max_att, fsal, tab, n_groups,
+            times_cap, trace_cap, controller);
   if (times && times_cap > 0) times[0] = t0;
 }
 
@@ -337,12 +182,7 @@ __global__ void k_init_step_a(SolverState* st, const double* pa,
   __syncthreads();
   const double sb = fold_partials_block(pb, st->n_groups);
   if (threadIdx.x != 0) return;
-  const double nv = static_cast<double>(static_cast<long>(n_var));
-  const double d0 = std::sqrt(sa / nv);
-  const double d1 = std::sqrt(sb / nv);
-  const double dt0 = (d0 < 1.0e-5 or d1 < 1.0e-5) ? 1.0e-6 : 0.01 * (d0 / d1);
-  st->dt0 = dt0;
-  st->d1 = d1;
+  ctrl_init_step_a(st, sa, sb, n_var);
 }
 
 // Host-dt forms of the initial-step kernels (:151-168).
@@ -368,21 +208,7 @@ __global__ void k_euler(const SolverState* st, double* x1, const double* x0,
 __global__ void k_init_step_b(SolverState* st, const double* pa, size_t n_var) {
   const double sa = fold_partials_block(pa, st->n_groups);
   if (threadIdx.x != 0) return;
-  const Tableau& T = c_tab[st->tableau];
-  const double nv = static_cast<double>(static_cast<long>(n_var));
-  const double dt0 = st->dt0;
-  const double d1 = st->d1;
-  const double d2 = std::sqrt(sa / nv) / dt0;
-  const int p = T.p;
-  const double m = (d1 < d2) ? d2 : d1;
-  const double hi = 1.0e-6, lo = dt0 * 1.0e-3;
-  const double dt1 = (m <= 1.0e-15) ? ((hi < lo) ? lo : hi)
-                                    : cr_pow(0.01 / m, (1.0 / (1.0 + p)));
-  const double cap = 100.0 * dt0;
-  const double dt = (dt1 < cap) ? dt1 : cap;
-  st->dt = dt;
-  st->dt_initial = dt;
-  st->rhs_evals += 2;
+  ctrl_init_step_b(st, c_tab[st->tableau], sa, n_var);
 }
 
 // Final state out: x_out = the x0 buffer.
@@ -538,7 +364,7 @@ hipError_t enqueue_solve_init(const SolveBufs& b, const double* x_in,
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_init_state, dim3(1), dim3(64), 0, s, b.st, t0, tf,
                      max_acc, max_att, fsal, b.tableau, n_groups_for(b.n_var),
-                     b.times, b.times_cap, b.trace_cap);
+                     b.times, b.times_cap, b.trace_cap, b.controller);
   return hipGetLastError();
 }
 

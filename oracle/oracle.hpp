@@ -232,7 +232,16 @@ enum Status : int {
   kMaxAttempts = 3,    // attempt budget exhausted
 };
 
+// Step-size controllers: the reference's (RKEmbeddedParallel.hpp:64-74) and
+// a PI controller (Gustafsson's PI.3.4; Hairer & Wanner, Solving ODEs II,
+// IV.2): dt * (0.38/err)^(0.3/k) * (err_prev/err)^(0.4/k), k = 1 + min(p, pt)
+// -- the same steady-state error 0.38 as the reference's controller --,
+// err_prev the error of the last accepted step (1e-4 at first, floored at
+// 1e-4), no growth right after a rejection; same clamps, floor and clip.
+enum class Controller { Reference = 0, PI = 1 };
+
 struct Options {
+  Controller controller = Controller::Reference;
   long max_accepted = 0;  // 0: unlimited
   long max_attempts = 0;  // 0: unlimited
   PowMode pow_mode = PowMode::Libm;
@@ -326,6 +335,8 @@ inline Result integrate(const Tableau& bt, const double* x0_in, size_t n_var,
   };
   save(t, x);
   long attempts = 0;
+  double err_prev = 1.0e-4;
+  const double inv_q1 = 1.0 / (1.0 + q);
   while (t < tf) {
     if (opt.max_attempts > 0 && attempts >= opt.max_attempts) {
       res.status = kMaxAttempts;
@@ -385,8 +396,16 @@ inline Result integrate(const Tableau& bt, const double* x0_in, size_t n_var,
       ++res.rejected;
     }
     // controller (:65-74)
-    double dtnew =
-        dt * safety_factor / opow(scaled_error, 1.0 / (1.0 + q), opt.pow_mode);
+    double dtnew;
+    if (opt.controller == Controller::PI) {
+      double fac = opow(0.38 / scaled_error, 0.3 * inv_q1, opt.pow_mode) *
+                   opow(err_prev / scaled_error, 0.4 * inv_q1, opt.pow_mode);
+      if (!accept && fac > 1.0) fac = 1.0;
+      dtnew = dt * fac;
+    } else {
+      dtnew = dt * safety_factor / opow(scaled_error, 1.0 / (1.0 + q), opt.pow_mode);
+    }
+    if (accept) err_prev = scaled_error < 1.0e-4 ? 1.0e-4 : scaled_error;
     if (std::abs(dtnew) > max_step_scale * std::abs(dt)) {
       dt *= max_step_scale;
     } else if (std::abs(dtnew) < min_step_scale * std::abs(dt)) {

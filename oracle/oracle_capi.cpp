@@ -69,11 +69,13 @@ void* oracle_integrate(const char* tableau, int rhs_kind, const double* x0,
                        const double* rtol, double softening,
                        const double* masses, int reduce_kind, long reduce_param,
                        int pow_mode, long max_accepted, long max_attempts,
-                       int store_states, int store_trace, int threads) {
+                       int store_states, int store_trace, int threads,
+                       int controller) {
   try {
     Tableau bt = tableau_by_name(tableau);
     Executor e = make_exe(reduce_kind, reduce_param, threads);
     Options o;
+    o.controller = static_cast<Controller>(controller);
     o.max_accepted = max_accepted;
     o.max_attempts = max_attempts;
     o.pow_mode = static_cast<PowMode>(pow_mode);

@@ -14,6 +14,7 @@ RHS_NBODY, RHS_EXPONENTIAL, RHS_VANDERPOL = 0, 1, 2
 REDUCE_SEQUENTIAL, REDUCE_CHUNKED, REDUCE_THREADPOOL, REDUCE_HIP = 0, 1, 2, 3
 HIP_ORDER = (REDUCE_HIP, 0)
 POW_LIBM, POW_CORRECT = 0, 1
+CONTROLLER_REFERENCE, CONTROLLER_PI = 0, 1
 STATUS_REACHED_END, STATUS_MAX_ACCEPTED, STATUS_NONFINITE, STATUS_MAX_ATTEMPTS = 0, 1, 2, 3
 
 _dp = C.POINTER(C.c_double)
@@ -37,7 +38,7 @@ def lib():
         L.oracle_rk_norm.restype = C.c_double
         L.oracle_integrate.argtypes = [C.c_char_p, C.c_int, _dp, C.c_size_t, C.c_double, C.c_double,
                                        _dp, _dp, C.c_double, _dp, C.c_int, C.c_long, C.c_int,
-                                       C.c_long, C.c_long, C.c_int, C.c_int, C.c_int]
+                                       C.c_long, C.c_long, C.c_int, C.c_int, C.c_int, C.c_int]
         L.oracle_integrate.restype = C.c_void_p
         L.oracle_result_count.argtypes = [C.c_void_p]
         L.oracle_result_count.restype = C.c_long
@@ -104,7 +105,8 @@ class Result:
 
 def integrate(tableau, rhs_kind, x0, t0, tf, atol, rtol, softening=0.0, masses=None,
               reduce=(REDUCE_SEQUENTIAL, 0), pow_mode=POW_LIBM, max_accepted=0,
-              max_attempts=0, store_states=True, store_trace=False, threads=1):
+              max_attempts=0, store_states=True, store_trace=False, threads=1,
+              controller=CONTROLLER_REFERENCE):
     x0 = np.ascontiguousarray(x0, dtype=np.float64)
     n = x0.size
     atol = np.ascontiguousarray(np.broadcast_to(np.asarray(atol, dtype=np.float64), (n,)))
@@ -113,7 +115,7 @@ def integrate(tableau, rhs_kind, x0, t0, tf, atol, rtol, softening=0.0, masses=N
     h = lib().oracle_integrate(tableau.encode(), rhs_kind, _ptr(x0), n, t0, tf, _ptr(atol),
                                _ptr(rtol), softening, _ptr(m), reduce[0], reduce[1], pow_mode,
                                max_accepted, max_attempts, int(store_states), int(store_trace),
-                               threads)
+                               threads, controller)
     if not h:
         raise RuntimeError("oracle_integrate failed")
     return Result(h, n)

@@ -1,0 +1,226 @@
+"""Batched ensemble solve and right-hand side against the CPU oracle.
+
+Every system of an ensemble must come out bit for bit as the oracle (HIP
+reduction order, correctly rounded pow) integrates it alone: final state,
+counters, per-attempt trace; and as the single-system device solve does.
+The ensemble right-hand side in the exact variant must equal the oracle's
+(the reference's single-threaded pair loop) bit for bit in every tile
+configuration; the fast and mixed variants hold the documented tolerances.
+Sizes cover 1, 2 and 3 bodies, sizes around the 64- and 128-target tile
+edges and the partner-group edges of the kernels.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [1, 2, 3, 5, 16, 37, 63, 64, 65, 100, 127, 128, 129, 200, 257]
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def make_ensemble(sizes, seed0=0):
+    import ode_prototyping_amd as oh
+    xs, ms = [], []
+    for k, n in enumerate(sizes):
+        x, m = oh.scenarios.uniform_cube(int(n), seed=seed0 + k)
+        xs.append(x)
+        ms.append(m)
+    return xs, ms
+
+
+@pytest.fixture(scope="module")
+def ens():
+    return make_ensemble(SIZES)
+
+
+@pytest.mark.parametrize("cfg", list(range(12)))
+@pytest.mark.parametrize("with_mass", [True, False])
+def test_ensemble_rhs_exact_bitwise(exe, ens, cfg, with_mass):
+    from ode_prototyping_amd.ensemble import ensemble_rhs, pack, unpack
+    xs, ms = ens
+    masses = pack(ms) if with_mass else None
+    d, ms_ = ensemble_rhs(exe, SIZES, pack(xs), 1e-3, masses, "exact", tile_config=cfg)
+    for k, (x, part) in enumerate(zip(xs, unpack(d, SIZES))):
+        ref = O.rhs(O.RHS_NBODY, x, 1e-3, ms[k] if with_mass else None)
+        assert np.array_equal(bits(part), bits(ref)), (cfg, SIZES[k])
+    assert ms_ > 0.0
+
+
+def test_ensemble_rhs_exact_zero_softening(exe):
+    """eps = 0: the IEEE term path (no shared reciprocal) and the skipped
+    diagonal (0/0 would be NaN)."""
+    from ode_prototyping_amd.ensemble import ensemble_rhs, pack, unpack
+    sizes = [7, 70, 130]
+    xs, ms = make_ensemble(sizes, seed0=50)
+    d, _ = ensemble_rhs(exe, sizes, pack(xs), 0.0, pack(ms), "exact", tile_config=2)
+    for k, part in enumerate(unpack(d, sizes)):
+        ref = O.rhs(O.RHS_NBODY, xs[k], 0.0, ms[k])
+        assert np.array_equal(bits(part), bits(ref))
+
+
+@pytest.mark.parametrize("variant,tol", [("fast", 2e-14), ("mixed", 5e-6)])
+@pytest.mark.parametrize("cfg", [0, 5, 10])
+def test_ensemble_rhs_fast_mixed(exe, ens, variant, tol, cfg):
+    """Relative error of every acceleration against the exact oracle, scaled
+    by the sum of the absolute pair terms (the conditioning of the sum)."""
+    from ode_prototyping_amd.ensemble import ensemble_rhs, pack, unpack
+    xs, ms = ens
+    d, _ = ensemble_rhs(exe, SIZES, pack(xs), 1e-3, pack(ms), variant, tile_config=cfg)
+    for k, part in enumerate(unpack(d, SIZES)):
+        n = SIZES[k]
+        ref = O.rhs(O.RHS_NBODY, xs[k], 1e-3, ms[k])
+        assert np.array_equal(part[:3 * n], ref[:3 * n])  # velocities copied
+        pos = xs[k][:3 * n].reshape(n, 3)
+        r = pos[None, :, :] - pos[:, None, :]
+        w = ms[k][None, :] / (np.sum(r * r, axis=2) + 1e-6) ** 1.5
+        scale = np.sum(np.abs(r) * w[:, :, None], axis=1).ravel() + 1e-300
+        err = np.abs(part[3 * n:] - ref[3 * n:]) / scale
+        assert err.max() < tol, (variant, n, err.max())
+
+
+def oracle_solve(tableau, x, tf, tol, eps, m, **kw):
+    return O.integrate(tableau, O.RHS_NBODY, x, 0.0, tf, tol, tol, softening=eps, masses=m,
+                       reduce=O.HIP_ORDER, pow_mode=O.POW_CORRECT, store_states=False,
+                       store_trace=True, threads=8, **kw)
+
+
+def assert_system_matches(res, k, ref, x_final, trace=True):
+    assert (res.accepted[k], res.rejected[k], res.rhs_evals[k]) == (
+        ref.accepted, ref.rejected, ref.rhs_evals), k
+    assert res.t[k] == ref.t and res.dt[k] == ref.dt and res.dt_initial[k] == ref.dt_initial
+    assert np.array_equal(bits(x_final), bits(ref.x_final)), k
+    if trace:
+        tr = res.trace[k]
+        assert np.array_equal(bits(tr[:, 0]), bits(ref.trace_err))
+        assert np.array_equal(bits(tr[:, 1]), bits(ref.trace_dt))
+        assert np.array_equal(tr[:, 2].astype(int), ref.trace_acc)
+
+
+@pytest.mark.parametrize("tableau,tf,cfg", [("DOPRI5", 0.05, -1), ("DOP853", 0.03, 7),
+                                            ("RKF45", 0.02, 3)])
+def test_ensemble_solve_matches_oracle(exe, ens, tableau, tf, cfg):
+    from ode_prototyping_amd.ensemble import pack, solve_ensemble
+    xs, ms = ens
+    res = solve_ensemble(tableau, SIZES, pack(xs), 0.0, tf, 1e-10, 1e-10, 1e-3, pack(ms),
+                         variant="exact", exe=exe, tile_config=cfg, trace_cap=4096)
+    assert res.n_unfinished == 0
+    finals = res.states()
+    for k, n in enumerate(SIZES):
+        ref = oracle_solve(tableau, xs[k], tf, 1e-10, 1e-3, ms[k])
+        assert_system_matches(res, k, ref, finals[k])
+    assert res.total_accepted == res.accepted.sum()
+    assert res.particle_steps == float(np.sum(np.asarray(SIZES) * res.accepted))
+    assert res.rounds == res.attempts.max()
+
+
+def test_ensemble_matches_single_solves_fsal(exe, ens):
+    """FSAL on, max_accepted budget: each system as odehip_rk_solve alone."""
+    import ode_prototyping_amd as oh
+    from ode_prototyping_amd.ensemble import pack, solve_ensemble
+    xs, ms = ens
+    res = solve_ensemble("DOPRI5", SIZES, pack(xs), 0.0, 1.0, 1e-10, 1e-10, 1e-3, pack(ms),
+                         variant="exact", exe=exe, fsal=True, max_accepted=25,
+                         rounds_per_launch=3)
+    finals = res.states()
+    for k, n in enumerate(SIZES):
+        one = oh.integrate("DOPRI5", xs[k], 0.0, 1.0, 1e-10, 1e-10,
+                           oh.NBodyODE(1e-3, ms[k], "exact"), exe=exe, fsal=True,
+                           max_accepted=25)
+        assert (res.accepted[k], res.rejected[k], res.rhs_evals[k], res.status_names[k]) == (
+            one.accepted, one.rejected, one.rhs_evals, one.status)
+        assert res.t[k] == one.t and res.dt[k] == one.dt
+        assert np.array_equal(bits(finals[k]), bits(one.x_final))
+    assert set(res.status_names) <= {"max_accepted", "reached_end"}
+    assert np.all(res.accepted <= 25)
+
+
+@pytest.mark.parametrize("variant", ["fast", "mixed"])
+def test_ensemble_matches_single_solves_variants(exe, variant):
+    """The fast and mixed right-hand sides of the ensemble sum partners in
+    increasing j per target like the single-system kernels' waves, but the
+    single-system kernels split j over 16 waves: compare at fixed time."""
+    import ode_prototyping_amd as oh
+    from ode_prototyping_amd.ensemble import pack, solve_ensemble
+    sizes = [64, 200, 333]
+    xs, ms = make_ensemble(sizes, seed0=7)
+    res = solve_ensemble("DOPRI5", sizes, pack(xs), 0.0, 0.05, 1e-10, 1e-10, 1e-3, pack(ms),
+                         variant=variant, exe=exe)
+    tol = 1e-11 if variant == "fast" else 1e-6
+    for k, n in enumerate(sizes):
+        ref = oracle_solve("DOPRI5", xs[k], 0.05, 1e-10, 1e-3, ms[k])
+        assert res.t[k] == 0.05
+        err = np.abs(res.states()[k] - ref.x_final).max() / np.abs(ref.x_final).max()
+        assert err < tol, (variant, n, err)
+
+
+def test_ensemble_pi_controller_matches_oracle(exe):
+    from ode_prototyping_amd.ensemble import pack, solve_ensemble
+    sizes = [16, 65, 150]
+    xs, ms = make_ensemble(sizes, seed0=21)
+    res = solve_ensemble("DOPRI5", sizes, pack(xs), 0.0, 0.05, 1e-10, 1e-10, 1e-3, pack(ms),
+                         exe=exe, controller="pi", trace_cap=4096)
+    finals = res.states()
+    for k in range(len(sizes)):
+        ref = oracle_solve("DOPRI5", xs[k], 0.05, 1e-10, 1e-3, ms[k],
+                           controller=O.CONTROLLER_PI)
+        assert_system_matches(res, k, ref, finals[k])
+
+
+def test_single_solve_pi_controller_matches_oracle(exe):
+    import ode_prototyping_amd as oh
+    x, m = oh.scenarios.plummer(300, seed=3)
+    res = oh.integrate("DOP853", x, 0.0, 0.5, 1e-10, 1e-10, oh.NBodyODE(1e-2, m, "exact"),
+                       exe=exe, controller="pi", trace_cap=4096)
+    ref = O.integrate("DOP853", O.RHS_NBODY, x, 0.0, 0.5, 1e-10, 1e-10, softening=1e-2,
+                      masses=m, reduce=O.HIP_ORDER, pow_mode=O.POW_CORRECT,
+                      store_states=False, store_trace=True, threads=8,
+                      controller=O.CONTROLLER_PI)
+    assert (res.accepted, res.rejected, res.rhs_evals) == (ref.accepted, ref.rejected,
+                                                           ref.rhs_evals)
+    assert np.array_equal(bits(res.x_final), bits(ref.x_final))
+    assert np.array_equal(bits(res.trace[:, 1]), bits(ref.trace_dt))
+
+
+def test_ensemble_edge_cases(exe):
+    from ode_prototyping_amd.ensemble import pack, solve_ensemble
+    # one system; and t0 == tf (nothing to do: the state comes back as given)
+    xs, ms = make_ensemble([40], seed0=99)
+    res = solve_ensemble("DOPRI5", [40], pack(xs), 0.0, 0.01, 1e-10, 1e-10, 1e-3, pack(ms),
+                         exe=exe)
+    ref = oracle_solve("DOPRI5", xs[0], 0.01, 1e-10, 1e-3, ms[0])
+    assert np.array_equal(bits(res.x_final), bits(ref.x_final))
+    xs, ms = make_ensemble([5, 9], seed0=3)
+    res = solve_ensemble("DOPRI5", [5, 9], pack(xs), 1.0, 1.0, 1e-10, 1e-10, 1e-3, pack(ms),
+                         exe=exe)
+    assert np.array_equal(bits(res.x_final), bits(pack(xs)))
+    assert res.total_attempts == 0 and res.status_names == ["reached_end"] * 2
+
+
+def test_ensemble_autotune_reports_all_candidates(exe):
+    from ode_prototyping_amd.ensemble import pack, solve_ensemble
+    sizes = [300, 20, 1000, 64, 513]
+    xs, ms = make_ensemble(sizes, seed0=11)
+    res = solve_ensemble("DOPRI5", sizes, pack(xs), 0.0, 1e-3, 1e-10, 1e-10, 1e-3, pack(ms),
+                         variant="fast", exe=exe)
+    assert np.all(res.tune_ms > 0.0)
+    assert res.tile_config == int(np.argmin(res.tune_ms))
+    assert res.rhs_ms == res.tune_ms[res.tile_config]
+
+
+def test_ensemble_rejects_bad_arguments(exe):
+    from ode_prototyping_amd.core import OdeHipError
+    from ode_prototyping_amd.ensemble import ensemble_rhs, solve_ensemble
+    with pytest.raises(OdeHipError):
+        ensemble_rhs(exe, [4, 0], np.zeros(24), 1e-3)
+    with pytest.raises(OdeHipError):
+        ensemble_rhs(exe, [4], np.zeros(24), 1e-3, tile_config=12)
+    with pytest.raises(ValueError):
+        ensemble_rhs(exe, [4], np.zeros(23), 1e-3)
+    with pytest.raises(OdeHipError):
+        solve_ensemble("DOPRI5", [4], np.zeros(24), 0.0, 1.0, 1e-8, 1e-8, 1e-3,
+                       controller="learned", exe=exe)

@@ -140,3 +140,45 @@ def test_pow_modes():
         assert abs(a - b) <= np.spacing(b)
         diff += a != b
     assert diff < 60  # glibc is correctly rounded ~99.9 % of the time
+
+
+def test_pi_controller_follows_its_formula():
+    """The PI controller (no reference counterpart): every step size of a run
+    follows dt' = dt * (0.38/err)^(0.3/k) * (err_prev/err)^(0.4/k) (k = 6 for
+    DOP853; no growth after a rejection; clamps [0.33, 6]) with err_prev the
+    error of the last accepted attempt, recomputed here from the run's own
+    trace with numpy's pow.  A wrong history (err_prev from a rejected
+    attempt), swapped exponents or a missing post-rejection cap fail it."""
+    x = np.array([2.0, 0.0])
+    r = O.integrate("DOP853", O.RHS_VANDERPOL, x, 0.0, 10.0, 1e-8, 1e-8,
+                    store_states=False, store_trace=True, controller=O.CONTROLLER_PI)
+    err, dt, acc = np.array(r.trace_err), np.array(r.trace_dt), np.array(r.trace_acc)
+    assert acc.sum() == r.accepted and (1 - acc).sum() == r.rejected > 0
+    k = 6.0
+    err_prev = 1e-4
+    t = 0.0
+    for a in range(len(err) - 1):
+        fac = (0.38 / err[a]) ** (0.3 / k) * (err_prev / err[a]) ** (0.4 / k)
+        if not acc[a]:
+            fac = min(fac, 1.0)
+        want = dt[a] * min(max(fac, 0.33), 6.0)
+        if acc[a]:
+            t += dt[a]
+            err_prev = max(err[a], 1e-4)
+        if t + want > 10.0:
+            want = 10.0 - t
+        assert abs(dt[a + 1] - want) <= 1e-13 * want, a
+
+
+def test_pi_controller_keeps_accuracy():
+    """Only the step sizes change: on x' = x (closed form e^t) the PI run's
+    final error stays at the tolerance level like the reference controller's,
+    and every accepted attempt had an error norm <= 1."""
+    x = np.array([1.0, 0.5, 2.0])
+    for name in ("DOPRI5", "DOP853", "RKF78"):
+        r = O.integrate(name, O.RHS_EXPONENTIAL, x, 0.0, 5.0, 1e-10, 1e-10,
+                        store_states=False, store_trace=True, controller=O.CONTROLLER_PI)
+        exact = x * np.exp(5.0)
+        assert np.max(np.abs(r.x_final - exact) / exact) < 1e-8, name
+        err, acc = np.array(r.trace_err), np.array(r.trace_acc)
+        assert np.all(err[acc == 1] <= 1.0) and np.all(err[acc == 0] > 1.0)

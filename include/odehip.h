@@ -243,6 +243,60 @@ int odehip_nbody_energy(odehip_context* ctx, const double* x, size_t n,
 int odehip_nbody_momentum(odehip_context* ctx, const double* x, size_t n,
                           const double* masses, double* host_out);
 
+/* ------------------------------------------------- learned step controller
+ * A small network proposes the next step ratio (no reference counterpart;
+ * the reference's controller is RKEmbeddedParallel.hpp:64-79, which stays
+ * the judge: a step is accepted exactly when its error norm is <= 1; the
+ * proposed ratio is capped at 1 right after a rejection, the reference's
+ * clamps and clip to tf apply, and the PI controller replaces the network
+ * before three attempts of history exist, after two rejections in a row, or
+ * on a non-finite output).
+ * Network: 10 -> 64 -> 64 -> 1, tanh, linear output = log(dt_new / dt),
+ * clamped to [log 0.2, log 5].  Inputs (raw features, normalised by the
+ * stored mean / std): log err of the last three attempts, log(dt / (tf -
+ * t0)) of the last, log of the two earlier dt over the last, the three
+ * accept flags (1/0), log N.  Parameters: FP32 flat array
+ * W1[64][10] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3.  Training: FP32
+ * forward/backward kernels + Adam; inference: bfloat16 matrix cores. */
+#define ODEHIP_MLP_IN 10
+#define ODEHIP_MLP_HIDDEN 64
+#define ODEHIP_MLP_PARAMS 4929
+
+typedef struct odehip_mlp odehip_mlp;
+
+typedef struct {
+  int epochs;
+  int batch;             /* samples per Adam step */
+  float lr;
+  float beta1, beta2, eps; /* Adam; 0 = 0.9, 0.999, 1e-8 */
+  uint64_t seed;         /* sample order */
+} odehip_mlp_train_options;
+
+/* params: host, ODEHIP_MLP_PARAMS floats, or NULL (Glorot-uniform weights
+ * from seed, zero biases); mean, stdv: host, ODEHIP_MLP_IN floats. */
+int odehip_mlp_create(odehip_context* ctx, const float* params, uint64_t seed,
+                      const float* mean, const float* stdv, odehip_mlp** out);
+int odehip_mlp_destroy(odehip_mlp* mlp);
+/* params_out: host, ODEHIP_MLP_PARAMS floats (FP32 master weights). */
+int odehip_mlp_get_params(odehip_context* ctx, const odehip_mlp* mlp,
+                          float* params_out);
+/* Mean-squared loss 0.5 mean((y - t)^2) of n samples and its gradient
+ * (host, ODEHIP_MLP_PARAMS floats) at the current weights.  x: device
+ * [n][ODEHIP_MLP_IN] raw features, t: device [n] targets. */
+int odehip_mlp_loss_grad(odehip_context* ctx, odehip_mlp* mlp, const float* x,
+                         const float* t, long n, float* grad_out,
+                         double* loss_out);
+/* Adam over the n samples for opt->epochs epochs; epoch_loss (host, may be
+ * NULL) receives each epoch's mean loss.  Publishes the bfloat16 inference
+ * copy at the end. */
+int odehip_mlp_train(odehip_context* ctx, odehip_mlp* mlp, const float* x,
+                     const float* t, long n, const odehip_mlp_train_options* opt,
+                     double* epoch_loss);
+/* Network outputs (device, n floats) for raw features x (device), on the
+ * matrix cores with the bfloat16 weights. */
+int odehip_mlp_infer(odehip_context* ctx, const odehip_mlp* mlp,
+                     const float* x, long n, float* y);
+
 /* ------------------------------------------------------------- ensembles
  * Many independent N-body systems of different sizes solved together, each
  * with its own step size, accept/reject sequence, counters and termination

@@ -69,6 +69,15 @@ class EnsembleStats(C.Structure):
                 ("tile_config", C.c_int), ("n_unfinished", C.c_int), ("rhs_ms", C.c_double)]
 
 
+class MlpTrainOptions(C.Structure):
+    _fields_ = [("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("seed", C.c_uint64)]
+
+
+MLP_IN, MLP_HIDDEN, MLP_PARAMS = 10, 64, 4929
+_fp = C.POINTER(C.c_float)
+
 ENS_TILE_CONFIGS = 12
 ENS_TILE_AUTO = -1
 
@@ -120,6 +129,14 @@ SIGNATURES = {
                                         C.POINTER(SystemStats)]),
     "odehip_ensemble_rhs": (C.c_int, [_ctxp, C.c_int, _ip, C.c_int, C.c_double, C.c_void_p,
                                       C.c_int, C.c_void_p, C.c_void_p, _dp]),
+    "odehip_mlp_create": (C.c_int, [_ctxp, _fp, C.c_uint64, _fp, _fp, C.POINTER(C.c_void_p)]),
+    "odehip_mlp_destroy": (C.c_int, [C.c_void_p]),
+    "odehip_mlp_get_params": (C.c_int, [_ctxp, C.c_void_p, _fp]),
+    "odehip_mlp_loss_grad": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long,
+                                       _fp, _dp]),
+    "odehip_mlp_train": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long,
+                                   C.POINTER(MlpTrainOptions), _dp]),
+    "odehip_mlp_infer": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_long, C.c_void_p]),
     "odehip_nbody_energy": (C.c_int, [_ctxp, C.c_void_p, C.c_size_t, C.c_double, C.c_void_p,
                                       _dp]),
     "odehip_nbody_momentum": (C.c_int, [_ctxp, C.c_void_p, C.c_size_t, C.c_void_p, _dp]),

@@ -301,7 +301,8 @@ def _as_state(exe, v, n, name):
 
 def integrate(tableau, x0, t0, tf, atol, rtol, ode, output=None, exe=None, *,
               max_accepted=0, max_attempts=0, fsal=False, attempts_per_launch=0,
-              times_cap=None, trace_cap=0, x_out=None, fetch=True, controller="reference"):
+              times_cap=None, trace_cap=0, x_out=None, fetch=True, controller="reference",
+              mlp=None):
     """RKEmbeddedParallel<BT>::integrate with the attempt loop on the device.
 
     ``x0``: HipState or host array (n_var); ``atol``/``rtol``: scalars, host
@@ -338,7 +339,7 @@ def integrate(tableau, x0, t0, tf, atol, rtol, ode, output=None, exe=None, *,
                             int(attempts_per_launch),
                             times.ctypes.data if times_cap > 0 else None, int(times_cap),
                             trace.ctypes.data if trace_cap > 0 else None, int(trace_cap),
-                            CONTROLLERS[controller], None)
+                            CONTROLLERS[controller], mlp.handle if mlp is not None else None)
     stats = _lib.SolveStats()
     if output is not None:
         output.save_state(t0, work)

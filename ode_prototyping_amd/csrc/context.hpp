@@ -20,11 +20,14 @@ struct GraphKey {
   const void* atol = nullptr;
   const void* rtol = nullptr;
   const void* ws = nullptr;
+  int controller = -1;
+  const void* mlp = nullptr;
   bool operator==(const GraphKey& o) const {
     return tableau == o.tableau && rhs_kind == o.rhs_kind &&
            variant == o.variant && k == o.k && fsal == o.fsal &&
            n_var == o.n_var && std::memcmp(&eps2, &o.eps2, sizeof eps2) == 0 &&
-           mass == o.mass && atol == o.atol && rtol == o.rtol && ws == o.ws;
+           mass == o.mass && atol == o.atol && rtol == o.rtol && ws == o.ws &&
+           controller == o.controller && mlp == o.mlp;
   }
 };
 

@@ -43,7 +43,8 @@ struct EnsBufs {
   double* trace;  // n_sys x trace_cap x 3, or null
   long trace_cap;
   const double* mass;  // packed, or null
-  const void* mlp;     // learned controller's device network, or null
+  const void* mlp;     // learned controller's MlpInfer (device), or null
+  int controller;      // ODEHIP_CONTROLLER_*
   double atol, rtol;
   size_t nv;
   int n_sys;

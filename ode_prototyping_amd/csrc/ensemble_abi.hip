@@ -13,6 +13,7 @@
 
 #include "context.hpp"
 #include "ensemble.hpp"
+#include "mlp.hpp"
 
 namespace odehip {
 
@@ -378,8 +379,8 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
   if (opt->controller < ODEHIP_CONTROLLER_REFERENCE ||
       opt->controller > ODEHIP_CONTROLLER_LEARNED)
     return fail(ctx, ODEHIP_ERR_INVALID, "bad controller");
-  if (opt->controller == ODEHIP_CONTROLLER_LEARNED)
-    return fail(ctx, ODEHIP_ERR_UNSUPPORTED, "learned controller not available");
+  if (opt->controller == ODEHIP_CONTROLLER_LEARNED && !opt->mlp)
+    return fail(ctx, ODEHIP_ERR_INVALID, "the learned controller needs a network (mlp)");
   Layout L;
   if ((rc = make_layout(ctx, n_systems, n_particles, &L))) return rc;
   const int S = kTableaus[tableau].S;
@@ -400,6 +401,8 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
   b.tableau = tableau;
   b.variant = nbody_variant;
   b.eps2 = eps2;
+  b.controller = opt->controller;
+  b.mlp = opt->controller == ODEHIP_CONTROLLER_LEARNED ? mlp_infer_ptr(opt->mlp) : nullptr;
 
   // tile configuration
   int cfg = opt->tile_config;

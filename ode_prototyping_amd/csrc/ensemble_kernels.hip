@@ -18,6 +18,7 @@
 // system first, so the long tiles start early and the short ones fill the
 // tail; the list is built on the host per tile configuration.
 #include "ensemble.hpp"
+#include "mlp.hpp"
 #include "nbody_terms.hpp"
 #include "rk_device.hpp"
 
@@ -403,6 +404,40 @@ __global__ void k_ens_controller(EnsBufs b) {
   if (st->done) atomicSub(b.n_active, 1);
 }
 
+// The same with the learned controller: one wave per 16 systems; their
+// features go through the network on the matrix cores together.
+__global__ __launch_bounds__(64) void k_ens_controller_learned(EnsBufs b) {
+  __shared__ MlpTileLds tile;
+  const MlpInfer* M = static_cast<const MlpInfer*>(b.mlp);
+  const int lane = threadIdx.x;
+  const int s = blockIdx.x * 16 + lane;
+  SolverState* st = (lane < 16 && s < b.n_sys) ? b.st + s : nullptr;
+  const bool active = st && !st->done;
+  bool accept = false, judged = false;
+  float raw[kMlpIn];
+  for (int i = 0; i < kMlpIn; ++i) raw[i] = 0.0f;
+  if (active) {
+    const EnsSys sy = b.sys[s];
+    const double sum = fold_partials(b.partA + sy.gbase, sy.n_groups);
+    double* trace =
+        b.trace ? b.trace + static_cast<size_t>(s) * b.trace_cap * 3 : nullptr;
+    judged = ctrl_judge(st, e_tab[st->tableau], sum, 6 * static_cast<size_t>(sy.n),
+                        nullptr, trace, &accept);
+    if (judged) mlp_features(st, sy.n, raw);
+  }
+  if (lane < 16) mlp_tile_set_row(M, tile, lane, raw);
+  __syncthreads();
+  mlp_tile_forward(M, tile);
+  if (active) {
+    if (judged) {
+      double y = tile.y[lane];
+      y = y < kMlpLogRatioMin ? kMlpLogRatioMin : (y > kMlpLogRatioMax ? kMlpLogRatioMax : y);
+      ctrl_step(st, accept, std::isfinite(y) ? std::exp(y) : NAN);
+    }
+    if (st->done) atomicSub(b.n_active, 1);
+  }
+}
+
 __global__ void k_ens_init(EnsBufs b, double t0, double tf, long max_acc,
                            long max_att, int fsal, int controller) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -573,7 +608,12 @@ hipError_t ens_enqueue_round(const EnsBufs& b, hipStream_t s) {
       return e;
   }
   hipLaunchKernelGGL(k_ens_final, dim3(groups_grid(b)), dim3(kBlock), 0, s, b);
-  hipLaunchKernelGGL(k_ens_controller, dim3(sys_grid(b)), dim3(64), 0, s, b);
+  if (b.controller == ODEHIP_CONTROLLER_LEARNED) {
+    hipLaunchKernelGGL(k_ens_controller_learned, dim3((b.n_sys + 15) / 16), dim3(64), 0, s,
+                       b);
+  } else {
+    hipLaunchKernelGGL(k_ens_controller, dim3(sys_grid(b)), dim3(64), 0, s, b);
+  }
   return hipGetLastError();
 }
 

@@ -118,6 +118,7 @@ struct SolveBufs {
   double eps2;
   const double* mass;
   int controller;
+  const struct MlpInfer* mlp;  // learned controller's network, or null
 };
 
 hipError_t enqueue_solve_init(const SolveBufs& b, const double* x_in,

@@ -16,6 +16,7 @@
 #include <string>
 
 #include "context.hpp"
+#include "mlp.hpp"
 
 using namespace odehip;
 
@@ -444,9 +445,10 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
   if (opt->controller < ODEHIP_CONTROLLER_REFERENCE ||
       opt->controller > ODEHIP_CONTROLLER_LEARNED)
     return fail(ctx, ODEHIP_ERR_INVALID, "bad controller");
-  if (opt->controller == ODEHIP_CONTROLLER_LEARNED)
-    return fail(ctx, ODEHIP_ERR_UNSUPPORTED,
-                "the learned controller runs in the ensemble solve");
+  if (opt->controller == ODEHIP_CONTROLLER_LEARNED &&
+      (!opt->mlp || ode->rhs_kind != ODEHIP_RHS_NBODY))
+    return fail(ctx, ODEHIP_ERR_INVALID,
+                "the learned controller needs a network and an N-body problem");
   const long times_cap = opt->times ? opt->times_cap : 0;
   const long trace_cap = opt->trace ? opt->trace_cap : 0;
   SolveBufs b{};
@@ -454,6 +456,7 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
                     trace_cap, &b);
   if (rc) return rc;
   b.controller = opt->controller;
+  b.mlp = opt->controller == ODEHIP_CONTROLLER_LEARNED ? mlp_infer_ptr(opt->mlp) : nullptr;
   const int S = kTableaus[tableau].S;
   const size_t n = ode->n_var;
 
@@ -486,6 +489,8 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
     key.ws = ctx->ws;
     // times/trace pointers live in ws; their presence changes the layout:
     key.ws = reinterpret_cast<const char*>(ctx->ws) + (b.times ? 1 : 0) + (b.trace ? 2 : 0);
+    key.controller = b.controller;
+    key.mlp = b.mlp;
     if (!ctx->exec || !(key == ctx->key)) {
       if (ctx->exec) {
         (void)hipGraphExecDestroy(ctx->exec);

@@ -198,9 +198,10 @@ __device__ inline double pi_proposal(const SolverState* st, double err,
 
 // Second half (RKEmbeddedParallel.hpp:64-79): the new step size, its clamps
 // and the clip to tf, then the termination tests.  `learned_ratio` is the
-// learned controller's proposed dt_new / dt (ODEHIP_CONTROLLER_LEARNED only);
-// the PI proposal replaces it when it is not finite, before three attempts
-// of history exist, or after two rejections in a row.
+// learned controller's proposed dt_new / dt (ODEHIP_CONTROLLER_LEARNED only,
+// capped at 1 right after a rejection); the PI proposal replaces it when it
+// is not finite, before three attempts of history exist, or after two
+// rejections in a row.
 __device__ inline void ctrl_step(SolverState* st, bool accept,
                                  double learned_ratio) {
   const double scaled_error = st->err;
@@ -213,7 +214,8 @@ __device__ inline void ctrl_step(SolverState* st, bool accept,
     const bool usable = std::isfinite(learned_ratio) && st->attempts >= 3 &&
                         st->rejects_in_row < 2;
     if (usable) {
-      dtnew = dt * learned_ratio;
+      // no growth right after a rejection
+      dtnew = dt * ((!accept && learned_ratio > 1.0) ? 1.0 : learned_ratio);
       st->learned_steps += 1;
     } else {
       dtnew = pi_proposal(st, scaled_error, accept);

@@ -7,6 +7,7 @@
 // reference's.
 #include <cmath>
 
+#include "mlp.hpp"
 #include "rk_device.hpp"
 
 namespace odehip {
@@ -127,6 +128,31 @@ __global__ void k_controller(SolverState* st, const double* partials,
   const double sum = fold_partials_block(partials, st->n_groups);
   if (threadIdx.x != 0) return;
   ctrl_attempt(st, c_tab[st->tableau], sum, n_var, times, trace);
+}
+
+// The same with the learned controller (a tile of one system).
+__global__ __launch_bounds__(64) void k_controller_learned(
+    SolverState* st, const double* partials, size_t n_var, double* times,
+    double* trace, const MlpInfer* M, int n_particles) {
+  __shared__ MlpTileLds tile;
+  if (st->done) return;  // read by every thread before thread 0 writes it
+  const double sum = fold_partials_block(partials, st->n_groups);
+  bool accept = false, judged = false;
+  float raw[kMlpIn];
+  for (int i = 0; i < kMlpIn; ++i) raw[i] = 0.0f;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    judged = ctrl_judge(st, c_tab[st->tableau], sum, n_var, times, trace, &accept);
+    if (judged) mlp_features(st, n_particles, raw);
+  }
+  if (threadIdx.x < 16) mlp_tile_set_row(M, tile, threadIdx.x, raw);
+  __syncthreads();
+  mlp_tile_forward(M, tile);
+  if (threadIdx.x == 0 && judged) {
+    double y = tile.y[0];
+    y = y < kMlpLogRatioMin ? kMlpLogRatioMin : (y > kMlpLogRatioMax ? kMlpLogRatioMax : y);
+    ctrl_step(st, accept, std::isfinite(y) ? std::exp(y) : NAN);
+  }
 }
 
 // ------------------------------------------------------ initial step -------
@@ -402,8 +428,13 @@ hipError_t enqueue_attempt(const SolveBufs& b, hipStream_t s) {
   hipLaunchKernelGGL(k_final_dev, dim3(blocks_for(n)), dim3(kBlock), 0, s, b.st,
                      b.tableau, b.ks, b.xb0, b.xb1, b.atol, b.rtol, b.partA,
                      b.kfsal, n);
-  hipLaunchKernelGGL(k_controller, dim3(1), dim3(64), 0, s, b.st, b.partA, n,
-                     b.times, b.trace);
+  if (b.controller == ODEHIP_CONTROLLER_LEARNED) {
+    hipLaunchKernelGGL(k_controller_learned, dim3(1), dim3(64), 0, s, b.st, b.partA, n,
+                       b.times, b.trace, b.mlp, b.n_particles);
+  } else {
+    hipLaunchKernelGGL(k_controller, dim3(1), dim3(64), 0, s, b.st, b.partA, n, b.times,
+                       b.trace);
+  }
   return hipGetLastError();
 }
 

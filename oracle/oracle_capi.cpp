@@ -7,6 +7,7 @@
 #include <memory>
 #include <string>
 
+#include "mlp.hpp"
 #include "oracle.hpp"
 
 using namespace oracle;
@@ -218,6 +219,21 @@ void oracle_spinning_cube_ic(long n, double* x, double* params) {
 
 double oracle_pow(double x, double y, int mode) {
   return opow(x, y, static_cast<PowMode>(mode));
+}
+
+// Learned-controller network (mlp.hpp): outputs for n samples of raw
+// features [n x 10]; bf16 != 0 rounds weights and layer inputs to bfloat16.
+void oracle_mlp_forward(const double* params, const double* mean,
+                        const double* stdv, const double* raw, long n, int bf16,
+                        double* out) {
+  for (long s = 0; s < n; ++s)
+    out[s] = mlp_forward_one(params, mean, stdv, raw + s * kIn, bf16 != 0);
+}
+
+double oracle_mlp_loss_grad(const double* params, const double* mean,
+                            const double* stdv, const double* raw,
+                            const double* target, long n, double* grad) {
+  return mlp_loss_grad(params, mean, stdv, raw, target, n, grad);
 }
 
 }  // extern "C"

@@ -58,6 +58,9 @@ def lib():
         L.oracle_spinning_cube_ic.argtypes = [C.c_long, _dp, _dp]
         L.oracle_pow.argtypes = [C.c_double, C.c_double, C.c_int]
         L.oracle_pow.restype = C.c_double
+        L.oracle_mlp_forward.argtypes = [_dp, _dp, _dp, _dp, C.c_long, C.c_int, _dp]
+        L.oracle_mlp_loss_grad.argtypes = [_dp, _dp, _dp, _dp, _dp, C.c_long, _dp]
+        L.oracle_mlp_loss_grad.restype = C.c_double
         _LIB = L
     return _LIB
 
@@ -182,3 +185,27 @@ def spinning_cube_ic(n):
     p = np.zeros(3)
     lib().oracle_spinning_cube_ic(n, _ptr(x), _ptr(p))
     return x, float(p[0]), float(p[1]), float(p[2])
+
+
+MLP_IN, MLP_HID = 10, 64
+MLP_PARAMS = MLP_HID * MLP_IN + MLP_HID + MLP_HID * MLP_HID + MLP_HID + MLP_HID + 1
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def mlp_forward(params, mean, std, raw, bf16=False):
+    raw = _f64(raw).reshape(-1, MLP_IN)
+    out = np.zeros(raw.shape[0])
+    lib().oracle_mlp_forward(_ptr(_f64(params)), _ptr(_f64(mean)), _ptr(_f64(std)),
+                             _ptr(raw), raw.shape[0], int(bf16), _ptr(out))
+    return out
+
+
+def mlp_loss_grad(params, mean, std, raw, target):
+    raw = _f64(raw).reshape(-1, MLP_IN)
+    g = np.zeros(MLP_PARAMS)
+    loss = lib().oracle_mlp_loss_grad(_ptr(_f64(params)), _ptr(_f64(mean)), _ptr(_f64(std)),
+                                      _ptr(raw), _ptr(_f64(target)), raw.shape[0], _ptr(g))
+    return loss, g

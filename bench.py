@@ -13,6 +13,18 @@ directed pair interaction, N(N-1) per launch, timed with HIP events on the
 library's stream); the bit-exact and mixed-precision RHS variants; and the
 reference's own thread-pool CPU path (oracle/_ref, built from the reference
 headers) timed on this host on a bounded sample (2 accepted steps at full N).
+
+Further blocks (each a separate workload, not the headline value):
+* "ensemble": 2048 ragged systems (N log-uniform in [16, 4096], uniform-cube
+  positions, N(0, 0.1^2) velocities, masses 1/N, eps 1e-3), DOPRI5, rtol =
+  atol = 1e-10, t in [0, 0.1], every system with its own step size: plain
+  FP64 (exact RHS, reference controller) against the chosen path (mixed
+  precision RHS + learned controller) and the other variants;
+* "controller": 4096 held-out systems of the same distribution, DOP853,
+  t in [0, 0.05], mixed-precision RHS: reference vs PI vs learned controller;
+* "precision": N = 65536 colliding Plummer spheres, eps 1e-2, DOP853, 10
+  accepted steps: mixed-precision against the FP64 path (final-state and
+  energy error).
 """
 import argparse
 import ctypes as C
@@ -75,6 +87,152 @@ def cpu_baseline(x0, m, softening, tol, n, accepted):
             "kind": kind, "sample": what, "seconds": secs}
 
 
+def nbody_energy(oh, exe, x_dev_or_host, n, eps, m):
+    """(KE + PE) of one system on the device."""
+    x = x_dev_or_host if isinstance(x_dev_or_host, oh.HipState) else oh.HipState(
+        exe, data=x_dev_or_host)
+    md = oh.HipState(exe, data=m)
+    out = np.zeros(2)
+    exe.check(oh.lib().odehip_nbody_energy(exe.handle, x.ptr, n, eps, md.ptr,
+                                           out.ctypes.data_as(C.POINTER(C.c_double))),
+              "nbody_energy")
+    return out.sum()
+
+
+def ensemble_block(oh, exe):
+    """Batched ensemble of 2048 ragged systems, DOPRI5."""
+    from ode_prototyping_amd.ensemble import solve_ensemble, unpack
+    from ode_prototyping_amd import train_controller as trc
+    tf, eps, tol, tab = 0.1, 1e-3, 1e-10, "DOPRI5"
+    sizes, x0, m = trc.workload_ensemble(2048, 0)
+    x0d, md = oh.HipState(exe, data=x0), oh.HipState(exe, data=m)
+    out_state = oh.HipState(exe, x0.size)
+    net, info = trc.pretrained(exe, tab)
+    inter = float(np.sum(sizes.astype(float) * (sizes - 1)))
+    runs = {"plain_exact": ("exact", "reference", None),
+            "fast": ("fast", "reference", None),
+            "mixed": ("mixed", "reference", None),
+            "chosen_mixed_learned": ("mixed", "learned", net)}
+    block = {"workload": f"{len(sizes)} systems, N log-uniform [16, 4096] (sum N "
+                         f"{int(sizes.sum())}), uniform cube, masses 1/N, eps={eps}, {tab}, "
+                         f"rtol=atol={tol}, t in [0, {tf}], per-system step size",
+             "interactions_per_rhs": inter, "learned_net": info}
+    finals = {}
+    for name, (var, ctrl, mlp) in runs.items():
+        # untimed: the tile autotuner runs on the first solve of a variant
+        solve_ensemble(tab, sizes, x0d, 0.0, tf, tol, tol, eps, md, variant=var, exe=exe,
+                       controller=ctrl, mlp=mlp, max_attempts=1, x_out=out_state,
+                       fetch=False)
+        exe.synchronize()
+        t = time.perf_counter()
+        r = solve_ensemble(tab, sizes, x0d, 0.0, tf, tol, tol, eps, md, variant=var, exe=exe,
+                           controller=ctrl, mlp=mlp, x_out=out_state, fetch=False)
+        exe.synchronize()
+        secs = time.perf_counter() - t
+        finals[name] = out_state.copy_to_span()
+        peak = 2 * FP64_PEAK_TFLOPS if var == "mixed" else FP64_PEAK_TFLOPS
+        ach = FLOP_PER_INTERACTION * inter / (r.rhs_ms * 1e-3) / 1e12 if r.rhs_ms else None
+        block[name] = {"variant": var, "controller": ctrl, "seconds": secs,
+                       "particle_steps_per_s": r.particle_steps / secs,
+                       "rhs_evals_per_solve": r.total_rhs_evals / len(sizes),
+                       "accepted": r.total_accepted, "rejected": r.total_rejected,
+                       "rounds": r.rounds, "learned_steps": r.total_learned_steps,
+                       "tile_config": r.tile_config, "rhs_ms": r.rhs_ms,
+                       "rhs_tflops": ach, "rhs_peak_tflops": peak,
+                       "rhs_frac": (ach / peak) if ach else None}
+        log(f"ensemble {name}: {block[name]}")
+    # error of the chosen path vs the plain FP64 path, and energy drift
+    ref = unpack(finals["plain_exact"], sizes)
+    ms = unpack(m, sizes, per_particle=1)
+    x0s = unpack(x0, sizes)
+    idx = np.linspace(0, len(sizes) - 1, 64).astype(int)  # energy on 64 systems
+    for name in ("chosen_mixed_learned", "mixed", "fast"):
+        got = unpack(finals[name], sizes)
+        err = [np.max(np.abs(a - b)) / np.max(np.abs(b)) for a, b in zip(got, ref)]
+        block[name]["final_state_rel_err_vs_plain"] = {"median": float(np.median(err)),
+                                                       "max": float(np.max(err))}
+    for name in ("plain_exact", "chosen_mixed_learned"):
+        got = unpack(finals[name], sizes)
+        drift = []
+        for k in idx:
+            e0 = nbody_energy(oh, exe, x0s[k], int(sizes[k]), eps, ms[k])
+            e1 = nbody_energy(oh, exe, got[k], int(sizes[k]), eps, ms[k])
+            drift.append(abs((e1 - e0) / e0))
+        block[name]["energy_rel_drift_64_systems"] = {"median": float(np.median(drift)),
+                                                      "max": float(np.max(drift))}
+    net.close()
+    return block
+
+
+def controller_block(oh, exe):
+    """Learned vs PI vs reference controller on 4096 held-out systems."""
+    from ode_prototyping_amd.ensemble import solve_ensemble
+    from ode_prototyping_amd import train_controller as trc
+    tf, eps, tol, tab = 0.05, 1e-3, 1e-10, "DOP853"
+    sizes, x0, m = trc.workload_ensemble(4096, 9000)
+    x0d, md = oh.HipState(exe, data=x0), oh.HipState(exe, data=m)
+    out_state = oh.HipState(exe, x0.size)
+    net, info = trc.pretrained(exe, tab)
+    block = {"workload": f"{len(sizes)} held-out systems (seed 9000), N log-uniform "
+                         f"[16, 4096], eps={eps}, {tab}, rtol=atol={tol}, t in [0, {tf}], "
+                         "mixed-precision RHS", "learned_net": info}
+    w2 = sizes.astype(float) ** 2
+    for ctrl in ("reference", "pi", "learned"):
+        mlp = net if ctrl == "learned" else None
+        solve_ensemble(tab, sizes, x0d, 0.0, tf, tol, tol, eps, md, variant="mixed", exe=exe,
+                       controller=ctrl, mlp=mlp, max_attempts=1, x_out=out_state, fetch=False)
+        exe.synchronize()
+        t = time.perf_counter()
+        r = solve_ensemble(tab, sizes, x0d, 0.0, tf, tol, tol, eps, md, variant="mixed",
+                           exe=exe, controller=ctrl, mlp=mlp, x_out=out_state, fetch=False)
+        exe.synchronize()
+        secs = time.perf_counter() - t
+        block[ctrl] = {"seconds": secs, "rhs_evals_per_solve": r.total_rhs_evals / len(sizes),
+                       "pair_work_per_solve": float(np.sum(w2 * r.rhs_evals)) / len(sizes),
+                       "accepted": r.total_accepted, "rejected": r.total_rejected,
+                       "rounds": r.rounds, "learned_steps": r.total_learned_steps,
+                       "particle_steps_per_s": r.particle_steps / secs}
+        log(f"controller {ctrl}: {block[ctrl]}")
+    ref = block["reference"]
+    block["learned_gain"] = {
+        "rhs_evals": 1.0 - block["learned"]["rhs_evals_per_solve"] / ref["rhs_evals_per_solve"],
+        "pair_work": 1.0 - block["learned"]["pair_work_per_solve"] / ref["pair_work_per_solve"],
+        "seconds": 1.0 - block["learned"]["seconds"] / ref["seconds"]}
+    net.close()
+    return block
+
+
+def precision_block(oh, exe):
+    """Mixed precision on 65536 colliding Plummer bodies, DOP853."""
+    n, eps, tol, tab, acc = 65536, 1e-2, 1e-10, "DOP853", 10
+    x0, m = oh.scenarios.colliding_plummer(n, seed=0)
+    x0d = oh.HipState(exe, data=x0)
+    atol = oh.HipState(exe, data=np.full(6 * n, tol))
+    e0 = nbody_energy(oh, exe, x0d, n, eps, m)
+    block = {"workload": f"{n} bodies: two Plummer spheres (a=1, M=1 total) at +-2 on x "
+                         f"closing at 0.3, eps={eps}, {tab}, rtol=atol={tol}, {acc} accepted "
+                         "steps"}
+    finals = {}
+    for var in ("exact", "mixed"):
+        o = oh.NBodyODE(eps, m, var)
+        t = time.perf_counter()
+        r = oh.integrate(tab, x0d, 0.0, 100.0, atol, atol, o, exe=exe, max_accepted=acc)
+        secs = time.perf_counter() - t
+        finals[var] = r.x_final
+        e1 = nbody_energy(oh, exe, r.x_final, n, eps, m)
+        block[var] = {"seconds": secs, "particle_steps_per_s": n * r.accepted / secs,
+                      "accepted": r.accepted, "rejected": r.rejected, "t": r.t,
+                      "rhs_evals": r.rhs_evals, "energy_rel_err": abs((e1 - e0) / e0)}
+        log(f"precision {var}: {block[var]}")
+    # the exact path equals the FP64 oracle bit for bit; compare at the same time
+    o = oh.NBodyODE(eps, m, "exact")
+    t_mixed = block["mixed"]["t"]
+    ref = oh.integrate(tab, x0d, 0.0, t_mixed, atol, atol, o, exe=exe).x_final
+    block["mixed"]["final_state_rel_err_vs_fp64_same_t"] = float(
+        np.max(np.abs(finals["mixed"] - ref)) / np.max(np.abs(ref)))
+    return block
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +243,12 @@ def main():
     ap.add_argument("--variant", default="exact", choices=["fast", "exact", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-controller", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--profile-ensemble", action="store_true",
+                    help="only run short ensemble solves (2048 systems, t in [0, 0.005]) in "
+                         "each RHS variant (the workload of the committed ensemble profiles)")
     ap.add_argument("--profile-rhs", action="store_true",
                     help="only run each RHS variant 3x and a 64 MiB calibration copy "
                          "(the workload of the committed counter profiles)")
@@ -109,6 +273,22 @@ def main():
             b.assign(a)
         exe.synchronize()
         log("profile-rhs done")
+        return
+
+    if args.profile_ensemble:
+        from ode_prototyping_amd.ensemble import solve_ensemble
+        from ode_prototyping_amd import train_controller as trc
+        sizes, x0, m = trc.workload_ensemble(2048, 0)
+        exe = oh.HipExecutor(0)
+        x0d, md = oh.HipState(exe, data=x0), oh.HipState(exe, data=m)
+        out_state = oh.HipState(exe, x0.size)
+        for var in ("exact", "fast", "mixed"):
+            for rep in range(2):
+                r = solve_ensemble("DOPRI5", sizes, x0d, 0.0, 0.005, 1e-10, 1e-10, 1e-3, md,
+                                   variant=var, exe=exe, x_out=out_state, fetch=False)
+                exe.synchronize()
+            log(f"profile-ensemble {var}: rounds {r.rounds} tile {r.tile_config} "
+                f"rhs_ms {r.rhs_ms:.3f}")
         return
 
     n, steps, warmup = args.n, args.steps, args.warmup
@@ -230,6 +410,13 @@ def main():
                                        "final_state_rel_err_vs_plain": rel}
         log(f"chosen: {extras['chosen_mixed_fsal']}")
         out["variants"] = extras
+
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_block(oh, exe)
+    if not args.no_controller:
+        out["controller"] = controller_block(oh, exe)
+    if not args.no_precision:
+        out["precision"] = precision_block(oh, exe)
 
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

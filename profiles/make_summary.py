@@ -1,35 +1,41 @@
-"""Summarise a rocprofv3 kernel trace (rocpd SQLite output) as a text table.
+"""Summarise a rocprofv3 kernel trace as a text table.
 
-Usage: python profiles/make_summary.py <run_results.db> "<command that made it>"
-Prints per-kernel dispatch count, average / min / max duration and share.
+Usage: python profiles/make_summary.py <run_results.db | kernel_trace.csv> "<command>"
+Prints per-kernel dispatch count, average / min / max duration and share, and
+the average over the launches that did most work (>= 50 % of the longest:
+the launches of a solve in which every system is still running -- a finished
+system's blocks return at once).
 """
+import csv
 import sqlite3
 import sys
 
-db, cmd = sys.argv[1], sys.argv[2]
-c = sqlite3.connect(db)
-rows = list(c.execute("select name, count(*), avg(duration), min(duration), max(duration), "
-                      "sum(duration), max(vgpr_count), max(sgpr_count), max(lds_size) "
-                      "from kernels group by name order by 6 desc"))
-durs = {}
-for name, d in c.execute("select name, duration from kernels"):
-    durs.setdefault(name, []).append(d)
+src, cmd = sys.argv[1], sys.argv[2]
+durs, regs = {}, {}
+if src.endswith(".csv"):
+    for r in csv.DictReader(open(src)):
+        name = r["Kernel_Name"]
+        durs.setdefault(name, []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+        regs[name] = (int(r["VGPR_Count"]), int(r["SGPR_Count"]), int(r["LDS_Block_Size"]))
+else:
+    c = sqlite3.connect(src)
+    for name, d, vg, sg, lds in c.execute(
+            "select name, duration, vgpr_count, sgpr_count, lds_size from kernels"):
+        durs.setdefault(name, []).append(d)
+        regs[name] = (vg, sg, lds)
 
-
-def working_avg(name):
-    """Average over launches that did work (a finished solve's kernels return
-    at once; those launches take < 10% of the longest)."""
-    d = sorted(durs[name])
-    keep = [x for x in d if x >= 0.1 * d[-1]]
-    return sum(keep) / len(keep), len(keep)
-tot = sum(r[5] for r in rows)
+tot = sum(sum(v) for v in durs.values())
 print(f"command: {cmd}")
-print(f"{'kernel':34s} {'calls':>6s} {'avg_us':>9s} {'work_n':>6s} {'work_avg_us':>11s} "
+print(f"{'kernel':42s} {'calls':>6s} {'avg_us':>9s} {'full_n':>6s} {'full_avg_us':>11s} "
       f"{'min_us':>8s} {'max_us':>8s} {'total_ms':>9s} {'share':>6s} {'vgpr':>4s} {'sgpr':>4s} "
       f"{'lds':>6s}")
-for name, n, avg, mn, mx, s, vg, sg, lds in rows:
+for name in sorted(durs, key=lambda k: -sum(durs[k])):
+    d = sorted(durs[name])
+    full = [x for x in d if x >= 0.5 * d[-1]]
     short = name.replace("void ", "").replace("odehip::(anonymous namespace)::", "")
-    short = short.split("(")[0]
-    wavg, wn = working_avg(name)
-    print(f"{short[:34]:34s} {n:6d} {avg / 1e3:9.2f} {wn:6d} {wavg / 1e3:11.2f} {mn / 1e3:8.2f} "
-          f"{mx / 1e3:8.2f} {s / 1e6:9.2f} {100 * s / tot:5.1f}% {vg:4d} {sg:4d} {lds:6d}")
+    short = short.replace("odehip::", "").split("(")[0]
+    vg, sg, lds = regs[name]
+    s = sum(d)
+    print(f"{short[:42]:42s} {len(d):6d} {s / len(d) / 1e3:9.2f} {len(full):6d} "
+          f"{sum(full) / len(full) / 1e3:11.2f} {d[0] / 1e3:8.2f} {d[-1] / 1e3:8.2f} "
+          f"{s / 1e6:9.2f} {100 * s / tot:5.1f}% {vg:4d} {sg:4d} {lds:6d}")

@@ -117,3 +117,29 @@ def test_learned_controller_keeps_the_judge(exe):
     assert (one.accepted, one.rejected) == (res.accepted[1], res.rejected[1])
     assert np.array_equal(one.x_final, res.states()[1])
     net.close()
+
+
+def test_trained_controller_gain(exe):
+    """The shipped DOP853 network (trained on the GPU from PI-controller
+    traces of the ensemble workload, mixed-precision RHS) against the
+    method without it, on held-out systems: fewer RHS evaluations to reach
+    the same time, and every step still accepted only at error norm <= 1."""
+    from ode_prototyping_amd import train_controller as trc
+    from ode_prototyping_amd.ensemble import solve_ensemble
+    net, info = trc.pretrained(exe, "DOP853")
+    assert info["tableau"] == "DOP853" and info["samples"] > 100000
+    sizes, x0, m = trc.workload_ensemble(256, 9100)
+    runs = {}
+    for ctrl in ("reference", "pi", "learned"):
+        runs[ctrl] = solve_ensemble("DOP853", sizes, x0, 0.0, 0.05, 1e-10, 1e-10, 1e-3, m,
+                                    variant="mixed", exe=exe, controller=ctrl,
+                                    mlp=net if ctrl == "learned" else None, trace_cap=1 << 14)
+    for ctrl, r in runs.items():
+        print(ctrl, r.total_rhs_evals, r.total_accepted, r.total_rejected, r.rounds)
+        assert r.n_unfinished == 0 and np.all(r.t == 0.05)
+        for tr in r.trace:
+            assert np.all(tr[tr[:, 2] == 1, 0] <= 1.0)
+    gain = 1.0 - runs["learned"].total_rhs_evals / runs["reference"].total_rhs_evals
+    assert gain > 0.05, gain
+    assert runs["learned"].total_rhs_evals < runs["pi"].total_rhs_evals
+    net.close()

@@ -217,6 +217,11 @@ typedef struct {
   long trace_cap;
   int controller;        /* ODEHIP_CONTROLLER_*; 0 = the reference's */
   const struct odehip_mlp* mlp; /* trained network (LEARNED only) */
+  double* states;        /* host, states_cap x n_var: x0 then the state after
+                            each accepted step (RawOutput::save_state,
+                            RawOutput.hpp:15-19), kept on the device during
+                            the solve; may be NULL */
+  long states_cap;
 } odehip_solve_options;
 
 typedef struct {
@@ -226,12 +231,44 @@ typedef struct {
   int status;            /* ODEHIP_STATUS_* */
   double t, dt, dt_initial, last_err;
   long learned_steps;    /* steps sized by the learned controller */
+  long n_states;         /* states written (accepted + 1, capped) */
 } odehip_solve_stats;
 
 /* x: device, n_var; in: x0, out: the state at the final time. */
 int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
                     const odehip_solve_options* opt, double* x,
                     odehip_solve_stats* stats);
+
+/* ---------------------------------------------------------------- leapfrog
+ * ParallelLeapfrogIntegrator::integrate (src/integrator/leapfrog/
+ * LeapfrogParallel.hpp:8-41): fixed-step velocity Verlet of an N-body state
+ * with the accelerations of the RHS variant ode->nbody_variant (NBodyODE::acc,
+ * NBodyODE.hpp:41-46); t advances by t += dt while t < tf (no clip of the last
+ * step, as the reference).  x: device, in/out.  states (host, states_cap x
+ * n_var) and times (host, states_cap) receive the first states_cap saved
+ * states (RawOutput::save_state at t0 and after every step); either may be
+ * NULL. */
+typedef struct {
+  long steps;            /* steps taken */
+  long rhs_evals;        /* steps + 1 */
+  long n_states;         /* states written */
+  double t;              /* final time */
+} odehip_leapfrog_stats;
+
+int odehip_leapfrog(odehip_context* ctx, const odehip_ode* ode, double t0,
+                    double tf, double dt, double* x, double* states,
+                    double* times, long states_cap,
+                    odehip_leapfrog_stats* stats);
+
+/* ------------------------------------------------------ trajectory output
+ * output_to_file (src/demo/nbody_io.hpp:4-22; read by
+ * scripts/nbody_io/bin_reader.py:4-18): little-endian u64 n_times, u64 n_var,
+ * f64 softening, then n_times rows of f64 t followed by n_var f64 state
+ * values.  times, states: host.  Returns ODEHIP_ERR_INVALID if the file cannot
+ * be written. */
+int odehip_write_trajectory(const char* path, size_t n_times, size_t n_var,
+                            double softening, const double* times,
+                            const double* states);
 
 /* ----------------------------------------------------- conserved quantities
  * scripts/nbody_physics/energy.py:118-192,224-241 (softened PE + KE), with

@@ -7,17 +7,22 @@ counterpart of ``src/par_exec/CudaExecutor.cuh``), device state containers
 (:class:`NBodyODE`, :class:`ExponentialODE`, :class:`VanDerPolODE`,
 ``src/ode``), output policies (:class:`RawOutput`, ``src/output/RawOutput.hpp``)
 and the integrator (:func:`integrate`, ``RKEmbeddedParallel<BT>::integrate``,
-``src/integrator/RKEmbeddedParallel/RKEmbeddedParallel.hpp:13-81``).  All
-numerical work runs in libodehip.so through its C ABI (include/odehip.h).
+``src/integrator/RKEmbeddedParallel/RKEmbeddedParallel.hpp:13-81``), plus
+the leapfrog integrator (:func:`leapfrog`, ``LeapfrogParallel.hpp``), batched
+ensembles (:mod:`.ensemble`), the learned step-size controller
+(:mod:`.controller`, :mod:`.train_controller`) and trajectory files
+(:mod:`.nbody_io`).  All numerical work runs in libodehip.so through its C ABI
+(include/odehip.h).
 """
 from ._lib import (CONTROLLERS, NBODY_VARIANTS, REDUCE_GROUP, STAGES, STATUS, TABLEAUS, lib)
 from .core import (ExponentialODE, HipExecutor, HipState, MostRecentOutput, NBodyODE,
-                   RawOutput, SolveResult, VanDerPolODE, integrate, integrate_host_driven)
-from . import scenarios
+                   RawOutput, SolveResult, VanDerPolODE, integrate, integrate_host_driven,
+                   leapfrog)
+from . import nbody_io, scenarios
 
 __all__ = [
     "HipExecutor", "HipState", "NBodyODE", "ExponentialODE", "VanDerPolODE",
     "RawOutput", "MostRecentOutput", "SolveResult", "integrate",
-    "integrate_host_driven", "scenarios", "TABLEAUS", "STAGES", "STATUS",
+    "integrate_host_driven", "leapfrog", "scenarios", "nbody_io", "TABLEAUS", "STAGES", "STATUS",
     "NBODY_VARIANTS", "REDUCE_GROUP", "lib",
 ]

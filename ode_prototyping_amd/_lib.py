@@ -36,7 +36,8 @@ class SolveOptions(C.Structure):
                 ("fsal", C.c_int), ("attempts_per_launch", C.c_int),
                 ("times", C.c_void_p), ("times_cap", C.c_long),
                 ("trace", C.c_void_p), ("trace_cap", C.c_long),
-                ("controller", C.c_int), ("mlp", C.c_void_p)]
+                ("controller", C.c_int), ("mlp", C.c_void_p),
+                ("states", C.c_void_p), ("states_cap", C.c_long)]
 
 
 class SolveStats(C.Structure):
@@ -44,7 +45,7 @@ class SolveStats(C.Structure):
                 ("rhs_evals", C.c_long), ("n_times", C.c_long), ("n_trace", C.c_long),
                 ("status", C.c_int), ("t", C.c_double), ("dt", C.c_double),
                 ("dt_initial", C.c_double), ("last_err", C.c_double),
-                ("learned_steps", C.c_long)]
+                ("learned_steps", C.c_long), ("n_states", C.c_long)]
 
 
 class EnsembleOptions(C.Structure):
@@ -77,6 +78,11 @@ class MlpTrainOptions(C.Structure):
 
 MLP_IN, MLP_HIDDEN, MLP_PARAMS = 10, 64, 4929
 _fp = C.POINTER(C.c_float)
+
+class LeapfrogStats(C.Structure):
+    _fields_ = [("steps", C.c_long), ("rhs_evals", C.c_long), ("n_states", C.c_long),
+                ("t", C.c_double)]
+
 
 ENS_TILE_CONFIGS = 12
 ENS_TILE_AUTO = -1
@@ -137,6 +143,11 @@ SIGNATURES = {
     "odehip_mlp_train": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long,
                                    C.POINTER(MlpTrainOptions), _dp]),
     "odehip_mlp_infer": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_long, C.c_void_p]),
+    "odehip_leapfrog": (C.c_int, [_ctxp, C.POINTER(OdeStruct), C.c_double, C.c_double,
+                                  C.c_double, C.c_void_p, _dp, _dp, C.c_long,
+                                  C.POINTER(LeapfrogStats)]),
+    "odehip_write_trajectory": (C.c_int, [C.c_char_p, C.c_size_t, C.c_size_t, C.c_double,
+                                          _dp, _dp]),
     "odehip_nbody_energy": (C.c_int, [_ctxp, C.c_void_p, C.c_size_t, C.c_double, C.c_void_p,
                                       _dp]),
     "odehip_nbody_momentum": (C.c_int, [_ctxp, C.c_void_p, C.c_size_t, C.c_void_p, _dp]),

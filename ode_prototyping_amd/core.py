@@ -302,14 +302,15 @@ def _as_state(exe, v, n, name):
 def integrate(tableau, x0, t0, tf, atol, rtol, ode, output=None, exe=None, *,
               max_accepted=0, max_attempts=0, fsal=False, attempts_per_launch=0,
               times_cap=None, trace_cap=0, x_out=None, fetch=True, controller="reference",
-              mlp=None):
+              mlp=None, states_cap=None):
     """RKEmbeddedParallel<BT>::integrate with the attempt loop on the device.
 
     ``x0``: HipState or host array (n_var); ``atol``/``rtol``: scalars, host
     arrays or HipStates (per component, as the reference).  ``output`` (if
-    given) receives ``save_state`` for every recorded time with the initial
-    and the final state (intermediate states are not copied back; use
-    :func:`integrate_host_driven` for RawOutput of every accepted state).
+    given) receives ``save_state(t, x)`` for the initial state and every
+    accepted step, as the reference's RawOutput (the states are kept on the
+    device during the solve: ``states_cap`` of them, default ``times_cap``;
+    a list larger than 8 GiB must be asked for explicitly).
     Returns a :class:`SolveResult` (final state on the host in ``x_final``
     unless ``fetch`` is False -- it stays in ``x_out`` on the device --,
     accepted times in ``times``, per-attempt ``trace`` rows (err, dt, accepted)
@@ -333,23 +334,29 @@ def integrate(tableau, x0, t0, tf, atol, rtol, ode, output=None, exe=None, *,
         times_cap = (max_accepted + 1) if max_accepted > 0 else 1 << 16
     times = np.zeros(max(int(times_cap), 1))
     trace = np.zeros((max(int(trace_cap), 1), 3))
+    if states_cap is None:
+        states_cap = times_cap if output is not None else 0
+        if states_cap * n * 8 > (8 << 30):
+            raise ValueError("the saved-state list would exceed 8 GiB: pass states_cap")
+    states = np.zeros((max(int(states_cap), 1), n)) if states_cap > 0 else None
     ode_s = ode.ode_struct(exe, n)
     opt = _lib.SolveOptions(float(t0), float(tf), at.ptr.value, rt.ptr.value,
                             int(max_accepted), int(max_attempts), 1 if fsal else 0,
                             int(attempts_per_launch),
                             times.ctypes.data if times_cap > 0 else None, int(times_cap),
                             trace.ctypes.data if trace_cap > 0 else None, int(trace_cap),
-                            CONTROLLERS[controller], mlp.handle if mlp is not None else None)
+                            CONTROLLERS[controller], mlp.handle if mlp is not None else None,
+                            states.ctypes.data if states is not None else None,
+                            int(states_cap))
     stats = _lib.SolveStats()
-    if output is not None:
-        output.save_state(t0, work)
     exe.check(lib().odehip_rk_solve(exe.handle, TABLEAUS[tableau], C.byref(ode_s),
                                     C.byref(opt), work.ptr, C.byref(stats)), "rk_solve")
     x_final = work.copy_to_span() if fetch else None
     res = SolveResult(stats, x_final, times[:stats.n_times].copy(),
                       trace[:stats.n_trace].copy() if trace_cap else None)
-    if output is not None and stats.accepted > 0:
-        output.save_state(stats.t, work)
+    if output is not None:
+        for k in range(min(stats.n_states, stats.n_times)):
+            output.save_state(times[k], states[k])
     return res
 
 
@@ -453,3 +460,49 @@ class _Sub:
         self.base = base
         self.size = size
         self.ptr = C.c_void_p(base.ptr.value + 8 * offset)
+
+
+def leapfrog(x0, t0, tf, dt, ode, output=None, exe=None, *, states_cap=None, x_out=None,
+             fetch=True):
+    """ParallelLeapfrogIntegrator::integrate (``LeapfrogParallel.hpp:8-41``):
+    fixed-step velocity Verlet of an N-body state on the device, the
+    accelerations from ``ode`` (an :class:`NBodyODE`).  ``output`` (if given)
+    receives ``save_state(t, x)`` for the initial state and after every step,
+    as in the reference; ``states_cap`` limits how many are kept (default:
+    all when ``output`` is given, none otherwise).  Returns
+    ``(x_final, stats)`` with stats a dict (steps, rhs_evals, t).
+    """
+    if not isinstance(ode, NBodyODE):
+        raise TypeError("leapfrog needs an NBodyODE")
+    exe = exe or HipExecutor()
+    x0_host = None if isinstance(x0, HipState) else np.ascontiguousarray(x0, np.float64)
+    n = x0.size if isinstance(x0, HipState) else x0_host.size
+    work = x_out if x_out is not None else HipState(exe, n)
+    if isinstance(x0, HipState):
+        work.assign(x0)
+    else:
+        work.upload(x0_host)
+    if states_cap is None:
+        if output is not None:
+            t, steps = float(t0), 0
+            while t < tf:  # the reference's time loop
+                t += dt
+                steps += 1
+            states_cap = steps + 1
+        else:
+            states_cap = 0
+    states = np.zeros((max(int(states_cap), 1), n))
+    times = np.zeros(max(int(states_cap), 1))
+    st = _lib.LeapfrogStats()
+    ode_s = ode.ode_struct(exe, n)
+    exe.check(lib().odehip_leapfrog(exe.handle, C.byref(ode_s), float(t0), float(tf),
+                                    float(dt), work.ptr,
+                                    dptr(states) if states_cap > 0 else None,
+                                    dptr(times) if states_cap > 0 else None,
+                                    int(states_cap), C.byref(st)), "leapfrog")
+    if output is not None:
+        for k in range(st.n_states):
+            output.save_state(times[k], states[k])
+    x_final = work.copy_to_span() if fetch else None
+    return x_final, {"steps": st.steps, "rhs_evals": st.rhs_evals, "t": st.t,
+                     "n_states": st.n_states}

@@ -36,6 +36,8 @@ struct SolverState {
   int n_groups;  // reduction groups of the error norm
   int controller;      // ODEHIP_CONTROLLER_*
   int rejects_in_row;  // consecutive rejections up to now
+  int just_accepted;   // the last judged attempt was accepted
+  int pad;
 };
 
 // Arguments of the N-body RHS kernels.  Exactly one of y / (xb0, xb1, st)
@@ -106,8 +108,10 @@ struct SolveBufs {
   double* partB;
   double* times;
   double* trace;
+  double* states;  // accepted states (states_cap x n_var), or null
   long times_cap;
   long trace_cap;
+  long states_cap;
   const double* atol;
   const double* rtol;
   size_t n_var;

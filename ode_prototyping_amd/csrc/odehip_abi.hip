@@ -76,7 +76,7 @@ int check_ode(odehip_context* ctx, const odehip_ode* ode) {
 
 // Lays out the solve workspace and returns its length in doubles.
 size_t layout(SolveBufs* b, double* base, SolverState* st, size_t n, int S,
-              long times_cap, long trace_cap) {
+              long times_cap, long trace_cap, long states_cap) {
   const size_t ng = static_cast<size_t>(n_groups_for(n));
   size_t off = 0;
   auto take = [&](size_t len) {
@@ -96,20 +96,22 @@ size_t layout(SolveBufs* b, double* base, SolverState* st, size_t n, int S,
   b->partB = take(ng);
   b->times = times_cap > 0 ? take(static_cast<size_t>(times_cap)) : nullptr;
   b->trace = trace_cap > 0 ? take(3 * static_cast<size_t>(trace_cap)) : nullptr;
+  b->states = states_cap > 0 ? take(static_cast<size_t>(states_cap) * n) : nullptr;
   b->times_cap = times_cap;
   b->trace_cap = trace_cap;
+  b->states_cap = states_cap;
   return off;
 }
 
 int prepare_bufs(odehip_context* ctx, int tableau, const odehip_ode* ode,
                  const double* atol, const double* rtol, long times_cap,
-                 long trace_cap, SolveBufs* b) {
+                 long trace_cap, long states_cap, SolveBufs* b) {
   const size_t n = ode->n_var;
   const int S = kTableaus[tableau].S;
-  size_t need = layout(b, nullptr, ctx->st, n, S, times_cap, trace_cap);
+  size_t need = layout(b, nullptr, ctx->st, n, S, times_cap, trace_cap, states_cap);
   int rc = ensure_ws(ctx, need);
   if (rc) return rc;
-  layout(b, ctx->ws, ctx->st, n, S, times_cap, trace_cap);
+  layout(b, ctx->ws, ctx->st, n, S, times_cap, trace_cap, states_cap);
   b->atol = atol;
   b->rtol = rtol;
   b->n_var = n;
@@ -417,7 +419,7 @@ int odehip_rk_initial_step(odehip_context* ctx, int tableau, int rhs_kind,
   int rc = check_ode(ctx, &ode);
   if (rc) return rc;
   SolveBufs b{};
-  rc = prepare_bufs(ctx, tableau, &ode, atol, rtol, 0, 0, &b);
+  rc = prepare_bufs(ctx, tableau, &ode, atol, rtol, 0, 0, 0, &b);
   if (rc) return rc;
   hipStream_t s = begin_work(ctx);
   HIP_TRY(ctx, enqueue_solve_init(b, x0, 0.0, 1.0, 0, 0, 0, s));
@@ -440,7 +442,7 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
   if (rc) return rc;
   if (!opt->atol || !opt->rtol) return fail(ctx, ODEHIP_ERR_INVALID, "atol/rtol null");
   if (opt->max_accepted < 0 || opt->max_attempts < 0 || opt->times_cap < 0 ||
-      opt->trace_cap < 0)
+      opt->trace_cap < 0 || opt->states_cap < 0)
     return fail(ctx, ODEHIP_ERR_INVALID, "negative budget or capacity");
   if (opt->controller < ODEHIP_CONTROLLER_REFERENCE ||
       opt->controller > ODEHIP_CONTROLLER_LEARNED)
@@ -451,9 +453,10 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
                 "the learned controller needs a network and an N-body problem");
   const long times_cap = opt->times ? opt->times_cap : 0;
   const long trace_cap = opt->trace ? opt->trace_cap : 0;
+  const long states_cap = opt->states ? opt->states_cap : 0;
   SolveBufs b{};
   rc = prepare_bufs(ctx, tableau, ode, opt->atol, opt->rtol, times_cap,
-                    trace_cap, &b);
+                    trace_cap, states_cap, &b);
   if (rc) return rc;
   b.controller = opt->controller;
   b.mlp = opt->controller == ODEHIP_CONTROLLER_LEARNED ? mlp_infer_ptr(opt->mlp) : nullptr;
@@ -488,7 +491,8 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
     key.rtol = opt->rtol;
     key.ws = ctx->ws;
     // times/trace pointers live in ws; their presence changes the layout:
-    key.ws = reinterpret_cast<const char*>(ctx->ws) + (b.times ? 1 : 0) + (b.trace ? 2 : 0);
+    key.ws = reinterpret_cast<const char*>(ctx->ws) + (b.times ? 1 : 0) + (b.trace ? 2 : 0) +
+             (b.states ? 4 : 0);
     key.controller = b.controller;
     key.mlp = b.mlp;
     if (!ctx->exec || !(key == ctx->key)) {
@@ -545,6 +549,10 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
   const SolverState& hs = *ctx->h_st;
   const long n_times = std::min(hs.accepted + 1, times_cap);
   const long n_trace = std::min(hs.attempts, trace_cap);
+  const long n_states = std::min(hs.accepted + 1, states_cap);
+  if (opt->states && n_states > 0)
+    HIP_TRY(ctx, hipMemcpy(opt->states, b.states, static_cast<size_t>(n_states) * n * sizeof(double),
+                           hipMemcpyDeviceToHost));
   if (opt->times && n_times > 0)
     HIP_TRY(ctx, hipMemcpy(opt->times, b.times, n_times * sizeof(double), hipMemcpyDeviceToHost));
   if (opt->trace && n_trace > 0)
@@ -563,6 +571,7 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
     stats->dt_initial = hs.dt_initial;
     stats->last_err = hs.err;
     stats->learned_steps = hs.learned_steps;
+    stats->n_states = n_states;
   }
   if (hs.status == ODEHIP_STATUS_RUNNING)
     return fail(ctx, ODEHIP_ERR_INVALID, "solve did not finish within the attempt cap");

@@ -125,6 +125,7 @@ __device__ inline void ctrl_init(SolverState* st, const Tableau& T, double t0,
   st->k1_valid = 0;
   st->controller = controller;
   st->rejects_in_row = 0;
+  st->just_accepted = 0;
   st->status = ODEHIP_STATUS_RUNNING;
   st->done = (t0 < tf) ? 0 : 1;
   if (st->done) st->status = ODEHIP_STATUS_REACHED_END;
@@ -147,6 +148,7 @@ __device__ inline bool ctrl_judge(SolverState* st, const Tableau& T,
   const bool finite = std::isfinite(scaled_error);
   const bool accept = finite && scaled_error <= 1.0;
   *accepted_out = accept;
+  st->just_accepted = accept ? 1 : 0;
   if (trace && attempt < st->trace_cap) {
     trace[3 * attempt] = scaled_error;
     trace[3 * attempt + 1] = dt;

@@ -237,6 +237,21 @@ __global__ void k_init_step_b(SolverState* st, const double* pa, size_t n_var) {
   ctrl_init_step_b(st, c_tab[st->tableau], sa, n_var);
 }
 
+// RawOutput::save_state of an accepted step (RawOutput.hpp:15-19): the new
+// x0 into slot `accepted` of the device state list.  Runs after every
+// controller launch; once a solve is done it rewrites the same slot with the
+// same state (harmless).
+__global__ void k_save_state(const SolverState* st, const double* xb0,
+                             const double* xb1, double* states, long cap,
+                             size_t n_var) {
+  if (!st->just_accepted) return;
+  const long slot = st->accepted;
+  if (slot >= cap) return;
+  const size_t i = gid();
+  const double* x0 = st->cur ? xb1 : xb0;
+  if (i < n_var) states[static_cast<size_t>(slot) * n_var + i] = x0[i];
+}
+
 // Final state out: x_out = the x0 buffer.
 __global__ void k_copy_out(const SolverState* st, const double* xb0,
                            const double* xb1, double* out, size_t n_var) {
@@ -388,6 +403,10 @@ hipError_t enqueue_solve_init(const SolveBufs& b, const double* x_in,
   hipError_t e = hipMemcpyAsync(b.xb0, x_in, b.n_var * sizeof(double),
                                 hipMemcpyDeviceToDevice, s);
   if (e != hipSuccess) return e;
+  if (b.states && b.states_cap > 0) {
+    e = hipMemcpyAsync(b.states, x_in, b.n_var * sizeof(double), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(k_init_state, dim3(1), dim3(64), 0, s, b.st, t0, tf,
                      max_acc, max_att, fsal, b.tableau, n_groups_for(b.n_var),
                      b.times, b.times_cap, b.trace_cap, b.controller);
@@ -435,6 +454,9 @@ hipError_t enqueue_attempt(const SolveBufs& b, hipStream_t s) {
     hipLaunchKernelGGL(k_controller, dim3(1), dim3(64), 0, s, b.st, b.partA, n, b.times,
                        b.trace);
   }
+  if (b.states && b.states_cap > 0)
+    hipLaunchKernelGGL(k_save_state, dim3(blocks_for(n)), dim3(kBlock), 0, s, b.st, b.xb0,
+                       b.xb1, b.states, b.states_cap, n);
   return hipGetLastError();
 }
 

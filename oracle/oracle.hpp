@@ -428,6 +428,44 @@ inline Result integrate(const Tableau& bt, const double* x0_in, size_t n_var,
   return res;
 }
 
+// ---------------------------------------------------------------- leapfrog --
+// ParallelLeapfrogIntegrator::integrate (LeapfrogParallel.hpp:8-41, kernels
+// :57-73): velocity Verlet on a [pos | vel] state with the N-body
+// accelerations (NBodyODE::acc, NBodyODE.hpp:41-46 -- the RHS's second half);
+// fixed dt, no clip of the last step, every state saved.
+inline Result leapfrog(const double* x0_in, size_t n_var, double t0, double tf,
+                       double dt, const Rhs& ode, const Executor& exe,
+                       bool store_states) {
+  Result res;
+  res.n_var = n_var;
+  const size_t n_dof = n_var / 2;
+  std::vector<double> x(x0_in, x0_in + n_var), d0(n_var), d1(n_var);
+  auto save = [&](double tt) {
+    res.times.push_back(tt);
+    if (store_states) res.states.insert(res.states.end(), x.begin(), x.end());
+  };
+  double t = t0;
+  save(t);
+  ode(x.data(), d0.data(), exe);  // acc0 = d0[n_dof:]
+  ++res.rhs_evals;
+  while (t < tf) {
+    for (size_t i = 0; i < n_dof; ++i)
+      x[i] += dt * (x[n_dof + i] + 0.5 * dt * d0[n_dof + i]);
+    ode(x.data(), d1.data(), exe);
+    ++res.rhs_evals;
+    for (size_t i = 0; i < n_dof; ++i)
+      x[n_dof + i] += 0.5 * dt * (d0[n_dof + i] + d1[n_dof + i]);
+    std::swap(d0, d1);  // acc0 = acc1
+    t += dt;
+    ++res.accepted;
+    save(t);
+  }
+  res.t = t;
+  res.dt = dt;
+  res.x_final = x;
+  return res;
+}
+
 // ------------------------------------------------- NR-compatible DOP853 ----
 // ODEIntegrator<StepperDopr853<D, RawOutput>> (NumericalRecipes/
 // ODEIntegrator.hpp:66-88, StepperDopr853.hpp:93-197, 291-342): the NR

@@ -91,6 +91,21 @@ void* oracle_integrate(const char* tableau, int rhs_kind, const double* x0,
   }
 }
 
+// Leapfrog (velocity Verlet) of an N-body state; the result handle is read
+// with the oracle_result_* accessors (accepted = steps).
+void* oracle_leapfrog(const double* x0, size_t n_var, double t0, double tf,
+                      double dt, double softening, const double* masses,
+                      int store_states, int threads) {
+  try {
+    Executor e = make_exe(0, 0, threads);
+    return new Result(leapfrog(x0, n_var, t0, tf, dt,
+                               make_rhs(kRhsNBody, n_var, softening, masses), e,
+                               store_states != 0));
+  } catch (...) {
+    return nullptr;
+  }
+}
+
 long oracle_result_count(void* h) {
   return static_cast<long>(static_cast<Result*>(h)->times.size());
 }

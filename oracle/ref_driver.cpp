@@ -15,6 +15,7 @@
 #include "BTDOPRI5.hpp"
 #include "BTRKF78.hpp"
 #include "HeapState.hpp"
+#include "LeapfrogParallel.hpp"
 #include "NBodyODE.hpp"
 #include "RKEmbeddedParallel.hpp"
 #include "SingleThreadedExecutor.hpp"
@@ -151,6 +152,26 @@ long ref_integrate_nbody64(int tableau, const double* x0, double t0, double tf,
                                    x_out, times_out, times_cap);
   return run_nbody<BTDOP853Ref, NV>(exe, x0, t0, tf, tol, softening, 1.0, 0,
                                     x_out, times_out, times_cap);
+}
+
+// Reference leapfrog (ParallelLeapfrogIntegrator, single-threaded) of an
+// N=64 unit-mass system; returns the number of saved states.
+long ref_leapfrog_nbody64(const double* x0, double t0, double tf, double dt,
+                          double softening, double* x_out, double* times_out,
+                          long times_cap) {
+  SingleThreadedExecutor exe;
+  constexpr std::size_t NV = 64 * 6;
+  auto init = std::make_unique<std::array<double, NV>>();
+  std::copy(x0, x0 + NV, init->begin());
+  auto xs = HeapState<double, NV>{*init};
+  CollectOutput<NV> out;
+  NBodyODE<double, NV> ode{softening};
+  ParallelLeapfrogIntegrator::integrate(xs, t0, tf, dt, ode, out, exe);
+  if (x_out) std::copy(out.last.begin(), out.last.end(), x_out);
+  long n = static_cast<long>(out.times.size());
+  if (times_out)
+    std::copy(out.times.begin(), out.times.begin() + std::min(n, times_cap), times_out);
+  return n;
 }
 
 // CPU baseline: the reference's ThreadPoolExecutor path (pair-parallel RHS

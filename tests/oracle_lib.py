@@ -58,6 +58,9 @@ def lib():
         L.oracle_spinning_cube_ic.argtypes = [C.c_long, _dp, _dp]
         L.oracle_pow.argtypes = [C.c_double, C.c_double, C.c_int]
         L.oracle_pow.restype = C.c_double
+        L.oracle_leapfrog.argtypes = [_dp, C.c_size_t, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, _dp, C.c_int, C.c_int]
+        L.oracle_leapfrog.restype = C.c_void_p
         L.oracle_mlp_forward.argtypes = [_dp, _dp, _dp, _dp, C.c_long, C.c_int, _dp]
         L.oracle_mlp_loss_grad.argtypes = [_dp, _dp, _dp, _dp, _dp, C.c_long, _dp]
         L.oracle_mlp_loss_grad.restype = C.c_double
@@ -122,6 +125,17 @@ def integrate(tableau, rhs_kind, x0, t0, tf, atol, rtol, softening=0.0, masses=N
     if not h:
         raise RuntimeError("oracle_integrate failed")
     return Result(h, n)
+
+
+def leapfrog(x0, t0, tf, dt, softening=0.0, masses=None, store_states=True, threads=1):
+    """ParallelLeapfrogIntegrator on an N-body state (accepted = steps)."""
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    m = None if masses is None else np.ascontiguousarray(masses, dtype=np.float64)
+    h = lib().oracle_leapfrog(_ptr(x0), x0.size, t0, tf, dt, softening, _ptr(m),
+                              int(store_states), threads)
+    if not h:
+        raise RuntimeError("oracle_leapfrog failed")
+    return Result(h, x0.size)
 
 
 def rhs(rhs_kind, x, softening=0.0, masses=None, threads=1):

@@ -269,3 +269,26 @@ def test_energy_drift_matches_oracle(exe):
     assert abs((e1 - e0) / e0) < 1e-9
     p1 = O.momentum(res.x_final, n, m)
     assert np.all(np.abs(p1) < 1e-14)
+
+
+def test_device_solve_raw_output_every_state(exe):
+    """The device solve keeps every accepted state on the device (RawOutput
+    semantics, RawOutput.hpp:15-19): each one equals the oracle's bit for
+    bit; a smaller states_cap keeps the first ones."""
+    import ode_prototyping_amd as oh
+    x0, m = oh.scenarios.uniform_cube(80, seed=6)
+    ref = O.integrate("DOPRI5", O.RHS_NBODY, x0, 0.0, 10.0, 1e-10, 1e-10, softening=1e-3,
+                      masses=m, reduce=O.HIP_ORDER, pow_mode=O.POW_CORRECT, max_accepted=40,
+                      threads=8)
+    out = oh.RawOutput()
+    r = oh.integrate("DOPRI5", x0, 0.0, 10.0, 1e-10, 1e-10, oh.NBodyODE(1e-3, m, "exact"),
+                     out, exe=exe, max_accepted=40)
+    assert len(out.times()) == len(ref.times) == r.accepted + 1
+    assert np.array_equal(bits(out.times()), bits(ref.times))
+    for k in range(len(ref.times)):
+        assert np.array_equal(bits(out.states()[k]), bits(ref.state(k))), k
+    out2 = oh.RawOutput()
+    oh.integrate("DOPRI5", x0, 0.0, 10.0, 1e-10, 1e-10, oh.NBodyODE(1e-3, m, "exact"), out2,
+                 exe=exe, max_accepted=40, states_cap=7)
+    assert len(out2.states()) == 7
+    assert np.array_equal(bits(out2.states()[6]), bits(ref.state(6)))

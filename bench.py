@@ -161,7 +161,33 @@ def ensemble_block(oh, exe):
         block[name]["energy_rel_drift_64_systems"] = {"median": float(np.median(drift)),
                                                       "max": float(np.max(drift))}
     net.close()
+    block["cpu_baseline"] = ensemble_cpu_baseline(sizes, x0, m, eps, tol, tab)
+    log(f"ensemble cpu baseline: {block['cpu_baseline']}")
     return block
+
+
+def ensemble_cpu_baseline(sizes, x0, m, eps, tol, tab, tf=0.005, count=16):
+    """The oracle restatement of the reference algorithm (per-system solves,
+    thread-parallel gather RHS) on a bounded sample of the ensemble."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from ode_prototyping_amd.ensemble import unpack
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    order = np.argsort(sizes, kind="stable")
+    idx = order[np.linspace(0, len(sizes) - 1, count).astype(int)]
+    xs, ms = unpack(x0, sizes), unpack(m, sizes, per_particle=1)
+    t = time.perf_counter()
+    steps = 0.0
+    for k in idx:
+        r = O.integrate(tab, O.RHS_NBODY, xs[k], 0.0, tf, tol, tol, softening=eps,
+                        masses=ms[k], store_states=False, threads=threads)
+        steps += float(sizes[k]) * r.accepted
+    secs = time.perf_counter() - t
+    return {"value": steps / secs, "unit": "particle-steps/s", "cores": threads,
+            "kind": "port", "seconds": secs,
+            "sample": f"oracle restatement ({threads} threads), {count} systems spread over "
+                      f"the size order (N {int(sizes[idx].min())}..{int(sizes[idx].max())}), "
+                      f"each solved alone for t in [0, {tf}]"}
 
 
 def controller_block(oh, exe):

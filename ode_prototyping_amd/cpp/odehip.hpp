@@ -23,6 +23,11 @@
 //                          integrate_on_device: the same solve with the attempt
 //                          loop and controller resident on the GPU.
 //   fill                   src/state/StateUtils.hpp:5-16
+//   ParallelLeapfrogIntegrator::integrate   src/integrator/leapfrog/
+//                          LeapfrogParallel.hpp:8-41 (device loop, every
+//                          state returned to the Output)
+//   solve_ensemble         many RKEmbeddedParallel::integrate calls at once
+//                          (odehip_ensemble_solve; no reference counterpart)
 // Errors: the reference prints CUDA errors and continues
 // (src/cuda_util/CudaErrorCheck.cuh:8-16); here every failing C-ABI call
 // throws odehip::Error with the library's message.
@@ -373,5 +378,59 @@ struct RKEmbeddedParallel {
     return std::min(100.0 * dt0, dt1);
   }
 };
+
+// A saved host state in the shape Output::save_state expects (copy_to_span).
+template <std::size_t N>
+struct HostStateView {
+  const double* p;
+  void copy_to_span(std::span<double, N> out) const { std::copy(p, p + N, out.begin()); }
+};
+
+// ParallelLeapfrogIntegrator::integrate (LeapfrogParallel.hpp:8-41): fixed-step
+// velocity Verlet on the device (odehip_leapfrog); output.save_state receives
+// the initial state and the state after every step, as in the reference.
+struct ParallelLeapfrogIntegrator {
+  template <std::size_t N, typename Output>
+  static void integrate(HipState<double, N> x0, double t0, double tf, double dt,
+                        const NBodyODE<double, N>& ode, Output& output, HipExecutor& exe) {
+    long steps = 0;
+    for (double t = t0; t < tf;) {  // the reference's time loop
+      t += dt;
+      ++steps;
+    }
+    std::vector<double> states(static_cast<std::size_t>(steps + 1) * N), times(steps + 1);
+    odehip_ode o = ode.describe();
+    odehip_leapfrog_stats st{};
+    exe.check(odehip_leapfrog(exe.context(), &o, t0, tf, dt, x0.data(), states.data(),
+                              times.data(), steps + 1, &st),
+              "leapfrog");
+    for (long k = 0; k < st.n_states; ++k)
+      output.save_state(times[k], HostStateView<N>{states.data() + k * N});
+  }
+};
+
+// Many independent N-body systems of different sizes in one device solve
+// (odehip_ensemble_solve): states packed system after system, each system's
+// result bit-identical to integrate_on_device on it alone.
+struct EnsembleResult {
+  odehip_ensemble_stats totals{};
+  std::vector<odehip_system_stats> systems;
+};
+
+template <typename BT>
+inline EnsembleResult solve_ensemble(HipExecutor& exe, const std::vector<int>& sizes,
+                                     const double* x0_dev, double* x_out_dev,
+                                     const odehip_ensemble_options& opt, double softening,
+                                     const double* masses_dev = nullptr,
+                                     NBodyArithmetic arithmetic = NBodyArithmetic::exact) {
+  EnsembleResult r;
+  r.systems.resize(sizes.size());
+  exe.check(odehip_ensemble_solve(exe.context(), BT::id, static_cast<int>(sizes.size()),
+                                  sizes.data(), static_cast<int>(arithmetic), softening,
+                                  masses_dev, &opt, x0_dev, x_out_dev, &r.totals,
+                                  r.systems.data()),
+            "ensemble_solve");
+  return r;
+}
 
 }  // namespace odehip

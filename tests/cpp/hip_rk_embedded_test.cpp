@@ -292,6 +292,75 @@ static void state_semantics() {
   std::printf("ok state_semantics\n");
 }
 
+// Leapfrog through the mirror: one saved state per step (t0 included), the
+// last equal to a run without output, energy kept (two-body orbit).
+static void leapfrog_two_body() {
+  HipExecutor exe{};
+  constexpr std::size_t N = 12;
+  // masses 1 and 1e-3 (device), circular-ish orbit
+  HipState<double, 2> m{std::array{1.0, 1.0e-3}};
+  const double v = std::sqrt(1.001);
+  HipState<double, N> x0{std::array{0.0, 0.0, 0.0, 1.0, 0.0, 0.0,
+                                    0.0, 0.0, 0.0, 0.0, v, 0.0}};
+  NBodyODE<double, N> ode{0.0, m.data()};
+  RawOutput<N> out{};
+  ParallelLeapfrogIntegrator::integrate(x0, 0.0, 1.0, 0.01, ode, out, exe);
+  EXPECT_EQ(out.times().size(), 101u);
+  EXPECT_EQ(out.times()[0], 0.0);
+  const auto& last = out.states().back();
+  auto energy = [](const std::array<double, N>& s) {
+    const double dx = s[3] - s[0], dy = s[4] - s[1], dz = s[5] - s[2];
+    const double ke = 0.5 * (s[6] * s[6] + s[7] * s[7] + s[8] * s[8]) +
+                      0.5e-3 * (s[9] * s[9] + s[10] * s[10] + s[11] * s[11]);
+    return ke - 1.0e-3 / std::sqrt(dx * dx + dy * dy + dz * dz);
+  };
+  EXPECT_NEAR(energy(last), energy(out.states()[0]), 1e-7);
+}
+
+// The ensemble mirror: two systems, each equal to integrate_on_device alone.
+static void ensemble_matches_single() {
+  HipExecutor exe{};
+  constexpr std::size_t N1 = 6 * 7, N2 = 6 * 40;
+  std::mt19937_64 gen(5);
+  std::uniform_real_distribution<double> u(-1.0, 1.0);
+  std::array<double, N1> a{};
+  std::array<double, N2> b{};
+  for (auto& v : a) v = 0.1 * u(gen);
+  for (auto& v : b) v = 0.1 * u(gen);
+  for (std::size_t i = 0; i < N1 / 2; ++i) a[i] *= 10.0;
+  for (std::size_t i = 0; i < N2 / 2; ++i) b[i] *= 10.0;
+  std::array<double, N1 + N2> packed{};
+  std::copy(a.begin(), a.end(), packed.begin());
+  std::copy(b.begin(), b.end(), packed.begin() + N1);
+  HipState<double, N1 + N2> x0{packed}, xo{};
+  odehip_ensemble_options opt{};
+  opt.t0 = 0.0;
+  opt.tf = 0.2;
+  opt.atol = opt.rtol = 1e-10;
+  opt.tile_config = ODEHIP_ENS_TILE_AUTO;
+  auto r = solve_ensemble<BTDOPRI5>(exe, {7, 40}, x0.data(), xo.data(), opt, 1e-3);
+  EXPECT_EQ(r.totals.n_unfinished, 0);
+  std::array<double, N1 + N2> got{};
+  xo.copy_to_span(std::span<double, N1 + N2>{got});
+  HipState<double, N1> s1{a}, t1{std::array<double, N1>{}};
+  fill(exe, t1, 1e-10);
+  HipState<double, N2> s2{b}, t2{std::array<double, N2>{}};
+  fill(exe, t2, 1e-10);
+  auto st1 = RKEmbeddedParallel<BTDOPRI5>::integrate_on_device(
+      s1, 0.0, 0.2, t1, t1, NBodyODE<double, N1>{1e-3}, exe);
+  auto st2 = RKEmbeddedParallel<BTDOPRI5>::integrate_on_device(
+      s2, 0.0, 0.2, t2, t2, NBodyODE<double, N2>{1e-3}, exe);
+  EXPECT_EQ(r.systems[0].accepted, st1.accepted);
+  EXPECT_EQ(r.systems[1].rejected, st2.rejected);
+  std::array<double, N1> o1{};
+  std::array<double, N2> o2{};
+  s1.copy_to_span(std::span<double, N1>{o1});
+  s2.copy_to_span(std::span<double, N2>{o2});
+  bool same = std::equal(o1.begin(), o1.end(), got.begin()) &&
+              std::equal(o2.begin(), o2.end(), got.begin() + N1);
+  EXPECT_EQ(same, true);
+}
+
 int main() {
   state_semantics();
   van_der_pol_consistency<BTDOPRI5>({"DOPRI5_VanDerPol", 76, 0.67149697578171041,
@@ -310,6 +379,8 @@ int main() {
   rkf78_nbody();
   rkf78_spinning_cube();
   device_resident_van_der_pol();
+  leapfrog_two_body();
+  ensemble_matches_single();
   std::printf("%d checks, %d failures\n", g_checks, g_failures);
   return g_failures == 0 ? 0 : 1;
 }

@@ -72,8 +72,7 @@ __device__ inline void tile_store(double* __restrict__ k, int n, int i0,
 // exact: every target sums its partners' terms in increasing j, skipping
 // j == i (NBodyODE.hpp:89-98 with the order of the single-threaded loop).
 // Terms of U partners are computed together; the shared-reciprocal form is
-// used when every lane's displacements are in its range (the diagonal
-// partner's term is discarded and does not count).
+// used when all of the system's coordinates pass coord_shared_ok.
 template <bool kMass, bool kSharedRcp, int TPL>
 __device__ inline void exact_tile(const double* __restrict__ y,
                                   const double* __restrict__ mass,
@@ -95,6 +94,13 @@ __device__ inline void exact_tile(const double* __restrict__ y,
     ax[t] = ay[t] = az[t] = 0.0;
   }
   double px[U], py[U], pz[U], pm[U];
+  // the shared-reciprocal form when every position coordinate of the system
+  // passes coord_shared_ok (one pass of the wave over its 3n coordinates)
+  bool shared_ok = kSharedRcp;
+  if (kSharedRcp) {
+    for (int e = lane; e < 3 * n; e += 64) shared_ok = shared_ok & coord_shared_ok(y[e]);
+    shared_ok = __all(shared_ok);
+  }
   auto load = [&](int j) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -117,7 +123,6 @@ __device__ inline void exact_tile(const double* __restrict__ y,
     }
     if (j + 2 * U <= n) load(j + U);  // next partners, one group ahead
     double dx[U][TPL], dy[U][TPL], dz[U][TPL];
-    bool ok = true;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
 #pragma unroll
@@ -125,13 +130,10 @@ __device__ inline void exact_tile(const double* __restrict__ y,
         dx[u][t] = cx[u] - xi[t];
         dy[u][t] = cy[u] - yi[t];
         dz[u][t] = cz[u] - zi[t];
-        if (kSharedRcp)
-          ok = ok & (in_shared_rcp_range(dx[u][t], dy[u][t], dz[u][t]) |
-                     (j + u == ic[t]));
       }
     }
     double tx[U][TPL], ty[U][TPL], tz[U][TPL];
-    if (kSharedRcp && __all(ok)) {
+    if (kSharedRcp && shared_ok) {
 #pragma unroll
       for (int u = 0; u < U; ++u)
 #pragma unroll

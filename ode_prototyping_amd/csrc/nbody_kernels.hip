@@ -72,6 +72,15 @@ __device__ inline void copy_velocities(const double* __restrict__ y,
   if (e < 192 && base + e < lim) k[base + e] = y[lim + base + e];
 }
 
+// Whether every position coordinate of the state passes coord_shared_ok:
+// all threads of the workgroup test a strided share of the 3N coordinates.
+__device__ inline bool all_positions_shared_ok(const double* __restrict__ y, int n) {
+  bool ok = true;
+  const long lim = 3L * n;
+  for (long e = threadIdx.x; e < lim; e += blockDim.x) ok = ok & coord_shared_ok(y[e]);
+  return __syncthreads_and(ok) != 0;
+}
+
 // ------------------------------------------------------------------ exact --
 constexpr int kExactWaves = 16;
 constexpr int kExactJW = 2;  // partners per wave per tile
@@ -100,6 +109,9 @@ __global__ __launch_bounds__(64 * kExactWaves) void k_nbody_exact(NBodyArgs a) {
   // Partner coordinates of the next tile are loaded one tile ahead (scalar
   // loads), so their latency hides behind the current tile's arithmetic.
   double px[kExactJW], py[kExactJW], pz[kExactJW], pm[kExactJW];
+  // the shared-reciprocal form is used when every position coordinate passes
+  // coord_shared_ok (one pass over the 3N coordinates per workgroup)
+  const bool shared_ok = kSharedRcp && all_positions_shared_ok(y, n);
   auto load_partners = [&](int t) {
     const int jb = t * kExactTJ + wave * kExactJW;
     if (jb + kExactJW <= n) {
@@ -144,18 +156,16 @@ __global__ __launch_bounds__(64 * kExactWaves) void k_nbody_exact(NBodyArgs a) {
     }
     if (t < n_tiles) {
       double dx[kExactJW], dy[kExactJW], dz[kExactJW], mj[kExactJW];
-      bool ok = true;
 #pragma unroll
       for (int jj = 0; jj < kExactJW; ++jj) {
         dx[jj] = px[jj] - xi;
         dy[jj] = py[jj] - yi;
         dz[jj] = pz[jj] - zi;
         mj[jj] = pm[jj];
-        if (kSharedRcp) ok = ok & in_shared_rcp_range(dx[jj], dy[jj], dz[jj]);
       }
       if (t + 1 < n_tiles) load_partners(t + 1);
       double ax[kExactJW], ay[kExactJW], az[kExactJW];
-      if (kSharedRcp && __all(ok)) {
+      if (kSharedRcp && shared_ok) {
 #pragma unroll
         for (int jj = 0; jj < kExactJW; ++jj)
           exact_term_shared_rcp<kMass>(dx[jj], dy[jj], dz[jj], mj[jj], softening_sq,
@@ -177,6 +187,96 @@ __global__ __launch_bounds__(64 * kExactWaves) void k_nbody_exact(NBodyArgs a) {
   }
   if (wave < 3 && valid_i) k[3L * n + 3 * i + wave] = acc;
   copy_velocities(y, k, n, i0);
+}
+
+// The same exact sum with 32-target tiles: twice the workgroups and half
+// the LDS, for N too small to fill the chip with 64-target tiles.  Lane l evaluates target
+// l % 32 against partner slot l / 32 of its wave; the partner coordinates of
+// both slots arrive through scalar loads and each lane selects its own.  Two
+// waves fold a tile's terms in partner order: wave 0 the x (lanes 0-31) and
+// y (lanes 32-63) chains, wave 1 the z chains.
+constexpr int kE32Waves = 16;
+constexpr int kE32TJ = 2 * kE32Waves;  // partners per tile
+
+template <bool kMass, bool kSharedRcp>
+__global__ __launch_bounds__(64 * kE32Waves) void k_nbody_exact32(NBodyArgs a) {
+  if (prologue_skip(a)) return;
+  __shared__ double terms[2][kE32TJ][3][32];
+  const double* __restrict__ y = select_input(a);
+  const uniform_ptr yu = as_uniform(y);
+  double* __restrict__ k = a.k;
+  const uniform_ptr massu = as_uniform(a.mass);
+  const int n = a.n;
+  const double softening_sq = a.eps2;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int tgt = lane & 31;
+  const int slot = lane >> 5;
+  const int i0 = blockIdx.x * 32;
+  const int i = i0 + tgt;
+  const int ic = i < n ? i : n - 1;
+  const double xi = y[3 * ic], yi = y[3 * ic + 1], zi = y[3 * ic + 2];
+  // fold role: wave 0 lanes 0-31 -> x, 32-63 -> y; wave 1 lanes 0-31 -> z
+  const int fc = wave == 0 ? slot : 2;
+  const bool folds = (wave == 0) || (wave == 1 && slot == 0);
+  double acc = 0.0;
+  const int n_tiles = (n + kE32TJ - 1) / kE32TJ;
+  double pa[4], pb[4];
+  const bool shared_ok = kSharedRcp && all_positions_shared_ok(y, n);
+  auto load_partners = [&](int t) {
+    const int ja = min(t * kE32TJ + 2 * wave, n - 1);
+    const int jb = min(t * kE32TJ + 2 * wave + 1, n - 1);
+    pa[0] = yu[3 * ja];
+    pa[1] = yu[3 * ja + 1];
+    pa[2] = yu[3 * ja + 2];
+    pa[3] = kMass ? massu[ja] : 1.0;
+    pb[0] = yu[3 * jb];
+    pb[1] = yu[3 * jb + 1];
+    pb[2] = yu[3 * jb + 2];
+    pb[3] = kMass ? massu[jb] : 1.0;
+  };
+  load_partners(0);
+  for (int t = 0; t <= n_tiles; ++t) {
+    if (t > 0 && wave < 2) {
+      const int base = (t - 1) * kE32TJ;
+      const int cnt = min(kE32TJ, n - base);
+      const double* src = &terms[(t - 1) & 1][0][fc][tgt];
+      if (folds) {
+        if (cnt == kE32TJ && (base + kE32TJ <= i0 || base >= i0 + 32)) {
+#pragma unroll
+          for (int m = 0; m < kE32TJ; ++m) acc += src[m * 96];
+        } else {
+          for (int m = 0; m < cnt; ++m) {
+            const double v = src[m * 96];
+            if (base + m != i) acc += v;
+          }
+        }
+      }
+    }
+    if (t < n_tiles) {
+      const double px = slot ? pb[0] : pa[0];
+      const double py = slot ? pb[1] : pa[1];
+      const double pz = slot ? pb[2] : pa[2];
+      const double pm = slot ? pb[3] : pa[3];
+      if (t + 1 < n_tiles) load_partners(t + 1);
+      const double dx = px - xi, dy = py - yi, dz = pz - zi;
+      double ax, ay, az;
+      if (kSharedRcp && shared_ok)
+        exact_term_shared_rcp<kMass>(dx, dy, dz, pm, softening_sq, ax, ay, az);
+      else
+        exact_term_ieee<kMass>(dx, dy, dz, pm, softening_sq, ax, ay, az);
+      const int mm = 2 * wave + slot;
+      terms[t & 1][mm][0][tgt] = ax;
+      terms[t & 1][mm][1][tgt] = ay;
+      terms[t & 1][mm][2][tgt] = az;
+    }
+    __syncthreads();
+  }
+  if (folds && i < n) k[3L * n + 3 * i + fc] = acc;
+  // dxdt[0:3n] = x[3n:6n] for this tile's 32 particles
+  const int e = threadIdx.x;
+  const long base = 3L * i0, lim = 3L * n;
+  if (e < 96 && base + e < lim) k[base + e] = y[lim + base + e];
 }
 
 // ------------------------------------------------------------------- fast --
@@ -381,6 +481,25 @@ hipError_t launch_nbody_rhs(const NBodyArgs& a, int variant, hipStream_t s) {
     case ODEHIP_NBODY_EXACT: {
       // the shared-reciprocal form needs softening^2 in [2^-200, 2^100]
       const bool shared = a.eps2 >= 0x1p-200 && a.eps2 <= 0x1p100;
+      // 64-target tiles while they fill the chip (N >= 12288: >= 192 of 256
+      // CUs busy), 32-target tiles below (twice the workgroups); both sum
+      // every target's partners in the same order.
+      if (a.n < 12288) {
+        const unsigned b32 = static_cast<unsigned>((a.n + 31) / 32);
+        const dim3 th(64 * kE32Waves);
+        if (mass) {
+          if (shared)
+            hipLaunchKernelGGL((k_nbody_exact32<true, true>), dim3(b32), th, 0, s, a);
+          else
+            hipLaunchKernelGGL((k_nbody_exact32<true, false>), dim3(b32), th, 0, s, a);
+        } else {
+          if (shared)
+            hipLaunchKernelGGL((k_nbody_exact32<false, true>), dim3(b32), th, 0, s, a);
+          else
+            hipLaunchKernelGGL((k_nbody_exact32<false, false>), dim3(b32), th, 0, s, a);
+        }
+        break;
+      }
       if (mass) {
         if (shared)
           hipLaunchKernelGGL((k_nbody_exact<true, true>), dim3(blocks), dim3(64 * kExactWaves), 0, s, a);

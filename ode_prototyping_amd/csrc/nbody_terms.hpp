@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+
 namespace odehip {
 
 // One pair term exactly as the reference writes it (NBodyODE.hpp:89-98):
@@ -28,7 +30,8 @@ __device__ inline void exact_term_ieee(double dx, double dy, double dz,
 
 // The same term, same bits, cheaper: valid when the three displacement
 // magnitudes lie in [2^-300, 2^100] and softening_sq in [2^-200, 2^100]
-// (then dist_sq in [2^-200, 2^202] and the denominator in [2^-300, 2^304]).
+// (then dist_sq in [2^-200, 2^202] and the denominator in [2^-300, 2^304]);
+// a zero displacement component gives a +-0 term (see coord_shared_ok).
 //  * sqrt: the compiler's correctly rounded sqrt is a rescale around a
 //    Goldschmidt refinement of v_rsq_f64 (y = rsq(s); g = s y; h = y/2;
 //    r = 1/2 - h g; g += g r; h += h r; twice g += (s - g^2) h); the rescale
@@ -74,12 +77,24 @@ __device__ inline void exact_term_shared_rcp(double dx, double dy, double dz,
   }
 }
 
-__device__ inline bool in_shared_rcp_range(double dx, double dy, double dz) {
-  const double lo = __builtin_fmin(__builtin_fmin(__builtin_fabs(dx), __builtin_fabs(dy)),
-                                   __builtin_fabs(dz));
-  const double hi = __builtin_fmax(__builtin_fmax(__builtin_fabs(dx), __builtin_fabs(dy)),
-                                   __builtin_fabs(dz));
-  return (lo >= 0x1p-300) & (hi <= 0x1p100);
+// Coordinates for which the shared-reciprocal term, summed into a target's
+// acceleration, gives the same bits as the IEEE expressions with no per-pair
+// test: the coordinate is 0 or its magnitude is in [2^-246, 2^98].  If every
+// partner and target coordinate satisfies this, |dx| <= 2^99 and a nonzero
+// displacement component has magnitude >= 2^-298 (a multiple of the smaller
+// operand's ulp), inside the shared form's range; a zero component gives a
+// +-0 term in both forms, and adding +-0 to a sum that starts at +0 (and so
+// is never -0) leaves it unchanged.  Pure integer tests on the bit pattern:
+// on wave-uniform partner coordinates they run on the scalar unit.
+__device__ inline bool coord_shared_ok(double c) {
+  constexpr uint64_t kLo = uint64_t{1023 - 246} << 52;  // bits of 2^-246
+  constexpr uint64_t kHi = uint64_t{1023 + 98} << 52;   // bits of 2^98
+  const uint64_t b = __builtin_bit_cast(uint64_t, c) & 0x7fffffffffffffffull;
+  return (b == 0) | (b - kLo <= kHi - kLo);
+}
+
+__device__ inline bool point_shared_ok(double x, double y, double z) {
+  return coord_shared_ok(x) & coord_shared_ok(y) & coord_shared_ok(z);
 }
 
 template <bool kMass, bool kGuard>

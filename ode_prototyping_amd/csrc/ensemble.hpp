@@ -27,7 +27,7 @@ struct EnsTile {
 // hold all systems packed (nv elements); ks holds S of them, nv apart.
 struct EnsBufs {
   const EnsSys* sys;
-  const int* group_sys;  // group -> system
+  const int2* groups;    // element workgroups: (system, reduction group id)
   const EnsTile* tiles;  // RHS work items in dispatch order
   int n_tiles;           // slots: RHS workgroups x waves per workgroup
   SolverState* st;       // one per system
@@ -48,7 +48,7 @@ struct EnsBufs {
   double atol, rtol;
   size_t nv;
   int n_sys;
-  int n_groups;
+  int n_groups;  // entries of `groups` (workgroups of an element launch)
   int tableau;
   int variant;
   int wpb;         // waves per RHS workgroup

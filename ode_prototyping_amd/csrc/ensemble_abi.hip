@@ -63,7 +63,7 @@ namespace {
 // Host-side layout of an ensemble.
 struct Layout {
   std::vector<EnsSys> sys;
-  std::vector<int> group_sys;
+  std::vector<int2> groups;  // (system, group id) of every reduction group
   std::vector<int> sizes;
   size_t nv = 0;
   int n_groups = 0;
@@ -93,9 +93,12 @@ int make_layout(odehip_context* ctx, int n_systems, const int* n_particles,
   if (groups > (1L << 30)) return fail(ctx, ODEHIP_ERR_INVALID, "ensemble too large");
   L->nv = static_cast<size_t>(eoff);
   L->n_groups = static_cast<int>(groups);
-  L->group_sys.resize(groups);
+  L->groups.resize(groups);
   for (int s = 0; s < n_systems; ++s)
-    for (int g = 0; g < L->sys[s].n_groups; ++g) L->group_sys[L->sys[s].gbase + g] = s;
+    for (int g = 0; g < L->sys[s].n_groups; ++g) {
+      const int id = L->sys[s].gbase + g;
+      L->groups[id] = int2{s, id};
+    }
   return ODEHIP_OK;
 }
 
@@ -104,10 +107,12 @@ int make_layout(odehip_context* ctx, int n_systems, const int* n_particles,
 // queues of about equal sum n^2 and workgroup b takes its tiles from queue
 // b mod 8, the XCD that the hardware dispatches workgroup b to, so all tiles
 // of one system share one XCD's L2.
-std::vector<EnsTile> make_tiles(const Layout& L, const EnsTileConfig& cfg) {
+std::vector<EnsTile> make_tiles(const Layout& L, const EnsTileConfig& cfg,
+                                const std::vector<char>* running = nullptr) {
   const int n_sys = static_cast<int>(L.sys.size());
-  std::vector<int> order(n_sys);
-  std::iota(order.begin(), order.end(), 0);
+  std::vector<int> order;
+  for (int s = 0; s < n_sys; ++s)
+    if (!running || (*running)[s]) order.push_back(s);
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return L.sys[a].n > L.sys[b].n; });
   auto tiles_of = [&](int s, std::vector<EnsTile>& out) {
@@ -150,8 +155,8 @@ size_t max_tiles(const Layout& L) {
 
 // Device arena: tables, per-system state, vectors.
 struct Plan {
-  size_t sys, group_sys, tiles, st, n_active, xb0, xb1, ks, ytmp, kfsal, sc0,
-      partA, partB, trace, total;
+  size_t sys, groups, groups_act, tiles, st, n_active, xb0, xb1, ks, ytmp, kfsal,
+      sc0, partA, partB, trace, total;
 };
 
 Plan plan(const Layout& L, size_t tiles_cap, int S, long trace_cap,
@@ -165,7 +170,8 @@ Plan plan(const Layout& L, size_t tiles_cap, int S, long trace_cap,
   };
   const size_t ns = L.sys.size();
   p.sys = take(ns * sizeof(EnsSys));
-  p.group_sys = take(L.group_sys.size() * sizeof(int));
+  p.groups = take(L.groups.size() * sizeof(int2));
+  p.groups_act = with_state ? take(L.groups.size() * sizeof(int2)) : p.groups;
   p.tiles = take(tiles_cap * sizeof(EnsTile));
   p.st = take(ns * sizeof(SolverState));
   p.n_active = take(sizeof(int));
@@ -177,8 +183,8 @@ Plan plan(const Layout& L, size_t tiles_cap, int S, long trace_cap,
     p.ytmp = take(v);
     p.kfsal = take(v);
     p.sc0 = take(v);
-    p.partA = take(L.group_sys.size() * sizeof(double));
-    p.partB = take(L.group_sys.size() * sizeof(double));
+    p.partA = take(L.groups.size() * sizeof(double));
+    p.partB = take(L.groups.size() * sizeof(double));
     p.trace = take(ns * static_cast<size_t>(trace_cap) * 3 * sizeof(double));
   }
   p.total = off;
@@ -224,8 +230,8 @@ int upload_tables(odehip_context* ctx, EnsembleWorkspace* w, const Plan& p,
   HIP_TRY(ctx, hipStreamSynchronize(s));  // no kernel may still read them
   HIP_TRY(ctx, hipMemcpy(w->dev + p.sys, L.sys.data(), L.sys.size() * sizeof(EnsSys),
                          hipMemcpyHostToDevice));
-  HIP_TRY(ctx, hipMemcpy(w->dev + p.group_sys, L.group_sys.data(),
-                         L.group_sys.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(w->dev + p.groups, L.groups.data(),
+                         L.groups.size() * sizeof(int2), hipMemcpyHostToDevice));
   HIP_TRY(ctx, hipMemcpy(w->dev + p.tiles, tiles.data(), tiles.size() * sizeof(EnsTile),
                          hipMemcpyHostToDevice));
   w->table_cfg = cfg;
@@ -240,7 +246,7 @@ EnsBufs make_bufs(EnsembleWorkspace* w, const Plan& p, const Layout& L,
   std::memset(&b, 0, sizeof b);  // padding too: the graph cache compares bytes
   char* d = w->dev;
   b.sys = reinterpret_cast<const EnsSys*>(d + p.sys);
-  b.group_sys = reinterpret_cast<const int*>(d + p.group_sys);
+  b.groups = reinterpret_cast<const int2*>(d + p.groups);
   b.tiles = reinterpret_cast<const EnsTile*>(d + p.tiles);
   b.n_active = reinterpret_cast<int*>(d + p.n_active);
   if (with_state) {
@@ -432,46 +438,96 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
                                 opt->max_attempts, opt->fsal ? 1 : 0, opt->controller, s));
   HIP_TRY(ctx, ens_enqueue_initial_step(b, s));
 
-  int K = opt->rounds_per_launch;
-  if (K <= 0) {
-    const double work = static_cast<double>(S) * L.work;
-    K = work > 1.0e10 ? 2 : (work > 1.0e9 ? 4 : (work > 1.0e8 ? 8 : 32));
-  }
-  if (!w->exec || w->graph_k != K || std::memcmp(&w->graph_bufs, &b, sizeof b) != 0) {
+  // Rounds run on lists of the running systems' workgroups: every system at
+  // first, rebuilt (and the graph re-captured) each time the number of
+  // running systems has halved, so the long tail of a few slow systems does
+  // not launch workgroups for all the finished ones.
+  auto rounds_per_replay = [&](double sum_n2) {
+    int k = opt->rounds_per_launch;
+    if (k <= 0) {
+      const double work = static_cast<double>(S) * sum_n2;
+      k = work > 1.0e10 ? 2 : (work > 1.0e9 ? 4 : (work > 1.0e8 ? 8 : 32));
+    }
+    return k;
+  };
+  auto capture = [&](const EnsBufs& rb, int k) -> int {
+    if (w->exec && w->graph_k == k && std::memcmp(&w->graph_bufs, &rb, sizeof rb) == 0)
+      return ODEHIP_OK;
     if (w->exec) (void)hipGraphExecDestroy(w->exec);
     if (w->graph) (void)hipGraphDestroy(w->graph);
     w->exec = nullptr;
     w->graph = nullptr;
     HIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipError_t ce = hipSuccess;
-    for (int r = 0; r < K && ce == hipSuccess; ++r) ce = ens_enqueue_round(b, s);
+    for (int r = 0; r < k && ce == hipSuccess; ++r) ce = ens_enqueue_round(rb, s);
     hipGraph_t g = nullptr;
     hipError_t ee = hipStreamEndCapture(s, &g);
     if (ce != hipSuccess) return hip_fail(ctx, ce, "capture round");
     if (ee != hipSuccess) return hip_fail(ctx, ee, "hipStreamEndCapture");
     w->graph = g;
     HIP_TRY(ctx, hipGraphInstantiate(&w->exec, g, nullptr, nullptr, 0));
-    w->graph_bufs = b;
-    w->graph_k = K;
-  }
+    w->graph_bufs = rb;
+    w->graph_k = k;
+    return ODEHIP_OK;
+  };
+  EnsBufs rb = b;
+  int K = rounds_per_replay(L.work);
+  if ((rc = capture(rb, K))) return rc;
 
   const long cap_rounds = opt->max_attempts > 0 ? opt->max_attempts : (1L << 40);
   long launched = 0;
   int slot = 0;
   bool have_prev = false;
+  int compact_below = n_systems / 2;
+  std::vector<SolverState> hst;
   for (;;) {
     HIP_TRY(ctx, hipGraphLaunch(w->exec, s));
     launched += K;
     HIP_TRY(ctx, hipMemcpyAsync(&w->h_active[slot], b.n_active, sizeof(int),
                                 hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipEventRecord(w->ev[slot], s));
+    if (launched >= cap_rounds + 2L * K) break;
     if (have_prev) {
       HIP_TRY(ctx, hipEventSynchronize(w->ev[slot ^ 1]));
-      if (w->h_active[slot ^ 1] == 0) break;
+      const int running = w->h_active[slot ^ 1];
+      if (running == 0) break;
+      if (running <= compact_below) {
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+        hst.resize(L.sys.size());
+        HIP_TRY(ctx, hipMemcpy(hst.data(), b.st, hst.size() * sizeof(SolverState),
+                               hipMemcpyDeviceToHost));
+        std::vector<char> live(L.sys.size());
+        std::vector<int2> act;
+        double sum_n2 = 0.0;
+        int n_live = 0;
+        for (size_t k = 0; k < L.sys.size(); ++k) {
+          live[k] = hst[k].done ? 0 : 1;
+          if (!live[k]) continue;
+          ++n_live;
+          sum_n2 += static_cast<double>(L.sys[k].n) * L.sys[k].n;
+          for (int g = 0; g < L.sys[k].n_groups; ++g)
+            act.push_back(L.groups[L.sys[k].gbase + g]);
+        }
+        if (n_live == 0) break;
+        const std::vector<EnsTile> tiles = make_tiles(L, kEnsTileConfigs[cfg], &live);
+        HIP_TRY(ctx, hipMemcpy(w->dev + p.groups_act, act.data(), act.size() * sizeof(int2),
+                               hipMemcpyHostToDevice));
+        HIP_TRY(ctx, hipMemcpy(w->dev + p.tiles, tiles.data(), tiles.size() * sizeof(EnsTile),
+                               hipMemcpyHostToDevice));
+        w->table_cfg = -2;  // the tile region now holds the running systems' tiles
+        rb.groups = reinterpret_cast<const int2*>(w->dev + p.groups_act);
+        rb.n_groups = static_cast<int>(act.size());
+        set_tile_cfg(&rb, cfg, static_cast<int>(tiles.size()));
+        K = rounds_per_replay(sum_n2);
+        if ((rc = capture(rb, K))) return rc;
+        compact_below = n_live / 2;
+        have_prev = false;
+        slot = 0;
+        continue;
+      }
     }
     have_prev = true;
     slot ^= 1;
-    if (launched >= cap_rounds + 2L * K) break;
   }
   HIP_TRY(ctx, ens_enqueue_copy_out(b, x_out, s));
   std::vector<SolverState> hs(L.sys.size());

@@ -340,14 +340,19 @@ __global__ __launch_bounds__(64 * WPB) void k_ens_rhs(EnsBufs b,
 
 struct GroupPos {
   int sys;
+  int g;  // reduction group id (index of its partial)
   bool valid;
   size_t e;  // packed element index
 };
 
-__device__ inline GroupPos group_pos(const EnsBufs& b, const EnsSys& sy, int s) {
-  const long l = static_cast<long>(blockIdx.x - sy.gbase) * kBlock + threadIdx.x;
+// Workgroup blockIdx.x of an element launch: its entry of the group list.
+__device__ inline GroupPos group_pos(const EnsBufs& b) {
+  const int2 ge = b.groups[blockIdx.x];
   GroupPos p;
-  p.sys = s;
+  p.sys = ge.x;
+  p.g = ge.y;
+  const EnsSys& sy = b.sys[ge.x];
+  const long l = static_cast<long>(ge.y - sy.gbase) * kBlock + threadIdx.x;
   p.valid = l < 6L * sy.n;
   p.e = static_cast<size_t>(sy.eoff + l);
   return p;
@@ -355,11 +360,9 @@ __device__ inline GroupPos group_pos(const EnsBufs& b, const EnsSys& sy, int s) 
 
 // rk_stage_kernel (RKEmbeddedParallel.hpp:91-101) for every active system.
 __global__ __launch_bounds__(kBlock) void k_ens_stage(EnsBufs b, int stage) {
-  const int s = b.group_sys[blockIdx.x];
-  const SolverState* st = b.st + s;
+  const GroupPos p = group_pos(b);
+  const SolverState* st = b.st + p.sys;
   if (st->done) return;
-  const EnsSys sy = b.sys[s];
-  const GroupPos p = group_pos(b, sy, s);
   if (!p.valid) return;
   const double* x0 = st->cur ? b.xb1 : b.xb0;
   b.ytmp[p.e] = stage_value(e_tab[b.tableau], stage, st->dt, b.ks, x0, p.e, b.nv);
@@ -369,11 +372,9 @@ __global__ __launch_bounds__(kBlock) void k_ens_stage(EnsBufs b, int stage) {
 // 103-137): new state, embedded error, error target, the scaled squared
 // error reduced into this group's partial, and the FSAL copy.
 __global__ __launch_bounds__(kBlock) void k_ens_final(EnsBufs b) {
-  const int s = b.group_sys[blockIdx.x];
-  const SolverState* st = b.st + s;
+  const GroupPos p = group_pos(b);
+  const SolverState* st = b.st + p.sys;
   if (st->done) return;  // uniform over the workgroup
-  const EnsSys sy = b.sys[s];
-  const GroupPos p = group_pos(b, sy, s);
   const Tableau& T = e_tab[b.tableau];
   const int cur = st->cur;
   const double* x0 = cur ? b.xb1 : b.xb0;
@@ -389,7 +390,7 @@ __global__ __launch_bounds__(kBlock) void k_ens_final(EnsBufs b) {
     if (st->fsal && T.fsal) b.kfsal[p.e] = b.ks[(T.S - 1) * b.nv + p.e];
   }
   const double g = group_reduce(term);
-  if (threadIdx.x == 0) b.partA[blockIdx.x] = g;
+  if (threadIdx.x == 0) b.partA[p.g] = g;
 }
 
 // Accept/reject and step size, one thread per system.
@@ -459,10 +460,8 @@ __global__ void k_ens_init_target(EnsBufs b) {
 // RKEmbeddedParallel.hpp:132-149, 165-168).
 __global__ __launch_bounds__(kBlock) void k_ens_norm_partials(
     EnsBufs b, const double* v, const double* sub, double* partials) {
-  const int s = b.group_sys[blockIdx.x];
-  if (b.st[s].done) return;
-  const EnsSys sy = b.sys[s];
-  const GroupPos p = group_pos(b, sy, s);
+  const GroupPos p = group_pos(b);
+  if (b.st[p.sys].done) return;
   double term = 0.0;
   if (p.valid) {
     const double d = sub ? (v[p.e] - sub[p.e]) : v[p.e];
@@ -470,7 +469,7 @@ __global__ __launch_bounds__(kBlock) void k_ens_norm_partials(
     term = q * q;
   }
   const double g = group_reduce(term);
-  if (threadIdx.x == 0) partials[blockIdx.x] = g;
+  if (threadIdx.x == 0) partials[p.g] = g;
 }
 
 __global__ void k_ens_init_step_a(EnsBufs b) {
@@ -486,11 +485,9 @@ __global__ void k_ens_init_step_a(EnsBufs b) {
 
 // euler_step_kernel (RKEmbeddedParallel.hpp:159-163).
 __global__ __launch_bounds__(kBlock) void k_ens_euler(EnsBufs b) {
-  const int s = b.group_sys[blockIdx.x];
-  const SolverState* st = b.st + s;
+  const GroupPos p = group_pos(b);
+  const SolverState* st = b.st + p.sys;
   if (st->done) return;
-  const EnsSys sy = b.sys[s];
-  const GroupPos p = group_pos(b, sy, s);
   if (p.valid) b.ytmp[p.e] = b.xb0[p.e] + b.ks[p.e] * st->dt0;
 }
 
@@ -506,11 +503,9 @@ __global__ void k_ens_init_step_b(EnsBufs b) {
 
 __global__ __launch_bounds__(kBlock) void k_ens_copy_out(EnsBufs b,
                                                          double* out) {
-  const int s = b.group_sys[blockIdx.x];
-  const EnsSys sy = b.sys[s];
-  const GroupPos p = group_pos(b, sy, s);
+  const GroupPos p = group_pos(b);
   if (!p.valid) return;
-  out[p.e] = (b.st[s].cur ? b.xb1 : b.xb0)[p.e];
+  out[p.e] = (b.st[p.sys].cur ? b.xb1 : b.xb0)[p.e];
 }
 
 template <int kVariant, bool kMass, bool kSpecial>

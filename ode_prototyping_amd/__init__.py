@@ -5,7 +5,8 @@ Host-side Python mirror of the reference's interfaces
 counterpart of ``src/par_exec/CudaExecutor.cuh``), device state containers
 (:class:`HipState`, ``src/state/CudaState.cuh``), ODE functors
 (:class:`NBodyODE`, :class:`ExponentialODE`, :class:`VanDerPolODE`,
-``src/ode``), output policies (:class:`RawOutput`, ``src/output/RawOutput.hpp``)
+``src/ode``), output policies (:class:`RawOutput`, ``src/output/RawOutput.hpp``; DOP853
+:class:`DenseOutput`, ``NumericalRecipes/Output.hpp``)
 and the integrator (:func:`integrate`, ``RKEmbeddedParallel<BT>::integrate``,
 ``src/integrator/RKEmbeddedParallel/RKEmbeddedParallel.hpp:13-81``), plus
 the leapfrog integrator (:func:`leapfrog`, ``LeapfrogParallel.hpp``), batched
@@ -15,14 +16,14 @@ ensembles (:mod:`.ensemble`), the learned step-size controller
 (include/odehip.h).
 """
 from ._lib import (CONTROLLERS, NBODY_VARIANTS, REDUCE_GROUP, STAGES, STATUS, TABLEAUS, lib)
-from .core import (ExponentialODE, HipExecutor, HipState, MostRecentOutput, NBodyODE,
-                   RawOutput, SolveResult, VanDerPolODE, integrate, integrate_host_driven,
-                   leapfrog)
+from .core import (DenseOutput, ExponentialODE, HipExecutor, HipState, MostRecentOutput,
+                   NBodyODE, RawOutput, SolveResult, VanDerPolODE, integrate,
+                   integrate_host_driven, leapfrog)
 from . import nbody_io, scenarios
 
 __all__ = [
     "HipExecutor", "HipState", "NBodyODE", "ExponentialODE", "VanDerPolODE",
-    "RawOutput", "MostRecentOutput", "SolveResult", "integrate",
+    "RawOutput", "MostRecentOutput", "DenseOutput", "SolveResult", "integrate",
     "integrate_host_driven", "leapfrog", "scenarios", "nbody_io", "TABLEAUS", "STAGES", "STATUS",
     "NBODY_VARIANTS", "REDUCE_GROUP", "lib",
 ]

@@ -37,7 +37,9 @@ class SolveOptions(C.Structure):
                 ("times", C.c_void_p), ("times_cap", C.c_long),
                 ("trace", C.c_void_p), ("trace_cap", C.c_long),
                 ("controller", C.c_int), ("mlp", C.c_void_p),
-                ("states", C.c_void_p), ("states_cap", C.c_long)]
+                ("states", C.c_void_p), ("states_cap", C.c_long),
+                ("dense_times", C.c_void_p), ("n_dense", C.c_long),
+                ("dense_states", C.c_void_p)]
 
 
 class SolveStats(C.Structure):
@@ -45,7 +47,9 @@ class SolveStats(C.Structure):
                 ("rhs_evals", C.c_long), ("n_times", C.c_long), ("n_trace", C.c_long),
                 ("status", C.c_int), ("t", C.c_double), ("dt", C.c_double),
                 ("dt_initial", C.c_double), ("last_err", C.c_double),
-                ("learned_steps", C.c_long), ("n_states", C.c_long)]
+                ("learned_steps", C.c_long), ("n_states", C.c_long),
+                ("n_dense", C.c_long), ("dense_steps", C.c_long),
+                ("dense_rhs_evals", C.c_long)]
 
 
 class EnsembleOptions(C.Structure):

@@ -268,6 +268,52 @@ class MostRecentOutput:
         self.state = x.copy_to_span() if isinstance(x, HipState) else np.array(x)
 
 
+class DenseOutput:
+    """DOP853 dense output at equally spaced points (the reference's
+    DenseOutput policy, NumericalRecipes/Output.hpp:63-147): the initial
+    state, then the state at t0 + k (tf - t0) / n_intervals for every such
+    point strictly before the end of the integration, from the continuous
+    extension of the step that covers it (steps are not shortened).  The
+    points are accumulated by repeated addition, as the reference does, so
+    they are bit-identical to its x values."""
+
+    default_n_intervals = 20
+
+    def __init__(self, n_intervals=default_n_intervals):
+        self.set_n_intervals(n_intervals)
+        self._x = []
+        self._y = []
+
+    def set_n_intervals(self, n_intervals):
+        if n_intervals < 1:
+            raise ValueError("n_intervals must be at least 1.")
+        self.n_intervals = int(n_intervals)
+
+    def output_times(self, t0, tf):
+        """The candidate points after t0 (those before tf)."""
+        width = (tf - t0) / self.n_intervals
+        out = []
+        t = t0 + width
+        while t < tf and len(out) <= self.n_intervals:
+            out.append(t)
+            t += width
+        return np.array(out, dtype=np.float64)
+
+    def save_state(self, t, x):
+        self._x.append(float(t))
+        self._y.append(x.copy_to_span() if isinstance(x, HipState) else np.array(x))
+
+    def n_steps(self):
+        return len(self._x)
+
+    def x_values(self):
+        return self._x
+
+    def y_values(self):
+        """Per component, the saved values (y_values()[i][k])."""
+        return np.array(self._y).T if self._y else np.zeros((0, 0))
+
+
 class SolveResult:
     """Outcome of a device-resident solve."""
 
@@ -284,6 +330,11 @@ class SolveResult:
         self.x_final = x_final
         self.times = times
         self.trace = trace
+        self.n_dense = stats.n_dense
+        self.dense_steps = stats.dense_steps
+        self.dense_rhs_evals = stats.dense_rhs_evals
+        self.dense_times = None
+        self.dense_states = None
 
     def __repr__(self):
         return (f"SolveResult(status={self.status}, accepted={self.accepted}, "
@@ -302,7 +353,7 @@ def _as_state(exe, v, n, name):
 def integrate(tableau, x0, t0, tf, atol, rtol, ode, output=None, exe=None, *,
               max_accepted=0, max_attempts=0, fsal=False, attempts_per_launch=0,
               times_cap=None, trace_cap=0, x_out=None, fetch=True, controller="reference",
-              mlp=None, states_cap=None):
+              mlp=None, states_cap=None, dense_times=None):
     """RKEmbeddedParallel<BT>::integrate with the attempt loop on the device.
 
     ``x0``: HipState or host array (n_var); ``atol``/``rtol``: scalars, host
@@ -315,6 +366,13 @@ def integrate(tableau, x0, t0, tf, atol, rtol, ode, output=None, exe=None, *,
     unless ``fetch`` is False -- it stays in ``x_out`` on the device --,
     accepted times in ``times``, per-attempt ``trace`` rows (err, dt, accepted)
     if ``trace_cap``).
+
+    Dense output (DOP853): ``dense_times`` (increasing, after t0) gives
+    ``res.dense_states`` rows at ``res.dense_times`` -- those reached before
+    the end --, from each covering step's continuous extension
+    (StepperDopr853::prepare_dense / dense_out).  An ``output`` that is a
+    :class:`DenseOutput` receives the initial state and its equally spaced
+    points instead of the accepted steps.
     """
     if tableau not in TABLEAUS:
         raise ValueError(f"unknown tableau {tableau}")
@@ -334,11 +392,24 @@ def integrate(tableau, x0, t0, tf, atol, rtol, ode, output=None, exe=None, *,
         times_cap = (max_accepted + 1) if max_accepted > 0 else 1 << 16
     times = np.zeros(max(int(times_cap), 1))
     trace = np.zeros((max(int(trace_cap), 1), 3))
+    dense_policy = isinstance(output, DenseOutput)
+    if dense_policy:
+        if dense_times is not None:
+            raise ValueError("pass either dense_times or a DenseOutput, not both")
+        dense_times = output.output_times(float(t0), float(tf))
+        x0_saved = x0_host.copy() if x0_host is not None else work.copy_to_span()
     if states_cap is None:
-        states_cap = times_cap if output is not None else 0
+        states_cap = times_cap if (output is not None and not dense_policy) else 0
         if states_cap * n * 8 > (8 << 30):
             raise ValueError("the saved-state list would exceed 8 GiB: pass states_cap")
     states = np.zeros((max(int(states_cap), 1), n)) if states_cap > 0 else None
+    dt_arr = None
+    dense_buf = None
+    if dense_times is not None and len(dense_times) > 0:
+        dt_arr = np.ascontiguousarray(dense_times, dtype=np.float64)
+        if dt_arr.size * n * 8 > (8 << 30):
+            raise ValueError("the dense output would exceed 8 GiB")
+        dense_buf = np.zeros((dt_arr.size, n))
     ode_s = ode.ode_struct(exe, n)
     opt = _lib.SolveOptions(float(t0), float(tf), at.ptr.value, rt.ptr.value,
                             int(max_accepted), int(max_attempts), 1 if fsal else 0,
@@ -347,14 +418,24 @@ def integrate(tableau, x0, t0, tf, atol, rtol, ode, output=None, exe=None, *,
                             trace.ctypes.data if trace_cap > 0 else None, int(trace_cap),
                             CONTROLLERS[controller], mlp.handle if mlp is not None else None,
                             states.ctypes.data if states is not None else None,
-                            int(states_cap))
+                            int(states_cap),
+                            dt_arr.ctypes.data if dt_arr is not None else None,
+                            int(dt_arr.size) if dt_arr is not None else 0,
+                            dense_buf.ctypes.data if dense_buf is not None else None)
     stats = _lib.SolveStats()
     exe.check(lib().odehip_rk_solve(exe.handle, TABLEAUS[tableau], C.byref(ode_s),
                                     C.byref(opt), work.ptr, C.byref(stats)), "rk_solve")
     x_final = work.copy_to_span() if fetch else None
     res = SolveResult(stats, x_final, times[:stats.n_times].copy(),
                       trace[:stats.n_trace].copy() if trace_cap else None)
-    if output is not None:
+    if dt_arr is not None:
+        res.dense_times = dt_arr[:stats.n_dense].copy()
+        res.dense_states = dense_buf[:stats.n_dense]
+    if dense_policy:
+        output.save_state(float(t0), x0_saved)
+        for k in range(stats.n_dense):
+            output.save_state(dt_arr[k], dense_buf[k])
+    elif output is not None:
         for k in range(min(stats.n_states, stats.n_times)):
             output.save_state(times[k], states[k])
     return res

@@ -22,12 +22,16 @@ struct GraphKey {
   const void* ws = nullptr;
   int controller = -1;
   const void* mlp = nullptr;
+  // times, trace, states and dense-output capacities: they place the
+  // buffers inside the workspace
+  long caps[4] = {-1, -1, -1, -1};
   bool operator==(const GraphKey& o) const {
     return tableau == o.tableau && rhs_kind == o.rhs_kind &&
            variant == o.variant && k == o.k && fsal == o.fsal &&
            n_var == o.n_var && std::memcmp(&eps2, &o.eps2, sizeof eps2) == 0 &&
            mass == o.mass && atol == o.atol && rtol == o.rtol && ws == o.ws &&
-           controller == o.controller && mlp == o.mlp;
+           controller == o.controller && mlp == o.mlp &&
+           std::memcmp(caps, o.caps, sizeof caps) == 0;
   }
 };
 

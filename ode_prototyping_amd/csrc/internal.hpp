@@ -27,6 +27,13 @@ struct SolverState {
   long max_accepted, max_attempts;
   long times_cap, trace_cap;
   long learned_steps;  // steps sized by the learned controller
+  double dt_acc;       // length of the last accepted step
+  long dense_next;     // first dense-output time not yet written
+  long dense_lo, dense_hi;  // rows the current dense pass writes
+  long dense_seen;     // accepted count the dense pass last examined
+  long dense_steps, dense_rhs_evals;
+  int dense_go;        // the dense pass of this attempt runs
+  int pad2;
   int cur;       // which ping-pong buffer holds x0
   int done;      // nonzero: every kernel returns immediately
   int status;    // ODEHIP_STATUS_*
@@ -51,7 +58,8 @@ struct NBodyArgs {
   const double* mass;   // nullable
   const double* kfsal;  // FSAL source for stage-0 reuse (solve only)
   int n;                // particles
-  int stage0;           // 1: this launch computes k1 of an attempt
+  int stage0;           // 1: this launch computes k1 of an attempt;
+                        // 2: a dense-output evaluation (runs iff st->dense_go)
   double eps2;
 };
 
@@ -123,6 +131,9 @@ struct SolveBufs {
   const double* mass;
   int controller;
   const struct MlpInfer* mlp;  // learned controller's network, or null
+  double* dense_t;        // device, n_dense output times (DOP853 only)
+  double* dense_out;      // device, n_dense x n_var
+  long n_dense;
 };
 
 hipError_t enqueue_solve_init(const SolveBufs& b, const double* x_in,

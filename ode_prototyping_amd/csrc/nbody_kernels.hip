@@ -46,9 +46,11 @@ __device__ inline const double* select_input(const NBodyArgs& a) {
 }
 
 // Returns true if the launch has nothing to compute (solve finished, or k1
-// of this attempt is reused).  On FSAL reuse the launch copies k1 instead.
+// of this attempt is reused, or a dense-output evaluation whose pass does not
+// run).  On FSAL reuse the launch copies k1 instead.
 __device__ inline bool prologue_skip(const NBodyArgs& a) {
   if (!a.st) return false;
+  if (a.stage0 == 2) return a.st->dense_go == 0;
   if (a.st->done) return true;
   if (a.stage0 && a.st->k1_valid) {
     if (a.st->k1_valid == 1) {

@@ -11,6 +11,7 @@
 // only polls the done flag of the previous replay (one replay in flight), so
 // the device never waits on the host between attempts.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <string>
@@ -76,7 +77,7 @@ int check_ode(odehip_context* ctx, const odehip_ode* ode) {
 
 // Lays out the solve workspace and returns its length in doubles.
 size_t layout(SolveBufs* b, double* base, SolverState* st, size_t n, int S,
-              long times_cap, long trace_cap, long states_cap) {
+              long times_cap, long trace_cap, long states_cap, long n_dense) {
   const size_t ng = static_cast<size_t>(n_groups_for(n));
   size_t off = 0;
   auto take = [&](size_t len) {
@@ -97,6 +98,9 @@ size_t layout(SolveBufs* b, double* base, SolverState* st, size_t n, int S,
   b->times = times_cap > 0 ? take(static_cast<size_t>(times_cap)) : nullptr;
   b->trace = trace_cap > 0 ? take(3 * static_cast<size_t>(trace_cap)) : nullptr;
   b->states = states_cap > 0 ? take(static_cast<size_t>(states_cap) * n) : nullptr;
+  b->dense_t = n_dense > 0 ? take(static_cast<size_t>(n_dense)) : nullptr;
+  b->dense_out = n_dense > 0 ? take(static_cast<size_t>(n_dense) * n) : nullptr;
+  b->n_dense = n_dense;
   b->times_cap = times_cap;
   b->trace_cap = trace_cap;
   b->states_cap = states_cap;
@@ -105,13 +109,13 @@ size_t layout(SolveBufs* b, double* base, SolverState* st, size_t n, int S,
 
 int prepare_bufs(odehip_context* ctx, int tableau, const odehip_ode* ode,
                  const double* atol, const double* rtol, long times_cap,
-                 long trace_cap, long states_cap, SolveBufs* b) {
+                 long trace_cap, long states_cap, long n_dense, SolveBufs* b) {
   const size_t n = ode->n_var;
   const int S = kTableaus[tableau].S;
-  size_t need = layout(b, nullptr, ctx->st, n, S, times_cap, trace_cap, states_cap);
+  size_t need = layout(b, nullptr, ctx->st, n, S, times_cap, trace_cap, states_cap, n_dense);
   int rc = ensure_ws(ctx, need);
   if (rc) return rc;
-  layout(b, ctx->ws, ctx->st, n, S, times_cap, trace_cap, states_cap);
+  layout(b, ctx->ws, ctx->st, n, S, times_cap, trace_cap, states_cap, n_dense);
   b->atol = atol;
   b->rtol = rtol;
   b->n_var = n;
@@ -419,7 +423,7 @@ int odehip_rk_initial_step(odehip_context* ctx, int tableau, int rhs_kind,
   int rc = check_ode(ctx, &ode);
   if (rc) return rc;
   SolveBufs b{};
-  rc = prepare_bufs(ctx, tableau, &ode, atol, rtol, 0, 0, 0, &b);
+  rc = prepare_bufs(ctx, tableau, &ode, atol, rtol, 0, 0, 0, 0, &b);
   if (rc) return rc;
   hipStream_t s = begin_work(ctx);
   HIP_TRY(ctx, enqueue_solve_init(b, x0, 0.0, 1.0, 0, 0, 0, s));
@@ -454,9 +458,25 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
   const long times_cap = opt->times ? opt->times_cap : 0;
   const long trace_cap = opt->trace ? opt->trace_cap : 0;
   const long states_cap = opt->states ? opt->states_cap : 0;
+  const long n_dense = opt->n_dense;
+  if (n_dense < 0) return fail(ctx, ODEHIP_ERR_INVALID, "negative n_dense");
+  if (n_dense > 0) {
+    if (tableau != ODEHIP_TABLEAU_DOP853)
+      return fail(ctx, ODEHIP_ERR_INVALID, "dense output needs DOP853");
+    if (!opt->dense_times || !opt->dense_states)
+      return fail(ctx, ODEHIP_ERR_INVALID, "dense_times / dense_states null");
+    double prev = opt->t0;
+    for (long r = 0; r < n_dense; ++r) {
+      const double tr = opt->dense_times[r];
+      if (!std::isfinite(tr) || !(tr > prev))
+        return fail(ctx, ODEHIP_ERR_INVALID,
+                    "dense_times must be finite and strictly increasing after t0");
+      prev = tr;
+    }
+  }
   SolveBufs b{};
   rc = prepare_bufs(ctx, tableau, ode, opt->atol, opt->rtol, times_cap,
-                    trace_cap, states_cap, &b);
+                    trace_cap, states_cap, n_dense, &b);
   if (rc) return rc;
   b.controller = opt->controller;
   b.mlp = opt->controller == ODEHIP_CONTROLLER_LEARNED ? mlp_infer_ptr(opt->mlp) : nullptr;
@@ -473,6 +493,9 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
   }
 
   hipStream_t s = begin_work(ctx);
+  if (n_dense > 0)
+    HIP_TRY(ctx, hipMemcpyAsync(b.dense_t, opt->dense_times, n_dense * sizeof(double),
+                                hipMemcpyHostToDevice, s));
   HIP_TRY(ctx, enqueue_solve_init(b, x, opt->t0, opt->tf, opt->max_accepted,
                                   opt->max_attempts, opt->fsal ? 1 : 0, s));
   HIP_TRY(ctx, enqueue_initial_step(b, s));
@@ -495,6 +518,10 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
              (b.states ? 4 : 0);
     key.controller = b.controller;
     key.mlp = b.mlp;
+    key.caps[0] = times_cap;
+    key.caps[1] = trace_cap;
+    key.caps[2] = states_cap;
+    key.caps[3] = n_dense;
     if (!ctx->exec || !(key == ctx->key)) {
       if (ctx->exec) {
         (void)hipGraphExecDestroy(ctx->exec);
@@ -550,6 +577,11 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
   const long n_times = std::min(hs.accepted + 1, times_cap);
   const long n_trace = std::min(hs.attempts, trace_cap);
   const long n_states = std::min(hs.accepted + 1, states_cap);
+  const long n_dense_out = n_dense > 0 ? hs.dense_next : 0;
+  if (n_dense_out > 0)
+    HIP_TRY(ctx, hipMemcpy(opt->dense_states, b.dense_out,
+                           static_cast<size_t>(n_dense_out) * n * sizeof(double),
+                           hipMemcpyDeviceToHost));
   if (opt->states && n_states > 0)
     HIP_TRY(ctx, hipMemcpy(opt->states, b.states, static_cast<size_t>(n_states) * n * sizeof(double),
                            hipMemcpyDeviceToHost));
@@ -572,6 +604,9 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
     stats->last_err = hs.err;
     stats->learned_steps = hs.learned_steps;
     stats->n_states = n_states;
+    stats->n_dense = n_dense_out;
+    stats->dense_steps = hs.dense_steps;
+    stats->dense_rhs_evals = hs.dense_rhs_evals;
   }
   if (hs.status == ODEHIP_STATUS_RUNNING)
     return fail(ctx, ODEHIP_ERR_INVALID, "solve did not finish within the attempt cap");

@@ -118,6 +118,15 @@ __device__ inline void ctrl_init(SolverState* st, const Tableau& T, double t0,
   st->times_cap = times_cap;
   st->trace_cap = trace_cap;
   st->learned_steps = 0;
+  st->dt_acc = 0.0;
+  st->dense_next = 0;
+  st->dense_lo = 0;
+  st->dense_hi = 0;
+  st->dense_seen = 0;
+  st->dense_steps = 0;
+  st->dense_rhs_evals = 0;
+  st->dense_go = 0;
+  st->pad2 = 0;
   st->cur = 0;
   st->fsal = fsal;
   st->tableau = tab;
@@ -171,6 +180,7 @@ __device__ inline bool ctrl_judge(SolverState* st, const Tableau& T,
   }
   if (accept) {
     st->t = st->t + dt;
+    st->dt_acc = dt;
     st->cur ^= 1;  // x0 = x (the ping-pong buffers swap roles)
     const long acc = st->accepted + 1;
     st->accepted = acc;

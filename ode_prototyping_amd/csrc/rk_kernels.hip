@@ -17,6 +17,8 @@ __constant__ Tableau c_tab[ODEHIP_NUM_TABLEAUS] = {
     kTableaus[0], kTableaus[1], kTableaus[2],
     kTableaus[3], kTableaus[4], kTableaus[5]};
 
+__constant__ Dop853Dense c_dense = kDop853Dense;
+
 constexpr int kBlock = 256;
 
 __device__ inline size_t gid() {
@@ -27,9 +29,9 @@ __device__ inline size_t gid() {
 __global__ void k_exp_rhs(const double* y, const double* xb0, const double* xb1,
                           const SolverState* st, double* k, const double* kfsal,
                           int stage0, size_t n_var) {
-  if (st && st->done) return;
+  if (st && (stage0 == 2 ? st->dense_go == 0 : st->done)) return;
   const size_t i = gid();
-  if (st && stage0 && st->k1_valid) {
+  if (st && stage0 == 1 && st->k1_valid) {
     if (st->k1_valid == 1 && i < n_var) k[i] = kfsal[i];
     return;
   }
@@ -40,9 +42,9 @@ __global__ void k_exp_rhs(const double* y, const double* xb0, const double* xb1,
 __global__ void k_vdp_rhs(const double* y, const double* xb0, const double* xb1,
                           const SolverState* st, double* k, const double* kfsal,
                           int stage0) {
-  if (st && st->done) return;
+  if (st && (stage0 == 2 ? st->dense_go == 0 : st->done)) return;
   const int i = threadIdx.x;
-  if (st && stage0 && st->k1_valid) {
+  if (st && stage0 == 1 && st->k1_valid) {
     if (st->k1_valid == 1 && i < 2) k[i] = kfsal[i];
     return;
   }
@@ -252,6 +254,96 @@ __global__ void k_save_state(const SolverState* st, const double* xb0,
   if (i < n_var) states[static_cast<size_t>(slot) * n_var + i] = x0[i];
 }
 
+// ------------------------------------------------------- dense output ----
+// DOP853's continuous extension after an accepted step (the reference's
+// StepperDopr853::prepare_dense / dense_out, StepperDopr853.hpp:200-289,
+// driven by DenseOutput::save, Output.hpp:99-114).  When the pass runs, the
+// controller has already swapped the ping-pong buffers: the step went from
+// the buffer st->cur does not name to the one it names, over st->dt_acc,
+// ending at st->t.
+
+// Stage vector j of the pass: 0..11 the accepted attempt's stages, 12 = f at
+// the step's end (k13), 13..15 the extra stages, kept in the slots of stages
+// 1..3, which the extension never reads.
+__device__ inline const double* dense_vec(const double* ks, const double* k13, int j,
+                                          size_t n) {
+  if (j < 12) return ks + j * n;
+  if (j == 12) return k13;
+  return ks + (j - 12) * n;
+}
+
+// DenseOutput::save's test (Output.hpp:109): the output times strictly
+// before the end of the step just accepted get rows [lo, hi) of this pass.
+// f at the step's end becomes the next attempt's k1 when k1 reuse is on.
+__global__ void k_dense_begin(SolverState* st, const double* times, long n_dense) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int go = 0;
+  if (st->just_accepted && st->accepted != st->dense_seen) {
+    st->dense_seen = st->accepted;
+    const long lo = st->dense_next;
+    long hi = lo;
+    const double t = st->t;
+    while (hi < n_dense && t > times[hi]) ++hi;
+    if (hi > lo) {
+      go = 1;
+      st->dense_lo = lo;
+      st->dense_hi = hi;
+      st->dense_next = hi;
+      st->dense_steps += 1;
+      st->dense_rhs_evals += 4;
+      if (st->fsal) st->k1_valid = 1;
+    }
+  }
+  st->dense_go = go;
+}
+
+// The input of extra stage 13 + e (StepperDopr853.hpp:239-262).
+__global__ void k_dense_stage(const SolverState* st, int e, double* ytmp,
+                              const double* ks, const double* k13,
+                              const double* xb0, const double* xb1, size_t n) {
+  if (!st->dense_go) return;
+  const size_t i = gid();
+  if (i >= n) return;
+  const Dop853Dense& D = c_dense;
+  const double* y = st->cur ? xb0 : xb1;
+  double acc = D.extra_w[e][0] * dense_vec(ks, k13, D.extra_idx[e][0], n)[i];
+  for (int m = 1; m < 8; ++m) acc = acc + D.extra_w[e][m] * dense_vec(ks, k13, D.extra_idx[e][m], n)[i];
+  ytmp[i] = y[i] + st->dt_acc * acc;
+}
+
+// The eight polynomial coefficients of element i (StepperDopr853.hpp:214-237,
+// 263-276) and the rows of this pass (dense_out, :278-289).
+__global__ __launch_bounds__(kBlock) void k_dense_out(
+    const SolverState* st, const double* ks, const double* k13,
+    const double* xb0, const double* xb1, const double* times, double* out,
+    size_t n) {
+  if (!st->dense_go) return;
+  const size_t i = gid();
+  if (i >= n) return;
+  const Dop853Dense& D = c_dense;
+  const double* y = st->cur ? xb0 : xb1;
+  const double* yout = st->cur ? xb1 : xb0;
+  const double h = st->dt_acc;
+  const double yi = y[i];
+  const double ydiff = yout[i] - yi;
+  const double bspl = h * ks[i] - ydiff;
+  const double rc3 = ydiff - h * k13[i] - bspl;
+  double rc[4];
+  for (int r = 0; r < 4; ++r) {
+    double acc = D.d_w[r][0] * dense_vec(ks, k13, D.d_idx[0], n)[i];
+    for (int m = 1; m < 12; ++m) acc = acc + D.d_w[r][m] * dense_vec(ks, k13, D.d_idx[m], n)[i];
+    rc[r] = h * acc;
+  }
+  const double t_end = st->t;
+  const long hi = st->dense_hi;
+  for (long row = st->dense_lo; row < hi; ++row) {
+    const double s = (times[row] - (t_end - h)) / h;
+    const double s1 = 1.0 - s;
+    out[static_cast<size_t>(row) * n + i] =
+        yi + s * (ydiff + s1 * (bspl + s * (rc3 + s1 * (rc[0] + s * (rc[1] + s1 * (rc[2] + s * rc[3]))))));
+  }
+}
+
 // Final state out: x_out = the x0 buffer.
 __global__ void k_copy_out(const SolverState* st, const double* xb0,
                            const double* xb1, double* out, size_t n_var) {
@@ -457,6 +549,18 @@ hipError_t enqueue_attempt(const SolveBufs& b, hipStream_t s) {
   if (b.states && b.states_cap > 0)
     hipLaunchKernelGGL(k_save_state, dim3(blocks_for(n)), dim3(kBlock), 0, s, b.st, b.xb0,
                        b.xb1, b.states, b.states_cap, n);
+  if (b.n_dense > 0) {
+    hipLaunchKernelGGL(k_dense_begin, dim3(1), dim3(64), 0, s, b.st, b.dense_t, b.n_dense);
+    if ((e = enqueue_rhs(b, nullptr, b.kfsal, 2, s)) != hipSuccess) return e;
+    for (int x = 0; x < 3; ++x) {
+      double* temp = (x & 1) ? b.ytmp1 : b.ytmp0;
+      hipLaunchKernelGGL(k_dense_stage, dim3(blocks_for(n)), dim3(kBlock), 0, s, b.st, x,
+                         temp, b.ks, b.kfsal, b.xb0, b.xb1, n);
+      if ((e = enqueue_rhs(b, temp, b.ks + (x + 1) * n, 2, s)) != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_dense_out, dim3(blocks_for(n)), dim3(kBlock), 0, s, b.st, b.ks,
+                       b.kfsal, b.xb0, b.xb1, b.dense_t, b.dense_out, n);
+  }
   return hipGetLastError();
 }
 

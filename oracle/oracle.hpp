@@ -247,6 +247,11 @@ struct Options {
   PowMode pow_mode = PowMode::Libm;
   bool store_states = true;  // keep every accepted state (RawOutput)
   bool store_trace = false;  // per-attempt (err, dt, accepted)
+  // DOP853 only: states at these times (increasing, after t0) from each
+  // accepted step's continuous extension, the reference's NR dense output
+  // (StepperDopr853.hpp:200-289, Output.hpp:99-114): a time is written by the
+  // first accepted step that ends strictly after it.
+  std::vector<double> dense_times;
 };
 
 struct Result {
@@ -256,6 +261,8 @@ struct Result {
   std::vector<double> trace_err, trace_dt;
   std::vector<int> trace_acc;
   long accepted = 0, rejected = 0, rhs_evals = 0;
+  std::vector<double> dense_states;  // n_dense x n_var
+  long n_dense = 0, dense_steps = 0, dense_rhs_evals = 0;
   int status = kReachedEnd;
   double t = 0.0, dt = 0.0, dt_initial = 0.0;
   size_t n_var = 0;
@@ -300,6 +307,60 @@ inline double estimate_initial_step(const Tableau& bt, const Executor& exe,
   return std::min(100.0 * dt0, dt1);
 }
 
+// prepare_dense (StepperDopr853.hpp:200-276) for one accepted step of length
+// h from y to yout: the eight polynomial coefficients rc (8 x n).  k[0..12]
+// are the step's stage vectors (k[1..4] are not read; k[12] = f(yout)); the
+// three extra stages are evaluated by f(stage, ytemp, out) into `extra`
+// (3 x n).  Every sum runs left to right from its first term, as the
+// reference writes them.
+template <class F>
+void dop853_prepare_dense(const Dop853Dense& D, size_t n, double h, const double* y,
+                          const double* yout, const double* const* k, F&& f,
+                          std::vector<double>& rc, std::vector<double>& extra) {
+  rc.assign(8 * n, 0.0);
+  extra.assign(3 * n, 0.0);
+  std::vector<double> ytemp(n);
+  const double* kv[16];
+  for (int j = 0; j < 13; ++j) kv[j] = k[j];
+  for (int e = 0; e < 3; ++e) {
+    for (size_t i = 0; i < n; ++i) {
+      double s = D.extra_w[e][0] * kv[D.extra_idx[e][0]][i];
+      for (int m = 1; m < 8; ++m) s = s + D.extra_w[e][m] * kv[D.extra_idx[e][m]][i];
+      ytemp[i] = y[i] + h * s;
+    }
+    f(e, ytemp.data(), extra.data() + e * n);
+    kv[13 + e] = extra.data() + e * n;
+  }
+  for (size_t i = 0; i < n; ++i) {
+    const double ydiff = yout[i] - y[i];
+    const double bspl = h * kv[0][i] - ydiff;
+    rc[i] = y[i];
+    rc[n + i] = ydiff;
+    rc[2 * n + i] = bspl;
+    rc[3 * n + i] = ydiff - h * kv[12][i] - bspl;
+    for (int r = 0; r < 4; ++r) {
+      double s = D.d_w[r][0] * kv[D.d_idx[0]][i];
+      for (int m = 1; m < 12; ++m) s = s + D.d_w[r][m] * kv[D.d_idx[m]][i];
+      rc[(4 + r) * n + i] = h * s;
+    }
+  }
+}
+
+// dense_out (StepperDopr853.hpp:278-289): component i at xout inside the step
+// that ends at x_end and has length h.
+inline double dop853_dense_eval(const std::vector<double>& rc, size_t n, size_t i,
+                                double xout, double x_end, double h) {
+  const double s = (xout - (x_end - h)) / h;
+  const double s1 = 1.0 - s;
+  const double* r = rc.data();
+  return r[i] + s * (r[n + i] +
+                     s1 * (r[2 * n + i] +
+                           s * (r[3 * n + i] +
+                                s1 * (r[4 * n + i] +
+                                      s * (r[5 * n + i] +
+                                           s1 * (r[6 * n + i] + s * r[7 * n + i]))))));
+}
+
 }  // namespace detail
 
 // RKEmbeddedParallel<BT>::integrate (RKEmbeddedParallel.hpp:13-81), with
@@ -334,6 +395,12 @@ inline Result integrate(const Tableau& bt, const double* x0_in, size_t n_var,
     if (opt.store_states) res.states.insert(res.states.end(), s.begin(), s.end());
   };
   save(t, x);
+  const bool dense = !opt.dense_times.empty();
+  if (dense && bt.name != "DOP853")
+    throw std::invalid_argument("dense output needs DOP853");
+  const Dop853Dense DD = dop853_dense();
+  size_t dense_next = 0;
+  std::vector<double> k13(dense ? n_var : 0), rc, extra;
   long attempts = 0;
   double err_prev = 1.0e-4;
   const double inv_q1 = 1.0 / (1.0 + q);
@@ -389,6 +456,25 @@ inline Result integrate(const Tableau& bt, const double* x0_in, size_t n_var,
     }
     if (accept) {
       t += dt;
+      if (dense && dense_next < opt.dense_times.size() && t > opt.dense_times[dense_next]) {
+        // the reference's prepare_dense / dense_out on this step
+        ode(x.data(), k13.data(), exe);
+        const double* kp[13];
+        for (int j = 0; j < 12; ++j) kp[j] = ks.data() + j * n_var;
+        kp[12] = k13.data();
+        detail::dop853_prepare_dense(
+            DD, n_var, dt, x0.data(), x.data(), kp,
+            [&](int, const double* yt, double* out) { ode(yt, out, exe); }, rc, extra);
+        res.dense_rhs_evals += 4;
+        ++res.dense_steps;
+        while (dense_next < opt.dense_times.size() && t > opt.dense_times[dense_next]) {
+          for (size_t i = 0; i < n_var; ++i)
+            res.dense_states.push_back(
+                detail::dop853_dense_eval(rc, n_var, i, opt.dense_times[dense_next], t, dt));
+          ++dense_next;
+          ++res.n_dense;
+        }
+      }
       x0 = x;
       save(t, x);
       ++res.accepted;
@@ -474,15 +560,20 @@ inline Result leapfrog(const double* x0_in, size_t n_var, double t0, double tf,
 struct NrResult {
   std::vector<double> xs;                 // saved x values
   std::vector<std::vector<double>> ys;    // saved y values
+  std::vector<double> y_final;            // ystart after a normal exit
   int status = 0;                         // 0 ok, 1 too many steps, 2 underflow
 };
 
 using NrDerivs = std::function<void(double, const std::vector<double>&,
                                     std::vector<double>&)>;
 
+// dense_intervals > 0 selects the DenseOutput policy (Output.hpp:63-147):
+// the first save records (x1, ystart); every accepted step then prepares its
+// continuous extension and records the points x1 + k (x2 - x1)/intervals
+// (accumulated by repeated addition) that lie strictly before its end.
 inline NrResult nr_dop853(std::vector<double> ystart, double x1, double x2,
                           double atol, double rtol, double h1, double hmin,
-                          const NrDerivs& derivs) {
+                          const NrDerivs& derivs, int dense_intervals = 0) {
   const Tableau T = dop853();
   const Dop853Extras E = dop853_extras();
   auto a = [&](int i, int j) { return T.A(i - 2, j - 1); };  // a_ij, 1-based
@@ -495,9 +586,38 @@ inline NrResult nr_dop853(std::vector<double> ystart, double x1, double x2,
   double hnext = 0.0, errold = 1.0e-4;
   bool reject = false;
   NrResult out;
+  const Dop853Dense DD = dop853_dense();
+  const double width = dense_intervals > 0 ? (x2 - x1) / dense_intervals : 0.0;
+  double next_x = x1 + width;
+  double hdid = 0.0;
+  bool first_save = true;
+  std::vector<double> rc, extra;
   auto save = [&] {
-    out.xs.push_back(x);
-    out.ys.push_back(yout);
+    if (dense_intervals <= 0 || first_save) {
+      first_save = false;
+      out.xs.push_back(x);
+      out.ys.push_back(yout);
+      return;
+    }
+    const double* kp[13] = {dydx.data(), nullptr, nullptr, nullptr, nullptr,
+                            k6.data(), k7.data(), k8.data(), k9.data(), k10.data(),
+                            k2.data(), k3.data(), dydxnew.data()};
+    detail::dop853_prepare_dense(
+        DD, static_cast<size_t>(n), hdid, y.data(), yout.data(), kp,
+        [&](int e, const double* yt, double* o) {
+          std::vector<double> yv(yt, yt + n), ov(n);
+          derivs(x + DD.c[e] * hdid, yv, ov);
+          std::copy(ov.begin(), ov.end(), o);
+        },
+        rc, extra);
+    while ((x - next_x) * (x2 - x1) > 0.0) {
+      std::vector<double> yd(n);
+      for (int i = 0; i < n; ++i)
+        yd[i] = detail::dop853_dense_eval(rc, static_cast<size_t>(n), i, next_x, x, hdid);
+      out.xs.push_back(next_x);
+      out.ys.push_back(yd);
+      next_x += width;
+    }
   };
   auto dy = [&](double h) {
     int i;
@@ -610,11 +730,15 @@ inline NrResult nr_dop853(std::vector<double> ystart, double x1, double x2,
     }
     derivs(x + hh, yout, dydxnew);
     x += hh;
+    hdid = hh;
     save();
     dydx = dydxnew;
     y = yout;
     h = hnext;
-    if ((x - x2) * (x2 - x1) >= 0.0) return out;
+    if ((x - x2) * (x2 - x1) >= 0.0) {
+      out.y_final = y;  // ystart = y (ODEIntegrator.hpp:80)
+      return out;
+    }
     if (std::abs(h) <= hmin) {
       out.status = 2;
       return out;

@@ -71,7 +71,7 @@ void* oracle_integrate(const char* tableau, int rhs_kind, const double* x0,
                        const double* masses, int reduce_kind, long reduce_param,
                        int pow_mode, long max_accepted, long max_attempts,
                        int store_states, int store_trace, int threads,
-                       int controller) {
+                       int controller, const double* dense_times, long n_dense) {
   try {
     Tableau bt = tableau_by_name(tableau);
     Executor e = make_exe(reduce_kind, reduce_param, threads);
@@ -82,6 +82,7 @@ void* oracle_integrate(const char* tableau, int rhs_kind, const double* x0,
     o.pow_mode = static_cast<PowMode>(pow_mode);
     o.store_states = store_states != 0;
     o.store_trace = store_trace != 0;
+    if (dense_times && n_dense > 0) o.dense_times.assign(dense_times, dense_times + n_dense);
     auto* r = new Result(integrate(bt, x0, n_var, t0, tf, atol, rtol,
                                    make_rhs(rhs_kind, n_var, softening, masses),
                                    e, o));
@@ -152,9 +153,8 @@ void oracle_result_free(void* h) { delete static_cast<Result*>(h); }
 // (dydx[1] = ((1 - y0^2) y1 - y0) / mu_eps).  Writes the count of saved
 // points, and x/y at indices 0, count/2 and count-1 into sel[0..8]
 // (x0,y00,y01, xm,ym0,ym1, xl,yl0,yl1).
-int oracle_nr_dop853(int problem, double y0a, double y0b, double x1, double x2,
-                     double atol, double rtol, double h1, double hmin,
-                     double mu_eps, long* count, double* sel) {
+namespace {
+NrDerivs nr_problem(int problem, double mu_eps) {
   NrDerivs d;
   if (problem == 0) {
     d = [](double, const std::vector<double>& y, std::vector<double>& dydx) {
@@ -168,7 +168,15 @@ int oracle_nr_dop853(int problem, double y0a, double y0b, double x1, double x2,
       dydx[1] = ((1.0 - y[0] * y[0]) * y[1] - y[0]) / mu_eps;
     };
   }
-  NrResult r = nr_dop853({y0a, y0b}, x1, x2, atol, rtol, h1, hmin, d);
+  return d;
+}
+}  // namespace
+
+int oracle_nr_dop853(int problem, double y0a, double y0b, double x1, double x2,
+                     double atol, double rtol, double h1, double hmin,
+                     double mu_eps, long* count, double* sel) {
+  NrResult r = nr_dop853({y0a, y0b}, x1, x2, atol, rtol, h1, hmin,
+                         nr_problem(problem, mu_eps));
   long c = static_cast<long>(r.xs.size());
   *count = c;
   long idx[3] = {0, c / 2, c - 1};
@@ -178,6 +186,82 @@ int oracle_nr_dop853(int problem, double y0a, double y0b, double x1, double x2,
     sel[3 * k + 2] = r.ys[idx[k]][1];
   }
   return r.status;
+}
+
+// The same with the DenseOutput policy (`intervals` equal intervals): writes
+// up to cap saved points (x, y0, y1) and the final state; returns the status
+// and the number of saved points in *count.
+int oracle_nr_dop853_dense(int problem, double y0a, double y0b, double x1, double x2,
+                           double atol, double rtol, double h1, double hmin,
+                           double mu_eps, int intervals, long cap, long* count,
+                           double* xs, double* y0s, double* y1s, double* y_final) {
+  NrResult r = nr_dop853({y0a, y0b}, x1, x2, atol, rtol, h1, hmin,
+                         nr_problem(problem, mu_eps), intervals);
+  const long c = static_cast<long>(r.xs.size());
+  *count = c;
+  for (long k = 0; k < c && k < cap; ++k) {
+    xs[k] = r.xs[k];
+    y0s[k] = r.ys[k][0];
+    y1s[k] = r.ys[k][1];
+  }
+  if (r.y_final.size() == 2) {
+    y_final[0] = r.y_final[0];
+    y_final[1] = r.y_final[1];
+  }
+  return r.status;
+}
+
+// One DOP853 step of length h from y (RK form: the twelve stages and the
+// 8th-order update of RKEmbeddedParallel.hpp:37-55) followed by its
+// continuous extension evaluated at the fractions s_k of the step
+// (xout = s_k h with the step ending at x_end = h): out is n_s x n_var.
+int oracle_dop853_step_dense(int rhs_kind, const double* y, size_t n_var, double h,
+                             double softening, const double* masses, const double* s_list,
+                             long n_s, double* yout, double* out) {
+  try {
+    Executor e = make_exe(0, 0, 1);
+    Rhs f = make_rhs(rhs_kind, n_var, softening, masses);
+    Tableau bt = dop853();
+    const size_t n = n_var;
+    std::vector<double> ks(12 * n), tmp(n), k13(n), rc, extra;
+    f(y, ks.data(), e);
+    for (int stage = 1; stage < 12; ++stage) {
+      for (size_t i = 0; i < n; ++i) {
+        double acc = 0.0;
+        for (int j = 0; j < stage; ++j) acc += bt.A(stage - 1, j) * ks[j * n + i];
+        tmp[i] = y[i] + acc * h;
+      }
+      f(tmp.data(), ks.data() + stage * n, e);
+    }
+    for (size_t i = 0; i < n; ++i) {
+      double xs = 0.0;
+      for (int j = 0; j < 12; ++j) xs += bt.b[j] * ks[j * n + i];
+      yout[i] = y[i] + xs * h;
+    }
+    f(yout, k13.data(), e);
+    const double* kp[13];
+    for (int j = 0; j < 12; ++j) kp[j] = ks.data() + j * n;
+    kp[12] = k13.data();
+    detail::dop853_prepare_dense(
+        dop853_dense(), n, h, y, yout, kp,
+        [&](int, const double* yt, double* o) { f(yt, o, e); }, rc, extra);
+    for (long k = 0; k < n_s; ++k)
+      for (size_t i = 0; i < n; ++i)
+        out[k * n + i] = detail::dop853_dense_eval(rc, n, i, s_list[k] * h, h, h);
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
+// Dense-output results of an oracle_integrate handle.
+long oracle_result_n_dense(void* h) { return static_cast<Result*>(h)->n_dense; }
+void oracle_result_dense(void* h, double* out, long* counts) {
+  auto* r = static_cast<Result*>(h);
+  std::memcpy(out, r->dense_states.data(), r->dense_states.size() * sizeof(double));
+  counts[0] = r->n_dense;
+  counts[1] = r->dense_steps;
+  counts[2] = r->dense_rhs_evals;
 }
 
 void oracle_energy(const double* x, size_t n, double softening,

@@ -38,8 +38,17 @@ def lib():
         L.oracle_rk_norm.restype = C.c_double
         L.oracle_integrate.argtypes = [C.c_char_p, C.c_int, _dp, C.c_size_t, C.c_double, C.c_double,
                                        _dp, _dp, C.c_double, _dp, C.c_int, C.c_long, C.c_int,
-                                       C.c_long, C.c_long, C.c_int, C.c_int, C.c_int, C.c_int]
+                                       C.c_long, C.c_long, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       _dp, C.c_long]
         L.oracle_integrate.restype = C.c_void_p
+        L.oracle_result_n_dense.argtypes = [C.c_void_p]
+        L.oracle_result_n_dense.restype = C.c_long
+        L.oracle_result_dense.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_long)]
+        L.oracle_nr_dop853_dense.argtypes = ([C.c_int] + [C.c_double] * 9 +
+                                             [C.c_int, C.c_long, C.POINTER(C.c_long)] +
+                                             [_dp] * 4)
+        L.oracle_dop853_step_dense.argtypes = [C.c_int, _dp, C.c_size_t, C.c_double,
+                                               C.c_double, _dp, _dp, C.c_long, _dp, _dp]
         L.oracle_result_count.argtypes = [C.c_void_p]
         L.oracle_result_count.restype = C.c_long
         L.oracle_result_times.argtypes = [C.c_void_p, _dp]
@@ -93,6 +102,11 @@ class Result:
         self.trace_acc = np.array(list(acc)[:nt], dtype=np.int32)
         self._states = {}
         self._handle = handle
+        nd = L.oracle_result_n_dense(handle)
+        self.dense_states = np.zeros((nd, n_var))
+        cnt = (C.c_long * 3)()
+        L.oracle_result_dense(handle, _ptr(self.dense_states), cnt)
+        self.n_dense, self.dense_steps, self.dense_rhs_evals = list(cnt)
 
     def state(self, k):
         if k < 0:
@@ -112,8 +126,9 @@ class Result:
 def integrate(tableau, rhs_kind, x0, t0, tf, atol, rtol, softening=0.0, masses=None,
               reduce=(REDUCE_SEQUENTIAL, 0), pow_mode=POW_LIBM, max_accepted=0,
               max_attempts=0, store_states=True, store_trace=False, threads=1,
-              controller=CONTROLLER_REFERENCE):
+              controller=CONTROLLER_REFERENCE, dense_times=None):
     x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    dt_ = None if dense_times is None else np.ascontiguousarray(dense_times, dtype=np.float64)
     n = x0.size
     atol = np.ascontiguousarray(np.broadcast_to(np.asarray(atol, dtype=np.float64), (n,)))
     rtol = np.ascontiguousarray(np.broadcast_to(np.asarray(rtol, dtype=np.float64), (n,)))
@@ -121,7 +136,7 @@ def integrate(tableau, rhs_kind, x0, t0, tf, atol, rtol, softening=0.0, masses=N
     h = lib().oracle_integrate(tableau.encode(), rhs_kind, _ptr(x0), n, t0, tf, _ptr(atol),
                                _ptr(rtol), softening, _ptr(m), reduce[0], reduce[1], pow_mode,
                                max_accepted, max_attempts, int(store_states), int(store_trace),
-                               threads, controller)
+                               threads, controller, _ptr(dt_), 0 if dt_ is None else dt_.size)
     if not h:
         raise RuntimeError("oracle_integrate failed")
     return Result(h, n)
@@ -159,6 +174,32 @@ def nr_dop853(problem, y0, x1, x2, atol, rtol, h1, hmin=0.0, mu_eps=1e-3):
     st = lib().oracle_nr_dop853(problem, y0[0], y0[1], x1, x2, atol, rtol, h1, hmin, mu_eps,
                                 C.byref(cnt), _ptr(sel))
     return st, cnt.value, sel
+
+
+def nr_dop853_dense(problem, y0, x1, x2, atol, rtol, h1, intervals, hmin=0.0, mu_eps=1e-3,
+                    cap=100000):
+    """NR DOP853 with the DenseOutput policy: (status, xs, ys (count x 2), y_final)."""
+    cnt = C.c_long()
+    xs, y0s, y1s, yf = np.zeros(cap), np.zeros(cap), np.zeros(cap), np.full(2, np.nan)
+    st = lib().oracle_nr_dop853_dense(problem, y0[0], y0[1], x1, x2, atol, rtol, h1, hmin,
+                                      mu_eps, intervals, cap, C.byref(cnt), _ptr(xs), _ptr(y0s),
+                                      _ptr(y1s), _ptr(yf))
+    c = min(cnt.value, cap)
+    return st, xs[:c], np.stack([y0s[:c], y1s[:c]], axis=1), yf
+
+
+def dop853_step_dense(rhs_kind, y, h, s_list, softening=0.0, masses=None):
+    """One RK-form DOP853 step of length h and its continuous extension at
+    the fractions s_list: (yout, dense (len(s_list) x n))."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    s_list = np.ascontiguousarray(s_list, dtype=np.float64)
+    m = None if masses is None else np.ascontiguousarray(masses, dtype=np.float64)
+    yout = np.zeros_like(y)
+    out = np.zeros((s_list.size, y.size))
+    if lib().oracle_dop853_step_dense(rhs_kind, _ptr(y), y.size, h, softening, _ptr(m),
+                                      _ptr(s_list), s_list.size, _ptr(yout), _ptr(out)) != 0:
+        raise RuntimeError("oracle_dop853_step_dense failed")
+    return yout, out
 
 
 def energy(x, n, softening, masses=None):

@@ -248,6 +248,43 @@ class RawOutput {
   std::vector<std::array<double, N>> states_{};
 };
 
+// NumericalRecipes/Output.hpp:63-147 (the DenseOutput policy) for the device
+// DOP853 solve: the initial state, then the state at x1 + k (x2 - x1) / n
+// (accumulated by repeated addition, as the reference does) for every such
+// point strictly before the end of the integration, from the continuous
+// extension of the step that covers it.
+template <std::size_t N>
+class DenseOutput {
+ public:
+  static constexpr int default_n_intervals = 20;
+
+  void set_n_intervals(int n_intervals) {
+    if (n_intervals < 1) throw std::invalid_argument{"n_intervals must be at least 1."};
+    n_intervals_ = n_intervals;
+  }
+  // The candidate points after x1 (those before x2).
+  std::vector<double> output_times(double x1, double x2) const {
+    std::vector<double> out;
+    const double width = (x2 - x1) / n_intervals_;
+    for (double x = x1 + width; x < x2 && out.size() <= static_cast<std::size_t>(n_intervals_);
+         x += width)
+      out.push_back(x);
+    return out;
+  }
+  void save(double x, const double* y) {
+    x_values_.push_back(x);
+    for (std::size_t i = 0; i < N; ++i) y_values_[i].push_back(y[i]);
+  }
+  auto n_steps() const { return x_values_.size(); }
+  const auto& x_values() const { return x_values_; }
+  const auto& y_values() const { return y_values_; }
+
+ private:
+  int n_intervals_{default_n_intervals};
+  std::vector<double> x_values_{};
+  std::array<std::vector<double>, N> y_values_{};
+};
+
 // ---------------------------------------------------------------- integrator
 struct DeviceSolveStats {
   long accepted = 0, rejected = 0, rhs_evals = 0;
@@ -343,6 +380,36 @@ struct RKEmbeddedParallel {
     odehip_solve_stats st{};
     exe.check(odehip_rk_solve(exe.context(), BT::id, &o, &opt, x.data(), &st), "rk_solve");
     if (times) times->resize(st.n_times);
+    return {st.accepted, st.rejected, st.rhs_evals, st.status, st.t};
+  }
+
+  // The device solve with DOP853 dense output into a DenseOutput policy
+  // (StepperDopr853::prepare_dense / dense_out, StepperDopr853.hpp:200-289).
+  template <typename ODE, std::size_t N>
+  static DeviceSolveStats integrate_on_device(HipState<double, N>& x, double t0, double tf,
+                                              const HipState<double, N>& atol,
+                                              const HipState<double, N>& rtol,
+                                              const ODE& ode, DenseOutput<N>& output,
+                                              HipExecutor& exe, bool fsal = true) {
+    static_assert(BT::id == ODEHIP_TABLEAU_DOP853, "dense output needs DOP853");
+    std::array<double, N> first{};
+    x.copy_to_span(std::span<double, N>{first});
+    const std::vector<double> when = output.output_times(t0, tf);
+    std::vector<double> rows(when.size() * N);
+    odehip_ode o = ode.describe();
+    odehip_solve_options opt{};
+    opt.t0 = t0;
+    opt.tf = tf;
+    opt.atol = atol.data();
+    opt.rtol = rtol.data();
+    opt.fsal = fsal ? 1 : 0;
+    opt.dense_times = when.data();
+    opt.n_dense = static_cast<long>(when.size());
+    opt.dense_states = rows.data();
+    odehip_solve_stats st{};
+    exe.check(odehip_rk_solve(exe.context(), BT::id, &o, &opt, x.data(), &st), "rk_solve");
+    output.save(t0, first.data());
+    for (long k = 0; k < st.n_dense; ++k) output.save(when[k], rows.data() + k * N);
     return {st.accepted, st.rejected, st.rhs_evals, st.status, st.t};
   }
 

@@ -259,6 +259,35 @@ static void device_resident_van_der_pol() {
   std::printf("ok device_resident_VanDerPol\n");
 }
 
+// The DenseOutput policy on the device DOP853 solve: the reference's 20
+// accumulated x values (StepperDopr853_test.cpp:181-196, all but the last,
+// which lies at the end and is never written) and, on x' = x, values that
+// track the closed form between steps.
+static void dense_output_policy() {
+  HipExecutor executor{};
+  constexpr std::size_t n = 3;
+  const std::array<double, n> init{1.0, 2.0, -0.5};
+  HipState<double, n> x{std::span<const double, n>{init}};
+  HipState<double, n> tol{};
+  fill(executor, tol, 1e-10);
+  DenseOutput<n> out{};
+  auto st = RKEmbeddedParallel<BTDOP853>::integrate_on_device(
+      x, 0.0, 2.0, tol, tol, ExponentialODE<double, n>{}, out, executor);
+  EXPECT_EQ(ODEHIP_STATUS_REACHED_END, st.status);
+  EXPECT_EQ(20u, out.n_steps());
+  const double golden_x[] = {0.0, 0.1, 0.2, 0.30000000000000004, 0.4, 0.5, 0.6, 0.7,
+                             0.7999999999999999, 0.8999999999999999, 0.9999999999999999,
+                             1.0999999999999999, 1.2, 1.3, 1.4000000000000001,
+                             1.5000000000000002, 1.6000000000000003, 1.7000000000000004,
+                             1.8000000000000005, 1.9000000000000006};
+  for (std::size_t k = 0; k < out.n_steps() && k < 20; ++k) {
+    EXPECT_EQ(golden_x[k], out.x_values()[k]);
+    EXPECT_NEAR(2.0 * std::exp(golden_x[k]), out.y_values()[1][k], 1e-8);
+  }
+  EXPECT_EQ(-0.5, out.y_values()[2][0]);
+  std::printf("ok dense_output_policy\n");
+}
+
 static void state_semantics() {
   // CudaStateTest.cu:18-92
   constexpr std::size_t N = 10;
@@ -381,6 +410,7 @@ int main() {
   device_resident_van_der_pol();
   leapfrog_two_body();
   ensemble_matches_single();
+  dense_output_policy();
   std::printf("%d checks, %d failures\n", g_checks, g_failures);
   return g_failures == 0 ? 0 : 1;
 }

@@ -85,7 +85,7 @@ __device__ inline bool all_positions_shared_ok(const double* __restrict__ y, int
 
 // ------------------------------------------------------------------ exact --
 constexpr int kExactWaves = 16;
-constexpr int kExactJW = 2;  // partners per wave per tile
+constexpr int kExactJW = 3;  // partners per wave per tile (48-partner tiles, 144 KB LDS)
 constexpr int kExactTJ = kExactWaves * kExactJW;
 
 template <bool kMass, bool kSharedRcp>
@@ -191,14 +191,15 @@ __global__ __launch_bounds__(64 * kExactWaves) void k_nbody_exact(NBodyArgs a) {
   copy_velocities(y, k, n, i0);
 }
 
-// The same exact sum with 32-target tiles: twice the workgroups and half
-// the LDS, for N too small to fill the chip with 64-target tiles.  Lane l evaluates target
-// l % 32 against partner slot l / 32 of its wave; the partner coordinates of
-// both slots arrive through scalar loads and each lane selects its own.  Two
-// waves fold a tile's terms in partner order: wave 0 the x (lanes 0-31) and
-// y (lanes 32-63) chains, wave 1 the z chains.
+// The same exact sum with 32-target tiles: twice the workgroups, for N too
+// small to fill the chip with 64-target tiles.  Lane l evaluates target l % 32
+// against the kE32JW partners of slot l / 32 of its wave; the partner
+// coordinates of both slots arrive through scalar loads and each lane selects
+// its own.  Two waves fold a tile's terms in partner order: wave 0 the x
+// (lanes 0-31) and y (lanes 32-63) chains, wave 1 the z chains.
 constexpr int kE32Waves = 16;
-constexpr int kE32TJ = 2 * kE32Waves;  // partners per tile
+constexpr int kE32JW = 2;                       // partners per slot per tile
+constexpr int kE32TJ = 2 * kE32JW * kE32Waves;  // partners per tile (96 KB LDS)
 
 template <bool kMass, bool kSharedRcp>
 __global__ __launch_bounds__(64 * kE32Waves) void k_nbody_exact32(NBodyArgs a) {
@@ -223,19 +224,23 @@ __global__ __launch_bounds__(64 * kE32Waves) void k_nbody_exact32(NBodyArgs a) {
   const bool folds = (wave == 0) || (wave == 1 && slot == 0);
   double acc = 0.0;
   const int n_tiles = (n + kE32TJ - 1) / kE32TJ;
-  double pa[4], pb[4];
+  // partner jj of slot s in this wave: tile index 2 kE32JW wave + s kE32JW + jj
+  double pa[kE32JW][4], pb[kE32JW][4];
   const bool shared_ok = kSharedRcp && all_positions_shared_ok(y, n);
   auto load_partners = [&](int t) {
-    const int ja = min(t * kE32TJ + 2 * wave, n - 1);
-    const int jb = min(t * kE32TJ + 2 * wave + 1, n - 1);
-    pa[0] = yu[3 * ja];
-    pa[1] = yu[3 * ja + 1];
-    pa[2] = yu[3 * ja + 2];
-    pa[3] = kMass ? massu[ja] : 1.0;
-    pb[0] = yu[3 * jb];
-    pb[1] = yu[3 * jb + 1];
-    pb[2] = yu[3 * jb + 2];
-    pb[3] = kMass ? massu[jb] : 1.0;
+#pragma unroll
+    for (int jj = 0; jj < kE32JW; ++jj) {
+      const int ja = min(t * kE32TJ + 2 * kE32JW * wave + jj, n - 1);
+      const int jb = min(t * kE32TJ + 2 * kE32JW * wave + kE32JW + jj, n - 1);
+      pa[jj][0] = yu[3 * ja];
+      pa[jj][1] = yu[3 * ja + 1];
+      pa[jj][2] = yu[3 * ja + 2];
+      pa[jj][3] = kMass ? massu[ja] : 1.0;
+      pb[jj][0] = yu[3 * jb];
+      pb[jj][1] = yu[3 * jb + 1];
+      pb[jj][2] = yu[3 * jb + 2];
+      pb[jj][3] = kMass ? massu[jb] : 1.0;
+    }
   };
   load_partners(0);
   for (int t = 0; t <= n_tiles; ++t) {
@@ -256,21 +261,27 @@ __global__ __launch_bounds__(64 * kE32Waves) void k_nbody_exact32(NBodyArgs a) {
       }
     }
     if (t < n_tiles) {
-      const double px = slot ? pb[0] : pa[0];
-      const double py = slot ? pb[1] : pa[1];
-      const double pz = slot ? pb[2] : pa[2];
-      const double pm = slot ? pb[3] : pa[3];
+      double dx[kE32JW], dy[kE32JW], dz[kE32JW], pm[kE32JW];
+#pragma unroll
+      for (int jj = 0; jj < kE32JW; ++jj) {
+        dx[jj] = (slot ? pb[jj][0] : pa[jj][0]) - xi;
+        dy[jj] = (slot ? pb[jj][1] : pa[jj][1]) - yi;
+        dz[jj] = (slot ? pb[jj][2] : pa[jj][2]) - zi;
+        pm[jj] = slot ? pb[jj][3] : pa[jj][3];
+      }
       if (t + 1 < n_tiles) load_partners(t + 1);
-      const double dx = px - xi, dy = py - yi, dz = pz - zi;
-      double ax, ay, az;
-      if (kSharedRcp && shared_ok)
-        exact_term_shared_rcp<kMass>(dx, dy, dz, pm, softening_sq, ax, ay, az);
-      else
-        exact_term_ieee<kMass>(dx, dy, dz, pm, softening_sq, ax, ay, az);
-      const int mm = 2 * wave + slot;
-      terms[t & 1][mm][0][tgt] = ax;
-      terms[t & 1][mm][1][tgt] = ay;
-      terms[t & 1][mm][2][tgt] = az;
+#pragma unroll
+      for (int jj = 0; jj < kE32JW; ++jj) {
+        double ax, ay, az;
+        if (kSharedRcp && shared_ok)
+          exact_term_shared_rcp<kMass>(dx[jj], dy[jj], dz[jj], pm[jj], softening_sq, ax, ay, az);
+        else
+          exact_term_ieee<kMass>(dx[jj], dy[jj], dz[jj], pm[jj], softening_sq, ax, ay, az);
+        const int mm = 2 * kE32JW * wave + slot * kE32JW + jj;
+        terms[t & 1][mm][0][tgt] = ax;
+        terms[t & 1][mm][1][tgt] = ay;
+        terms[t & 1][mm][2][tgt] = az;
+      }
     }
     __syncthreads();
   }
@@ -483,10 +494,11 @@ hipError_t launch_nbody_rhs(const NBodyArgs& a, int variant, hipStream_t s) {
     case ODEHIP_NBODY_EXACT: {
       // the shared-reciprocal form needs softening^2 in [2^-200, 2^100]
       const bool shared = a.eps2 >= 0x1p-200 && a.eps2 <= 0x1p100;
-      // 64-target tiles while they fill the chip (N >= 12288: >= 192 of 256
-      // CUs busy), 32-target tiles below (twice the workgroups); both sum
-      // every target's partners in the same order.
-      if (a.n < 12288) {
+      // Both tile shapes sum every target's partners in the same order.  The
+      // 32-target kernel (96 KB LDS: one workgroup per CU) is faster while
+      // its workgroups fit the 256 CUs in one round (N <= 8192); above, the
+      // 64-target kernel, which needs half as many rounds.
+      if (a.n <= 8192) {
         const unsigned b32 = static_cast<unsigned>((a.n + 31) / 32);
         const dim3 th(64 * kE32Waves);
         if (mass) {

@@ -39,6 +39,18 @@ def test_exact_rhs_bit_identical(exe, n, masses):
         f"n={n}: max abs diff {np.max(np.abs(got - want))}"
 
 
+@pytest.mark.parametrize("n", [4095, 8192, 8193, 12289, 16384])
+def test_exact_rhs_bit_identical_large(exe, n):
+    """Both tile shapes at the sizes where the dispatch switches (32-target
+    tiles up to 8192, 64-target tiles of 48 partners above) and with a
+    partial last tile (8193 = 170 * 48 + 33, 12289)."""
+    x, m = ic(n, seed=n, masses=True)
+    got = gpu_rhs(exe, x, 1e-3, m, "exact")
+    want = O.rhs(O.RHS_NBODY, x, 1e-3, m, threads=8)
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), \
+        f"n={n}: max abs diff {np.max(np.abs(got - want))}"
+
+
 def test_exact_rhs_unsoftened(exe):
     """eps = 0: the diagonal term must be skipped, not evaluated (0/0)."""
     x, _ = ic(300, seed=3, masses=False)
@@ -192,15 +204,15 @@ def test_energy_and_momentum(exe):
     assert np.all(np.abs(out - p) <= 1e-17)
 
 
+@pytest.mark.parametrize("n", [200, 9000])
 @pytest.mark.parametrize("eps", [1e-3, 1e-70, 0.0])
-def test_exact_rhs_extreme_inputs_bit_identical(exe, eps):
+def test_exact_rhs_extreme_inputs_bit_identical(exe, eps, n):
     """Inputs that leave the exact kernel's shared-reciprocal range: zero
     displacement components (coplanar particles), separations below 2^-300
     and above 2^100, and softenings outside [2^-100, 2^50].  Every term must
     still match the IEEE expression bit for bit (NaN where the reference
     also produces one)."""
     rng = np.random.default_rng(42)
-    n = 200
     pos = rng.uniform(-1, 1, (n, 3))
     pos[:40, 2] = 0.0                              # coplanar block: dz == 0
     pos[40:60] = pos[40] + rng.uniform(-1, 1, (20, 3)) * 1e-95   # tiny separations
@@ -209,7 +221,7 @@ def test_exact_rhs_extreme_inputs_bit_identical(exe, eps):
     x = np.concatenate([pos.ravel(), rng.normal(size=3 * n)])
     m = rng.uniform(0.5, 1.5, n) / n
     got = gpu_rhs(exe, x, eps, m, "exact")
-    want = O.rhs(O.RHS_NBODY, x, eps, m)
+    want = O.rhs(O.RHS_NBODY, x, eps, m, threads=8)
     finite = np.isfinite(want)
     assert np.array_equal(np.isfinite(got), finite)
     assert np.array_equal(got[finite].view(np.uint64), want[finite].view(np.uint64))

@@ -151,7 +151,7 @@ def ensemble_block(oh, exe):
         err = [np.max(np.abs(a - b)) / np.max(np.abs(b)) for a, b in zip(got, ref)]
         block[name]["final_state_rel_err_vs_plain"] = {"median": float(np.median(err)),
                                                        "max": float(np.max(err))}
-    for name in ("plain_exact", "chosen_mixed_learned"):
+    for name in ("plain_exact", "chosen_mixed_learned", "fast"):
         got = unpack(finals[name], sizes)
         drift = []
         for k in idx:

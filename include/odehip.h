@@ -222,15 +222,16 @@ typedef struct {
                             RawOutput.hpp:15-19), kept on the device during
                             the solve; may be NULL */
   long states_cap;
-  /* DOP853 dense output (StepperDopr853::prepare_dense / dense_out and the
-   * DenseOutput policy, NumericalRecipes/StepperDopr853.hpp:200-289,
-   * Output.hpp:99-114): the state at each of n_dense strictly increasing
-   * times in (t0, tf), from the continuous extension of the first accepted
-   * step that ends strictly after it -- steps are not shortened to hit them.
-   * dense_times: host, n_dense; dense_states: host, n_dense x n_var, rows
-   * written for the times reached (stats.n_dense).  Costs three extra RHS
-   * evaluations per step that covers an output time, plus one for f at the
-   * step's end, which the next attempt reuses as its k1 when fsal is set. */
+  /* Dense output, DOP853 or DOPRI5 (StepperDopr853::prepare_dense /
+   * dense_out, NumericalRecipes/StepperDopr853.hpp:200-289; StepperDopr5,
+   * StepperDopr5.hpp:181-221; and the DenseOutput policy, Output.hpp:99-114):
+   * the state at each of n_dense strictly increasing times after t0, from
+   * the continuous extension of the first accepted step that ends strictly
+   * after it -- steps are not shortened to hit them.  dense_times: host,
+   * n_dense; dense_states: host, n_dense x n_var, rows written for the times
+   * reached (stats.n_dense).  DOP853 costs three extra RHS evaluations per
+   * step that covers an output time, plus one for f at the step's end, which
+   * the next attempt reuses as its k1 when fsal is set; DOPRI5 costs none. */
   const double* dense_times;
   long n_dense;
   double* dense_states;
@@ -246,8 +247,9 @@ typedef struct {
   long n_states;         /* states written (accepted + 1, capped) */
   long n_dense;          /* dense-output rows written */
   long dense_steps;      /* accepted steps that produced dense output */
-  long dense_rhs_evals;  /* RHS evaluations of the dense output (4 per
-                            such step; not included in rhs_evals) */
+  long dense_rhs_evals;  /* RHS evaluations of the dense output (4 per such
+                            step for DOP853, 0 for DOPRI5; not included in
+                            rhs_evals) */
 } odehip_solve_stats;
 
 /* x: device, n_var; in: x0, out: the state at the final time. */

@@ -269,7 +269,7 @@ class MostRecentOutput:
 
 
 class DenseOutput:
-    """DOP853 dense output at equally spaced points (the reference's
+    """DOP853 / DOPRI5 dense output at equally spaced points (the reference's
     DenseOutput policy, NumericalRecipes/Output.hpp:63-147): the initial
     state, then the state at t0 + k (tf - t0) / n_intervals for every such
     point strictly before the end of the integration, from the continuous
@@ -367,10 +367,11 @@ def integrate(tableau, x0, t0, tf, atol, rtol, ode, output=None, exe=None, *,
     accepted times in ``times``, per-attempt ``trace`` rows (err, dt, accepted)
     if ``trace_cap``).
 
-    Dense output (DOP853): ``dense_times`` (increasing, after t0) gives
+    Dense output (DOP853, DOPRI5): ``dense_times`` (increasing, after t0) gives
     ``res.dense_states`` rows at ``res.dense_times`` -- those reached before
     the end --, from each covering step's continuous extension
-    (StepperDopr853::prepare_dense / dense_out).  An ``output`` that is a
+    (StepperDopr853 / StepperDopr5 prepare_dense, dense_out).  An ``output``
+    that is a
     :class:`DenseOutput` receives the initial state and its equally spaced
     points instead of the accepted steps.
     """

@@ -249,7 +249,7 @@ class RawOutput {
 };
 
 // NumericalRecipes/Output.hpp:63-147 (the DenseOutput policy) for the device
-// DOP853 solve: the initial state, then the state at x1 + k (x2 - x1) / n
+// DOP853 or DOPRI5 solve: the initial state, then the state at x1 + k (x2 - x1) / n
 // (accumulated by repeated addition, as the reference does) for every such
 // point strictly before the end of the integration, from the continuous
 // extension of the step that covers it.
@@ -383,15 +383,17 @@ struct RKEmbeddedParallel {
     return {st.accepted, st.rejected, st.rhs_evals, st.status, st.t};
   }
 
-  // The device solve with DOP853 dense output into a DenseOutput policy
-  // (StepperDopr853::prepare_dense / dense_out, StepperDopr853.hpp:200-289).
+  // The device solve with dense output into a DenseOutput policy
+  // (StepperDopr853 / StepperDopr5 ::prepare_dense, dense_out,
+  // StepperDopr853.hpp:200-289, StepperDopr5.hpp:181-221).
   template <typename ODE, std::size_t N>
   static DeviceSolveStats integrate_on_device(HipState<double, N>& x, double t0, double tf,
                                               const HipState<double, N>& atol,
                                               const HipState<double, N>& rtol,
                                               const ODE& ode, DenseOutput<N>& output,
                                               HipExecutor& exe, bool fsal = true) {
-    static_assert(BT::id == ODEHIP_TABLEAU_DOP853, "dense output needs DOP853");
+    static_assert(BT::id == ODEHIP_TABLEAU_DOP853 || BT::id == ODEHIP_TABLEAU_DOPRI5,
+                  "dense output needs DOP853 or DOPRI5");
     std::array<double, N> first{};
     x.copy_to_span(std::span<double, N>{first});
     const std::vector<double> when = output.output_times(t0, tf);

@@ -461,8 +461,8 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
   const long n_dense = opt->n_dense;
   if (n_dense < 0) return fail(ctx, ODEHIP_ERR_INVALID, "negative n_dense");
   if (n_dense > 0) {
-    if (tableau != ODEHIP_TABLEAU_DOP853)
-      return fail(ctx, ODEHIP_ERR_INVALID, "dense output needs DOP853");
+    if (tableau != ODEHIP_TABLEAU_DOP853 && tableau != ODEHIP_TABLEAU_DOPRI5)
+      return fail(ctx, ODEHIP_ERR_INVALID, "dense output needs DOP853 or DOPRI5");
     if (!opt->dense_times || !opt->dense_states)
       return fail(ctx, ODEHIP_ERR_INVALID, "dense_times / dense_states null");
     double prev = opt->t0;

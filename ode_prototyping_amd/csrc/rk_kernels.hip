@@ -18,6 +18,7 @@ __constant__ Tableau c_tab[ODEHIP_NUM_TABLEAUS] = {
     kTableaus[3], kTableaus[4], kTableaus[5]};
 
 __constant__ Dop853Dense c_dense = kDop853Dense;
+__constant__ Dopri5Dense c_dense5 = kDopri5Dense;
 
 constexpr int kBlock = 256;
 
@@ -274,8 +275,10 @@ __device__ inline const double* dense_vec(const double* ks, const double* k13, i
 
 // DenseOutput::save's test (Output.hpp:109): the output times strictly
 // before the end of the step just accepted get rows [lo, hi) of this pass.
-// f at the step's end becomes the next attempt's k1 when k1 reuse is on.
-__global__ void k_dense_begin(SolverState* st, const double* times, long n_dense) {
+// DOP853 (extra_rhs = 4): f at the step's end becomes the next attempt's k1
+// when k1 reuse is on.  DOPRI5 (extra_rhs = 0) needs no evaluation.
+__global__ void k_dense_begin(SolverState* st, const double* times, long n_dense,
+                              int extra_rhs) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   int go = 0;
   if (st->just_accepted && st->accepted != st->dense_seen) {
@@ -290,8 +293,8 @@ __global__ void k_dense_begin(SolverState* st, const double* times, long n_dense
       st->dense_hi = hi;
       st->dense_next = hi;
       st->dense_steps += 1;
-      st->dense_rhs_evals += 4;
-      if (st->fsal) st->k1_valid = 1;
+      st->dense_rhs_evals += extra_rhs;
+      if (extra_rhs && st->fsal) st->k1_valid = 1;
     }
   }
   st->dense_go = go;
@@ -341,6 +344,36 @@ __global__ __launch_bounds__(kBlock) void k_dense_out(
     const double s1 = 1.0 - s;
     out[static_cast<size_t>(row) * n + i] =
         yi + s * (ydiff + s1 * (bspl + s * (rc3 + s1 * (rc[0] + s * (rc[1] + s1 * (rc[2] + s * rc[3]))))));
+  }
+}
+
+// DOPRI5: the five coefficients of element i (StepperDopr5::prepare_dense,
+// StepperDopr5.hpp:195-207) from stages 1, 3..7 -- stage 7 is f at the new
+// state, its input being the new state bit for bit -- and the rows of this
+// pass (dense_out, :210-221).
+__global__ __launch_bounds__(kBlock) void k_dense_out5(
+    const SolverState* st, const double* ks, const double* xb0, const double* xb1,
+    const double* times, double* out, size_t n) {
+  if (!st->dense_go) return;
+  const size_t i = gid();
+  if (i >= n) return;
+  const Dopri5Dense& D = c_dense5;
+  const double* y = st->cur ? xb0 : xb1;
+  const double* yout = st->cur ? xb1 : xb0;
+  const double h = st->dt_acc;
+  const double yi = y[i];
+  const double ydiff = yout[i] - yi;
+  const double bspl = h * ks[i] - ydiff;
+  const double rc3 = ydiff - h * ks[6 * n + i] - bspl;
+  double acc = D.d[0] * ks[static_cast<size_t>(D.idx[0]) * n + i];
+  for (int m = 1; m < 6; ++m) acc = acc + D.d[m] * ks[static_cast<size_t>(D.idx[m]) * n + i];
+  const double rc4 = h * acc;
+  const double t_end = st->t;
+  const long hi = st->dense_hi;
+  for (long row = st->dense_lo; row < hi; ++row) {
+    const double s = (times[row] - (t_end - h)) / h;
+    const double s1 = 1.0 - s;
+    out[static_cast<size_t>(row) * n + i] = yi + s * (ydiff + s1 * (bspl + s * (rc3 + s1 * rc4)));
   }
 }
 
@@ -549,8 +582,12 @@ hipError_t enqueue_attempt(const SolveBufs& b, hipStream_t s) {
   if (b.states && b.states_cap > 0)
     hipLaunchKernelGGL(k_save_state, dim3(blocks_for(n)), dim3(kBlock), 0, s, b.st, b.xb0,
                        b.xb1, b.states, b.states_cap, n);
-  if (b.n_dense > 0) {
-    hipLaunchKernelGGL(k_dense_begin, dim3(1), dim3(64), 0, s, b.st, b.dense_t, b.n_dense);
+  if (b.n_dense > 0 && b.tableau == ODEHIP_TABLEAU_DOPRI5) {
+    hipLaunchKernelGGL(k_dense_begin, dim3(1), dim3(64), 0, s, b.st, b.dense_t, b.n_dense, 0);
+    hipLaunchKernelGGL(k_dense_out5, dim3(blocks_for(n)), dim3(kBlock), 0, s, b.st, b.ks,
+                       b.xb0, b.xb1, b.dense_t, b.dense_out, n);
+  } else if (b.n_dense > 0) {
+    hipLaunchKernelGGL(k_dense_begin, dim3(1), dim3(64), 0, s, b.st, b.dense_t, b.n_dense, 4);
     if ((e = enqueue_rhs(b, nullptr, b.kfsal, 2, s)) != hipSuccess) return e;
     for (int x = 0; x < 3; ++x) {
       double* temp = (x & 1) ? b.ytmp1 : b.ytmp0;

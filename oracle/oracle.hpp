@@ -24,6 +24,8 @@
 //   estimate_initial_step       RKEmbeddedParallel.hpp:151-200
 //   nr_dop853                   NumericalRecipes/ODEIntegrator.hpp:66-88,
 //                               NumericalRecipes/StepperDopr853.hpp:93-197,291-342
+//                               (dense output :200-289, Output.hpp:63-147)
+//   nr_dopr5                    NumericalRecipes/StepperDopr5.hpp:80-270
 //   energy                      scripts/nbody_physics/energy.py:118-192,224-241
 //                               (softened PE + KE), generalised to masses
 //
@@ -361,6 +363,35 @@ inline double dop853_dense_eval(const std::vector<double>& rc, size_t n, size_t 
                                            s1 * (r[6 * n + i] + s * r[7 * n + i]))))));
 }
 
+// StepperDopr5::prepare_dense (StepperDopr5.hpp:181-208) for one accepted
+// step of length h from y to yout: five coefficients rc (5 x n) from the
+// step's stages k[0..6] (k[1] is not read; k[6] = f(yout)).
+inline void dopri5_prepare_dense(const Dopri5Dense& D, size_t n, double h, const double* y,
+                                 const double* yout, const double* const* k,
+                                 std::vector<double>& rc) {
+  rc.assign(5 * n, 0.0);
+  for (size_t i = 0; i < n; ++i) {
+    const double ydiff = yout[i] - y[i];
+    const double bspl = h * k[0][i] - ydiff;
+    rc[i] = y[i];
+    rc[n + i] = ydiff;
+    rc[2 * n + i] = bspl;
+    rc[3 * n + i] = ydiff - h * k[6][i] - bspl;
+    double s = D.d[0] * k[D.idx[0]][i];
+    for (int m = 1; m < 6; ++m) s = s + D.d[m] * k[D.idx[m]][i];
+    rc[4 * n + i] = h * s;
+  }
+}
+
+// StepperDopr5::dense_out (:210-221).
+inline double dopri5_dense_eval(const std::vector<double>& rc, size_t n, size_t i,
+                                double xout, double x_end, double h) {
+  const double s = (xout - (x_end - h)) / h;
+  const double s1 = 1.0 - s;
+  const double* r = rc.data();
+  return r[i] + s * (r[n + i] + s1 * (r[2 * n + i] + s * (r[3 * n + i] + s1 * r[4 * n + i])));
+}
+
 }  // namespace detail
 
 // RKEmbeddedParallel<BT>::integrate (RKEmbeddedParallel.hpp:13-81), with
@@ -396,9 +427,11 @@ inline Result integrate(const Tableau& bt, const double* x0_in, size_t n_var,
   };
   save(t, x);
   const bool dense = !opt.dense_times.empty();
-  if (dense && bt.name != "DOP853")
-    throw std::invalid_argument("dense output needs DOP853");
+  const bool dense5 = bt.name == "DOPRI5";
+  if (dense && bt.name != "DOP853" && !dense5)
+    throw std::invalid_argument("dense output needs DOP853 or DOPRI5");
   const Dop853Dense DD = dop853_dense();
+  const Dopri5Dense D5 = dopri5_dense();
   size_t dense_next = 0;
   std::vector<double> k13(dense ? n_var : 0), rc, extra;
   long attempts = 0;
@@ -458,19 +491,28 @@ inline Result integrate(const Tableau& bt, const double* x0_in, size_t n_var,
       t += dt;
       if (dense && dense_next < opt.dense_times.size() && t > opt.dense_times[dense_next]) {
         // the reference's prepare_dense / dense_out on this step
-        ode(x.data(), k13.data(), exe);
-        const double* kp[13];
-        for (int j = 0; j < 12; ++j) kp[j] = ks.data() + j * n_var;
-        kp[12] = k13.data();
-        detail::dop853_prepare_dense(
-            DD, n_var, dt, x0.data(), x.data(), kp,
-            [&](int, const double* yt, double* out) { ode(yt, out, exe); }, rc, extra);
-        res.dense_rhs_evals += 4;
+        if (dense5) {
+          // DOPRI5's last stage is f at the new state (its stage input
+          // equals x bit for bit), so no evaluation is added
+          const double* kp[7];
+          for (int j = 0; j < 7; ++j) kp[j] = ks.data() + j * n_var;
+          detail::dopri5_prepare_dense(D5, n_var, dt, x0.data(), x.data(), kp, rc);
+        } else {
+          ode(x.data(), k13.data(), exe);
+          const double* kp[13];
+          for (int j = 0; j < 12; ++j) kp[j] = ks.data() + j * n_var;
+          kp[12] = k13.data();
+          detail::dop853_prepare_dense(
+              DD, n_var, dt, x0.data(), x.data(), kp,
+              [&](int, const double* yt, double* out) { ode(yt, out, exe); }, rc, extra);
+          res.dense_rhs_evals += 4;
+        }
         ++res.dense_steps;
         while (dense_next < opt.dense_times.size() && t > opt.dense_times[dense_next]) {
           for (size_t i = 0; i < n_var; ++i)
             res.dense_states.push_back(
-                detail::dop853_dense_eval(rc, n_var, i, opt.dense_times[dense_next], t, dt));
+                dense5 ? detail::dopri5_dense_eval(rc, n_var, i, opt.dense_times[dense_next], t, dt)
+                       : detail::dop853_dense_eval(rc, n_var, i, opt.dense_times[dense_next], t, dt));
           ++dense_next;
           ++res.n_dense;
         }
@@ -737,6 +779,142 @@ inline NrResult nr_dop853(std::vector<double> ystart, double x1, double x2,
     h = hnext;
     if ((x - x2) * (x2 - x1) >= 0.0) {
       out.y_final = y;  // ystart = y (ODEIntegrator.hpp:80)
+      return out;
+    }
+    if (std::abs(h) <= hmin) {
+      out.status = 2;
+      return out;
+    }
+  }
+  out.status = 1;
+  return out;
+}
+
+// ------------------------------------------------- NR-compatible DOPRI5 ----
+// ODEIntegrator<StepperDopr5<D, OP>> (NumericalRecipes/ODEIntegrator.hpp:66-88;
+// StepperDopr5.hpp:80-98 step, :109-177 dy, :224-231 error, :236-270
+// controller, :181-221 dense output), with the RawOutput or DenseOutput
+// policy (dense_intervals > 0, as nr_dop853).  Used to pin DOPRI5's
+// continuous extension against the NR goldens.
+inline NrResult nr_dopr5(std::vector<double> ystart, double x1, double x2, double atol,
+                         double rtol, double h1, double hmin, const NrDerivs& derivs,
+                         int dense_intervals = 0) {
+  const Tableau T = dopri5();
+  const std::vector<double> e = dopri5_nr_error_weights();
+  const Dopri5Dense D5 = dopri5_dense();
+  auto a = [&](int i, int j) { return T.A(i - 2, j - 1); };  // a_ij, 1-based
+  const double c[5] = {0.0, 0.2, 0.3, 0.8, 8.0 / 9.0};  // nodes c_1..c_5
+  const int n = static_cast<int>(ystart.size());
+  const double eps = std::numeric_limits<double>::epsilon();
+  std::vector<double> y = ystart, dydx(n), yout = ystart, yerr(n);
+  std::vector<double> k2(n), k3(n), k4(n), k5(n), k6(n), dydxnew(n), ytemp(n);
+  double x = x1, hnext = 0.0, errold = 1.0e-4, hdid = 0.0;
+  bool reject = false;
+  NrResult out;
+  const double width = dense_intervals > 0 ? (x2 - x1) / dense_intervals : 0.0;
+  double next_x = x1 + width;
+  bool first_save = true;
+  std::vector<double> rc;
+  auto save = [&] {
+    if (dense_intervals <= 0 || first_save) {
+      first_save = false;
+      out.xs.push_back(x);
+      out.ys.push_back(yout);
+      return;
+    }
+    const double* kp[7] = {dydx.data(), nullptr, k3.data(), k4.data(),
+                           k5.data(), k6.data(), dydxnew.data()};
+    detail::dopri5_prepare_dense(D5, static_cast<size_t>(n), hdid, y.data(), yout.data(), kp,
+                                 rc);
+    while ((x - next_x) * (x2 - x1) > 0.0) {
+      std::vector<double> yd(n);
+      for (int i = 0; i < n; ++i)
+        yd[i] = detail::dopri5_dense_eval(rc, static_cast<size_t>(n), i, next_x, x, hdid);
+      out.xs.push_back(next_x);
+      out.ys.push_back(yd);
+      next_x += width;
+    }
+  };
+  auto dy = [&](double h) {
+    int i;
+    for (i = 0; i < n; ++i) ytemp[i] = y[i] + h * a(2, 1) * dydx[i];
+    derivs(x + c[1] * h, ytemp, k2);
+    for (i = 0; i < n; ++i) ytemp[i] = y[i] + h * (a(3, 1) * dydx[i] + a(3, 2) * k2[i]);
+    derivs(x + c[2] * h, ytemp, k3);
+    for (i = 0; i < n; ++i)
+      ytemp[i] = y[i] + h * (a(4, 1) * dydx[i] + a(4, 2) * k2[i] + a(4, 3) * k3[i]);
+    derivs(x + c[3] * h, ytemp, k4);
+    for (i = 0; i < n; ++i)
+      ytemp[i] = y[i] + h * (a(5, 1) * dydx[i] + a(5, 2) * k2[i] + a(5, 3) * k3[i] +
+                             a(5, 4) * k4[i]);
+    derivs(x + c[4] * h, ytemp, k5);
+    for (i = 0; i < n; ++i)
+      ytemp[i] = y[i] + h * (a(6, 1) * dydx[i] + a(6, 2) * k2[i] + a(6, 3) * k3[i] +
+                             a(6, 4) * k4[i] + a(6, 5) * k5[i]);
+    const double xph = x + h;
+    derivs(xph, ytemp, k6);
+    for (i = 0; i < n; ++i)
+      yout[i] = y[i] + h * (a(7, 1) * dydx[i] + a(7, 3) * k3[i] + a(7, 4) * k4[i] +
+                            a(7, 5) * k5[i] + a(7, 6) * k6[i]);
+    derivs(xph, yout, dydxnew);
+    for (i = 0; i < n; ++i)
+      yerr[i] = h * (e[0] * dydx[i] + e[2] * k3[i] + e[3] * k4[i] + e[4] * k5[i] +
+                     e[5] * k6[i] + e[6] * dydxnew[i]);
+  };
+  auto error = [&] {
+    double err = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double sk = atol + rtol * std::max(std::fabs(y[i]), std::fabs(yout[i]));
+      err += std::pow(yerr[i] / sk, 2);
+    }
+    return std::sqrt(err / n);
+  };
+  auto success = [&](double err, double& h) {
+    const double beta = 0.0, alpha = 0.2 - beta * 0.75, safe = 0.9, minscale = 0.2,
+                 maxscale = 10.0;
+    double scale;
+    if (err <= 1.0) {
+      if (err == 0.0) {
+        scale = maxscale;
+      } else {
+        scale = safe * std::pow(err, -alpha) * std::pow(errold, beta);
+        if (scale < minscale) scale = minscale;
+        if (scale > maxscale) scale = maxscale;
+      }
+      hnext = reject ? h * std::min(scale, 1.0) : h * scale;
+      errold = std::max(err, 1.0e-4);
+      reject = false;
+      return true;
+    }
+    scale = std::max(safe * std::pow(err, -alpha), minscale);
+    h *= scale;
+    reject = true;
+    return false;
+  };
+  double h = x2 - x1 > 0.0 ? std::fabs(h1) : -std::fabs(h1);
+  derivs(x, y, dydx);
+  save();
+  const int max_step = 50000;
+  for (int nstp = 0; nstp < max_step; ++nstp) {
+    if ((x + h * 1.0001 - x2) * (x2 - x1) > 0.0) h = x2 - x;
+    double hh = h;
+    for (;;) {
+      dy(hh);
+      const double err = error();
+      if (success(err, hh)) break;
+      if (std::abs(hh) <= std::abs(x) * eps) {
+        out.status = 2;
+        return out;
+      }
+    }
+    x += hh;
+    hdid = hh;
+    save();
+    dydx = dydxnew;
+    y = yout;
+    h = hnext;
+    if ((x - x2) * (x2 - x1) >= 0.0) {
+      out.y_final = y;
       return out;
     }
     if (std::abs(h) <= hmin) {

@@ -211,6 +211,64 @@ int oracle_nr_dop853_dense(int problem, double y0a, double y0b, double x1, doubl
   return r.status;
 }
 
+// NR-compatible DOPRI5 (StepperDopr5 + RawOutput / DenseOutput), as
+// oracle_nr_dop853_dense; intervals = 0 selects RawOutput.
+int oracle_nr_dopr5(int problem, double y0a, double y0b, double x1, double x2, double atol,
+                    double rtol, double h1, double hmin, double mu_eps, int intervals, long cap,
+                    long* count, double* xs, double* y0s, double* y1s, double* y_final) {
+  NrResult r = nr_dopr5({y0a, y0b}, x1, x2, atol, rtol, h1, hmin, nr_problem(problem, mu_eps),
+                        intervals);
+  const long c = static_cast<long>(r.xs.size());
+  *count = c;
+  for (long k = 0; k < c && k < cap; ++k) {
+    xs[k] = r.xs[k];
+    y0s[k] = r.ys[k][0];
+    y1s[k] = r.ys[k][1];
+  }
+  if (r.y_final.size() == 2) {
+    y_final[0] = r.y_final[0];
+    y_final[1] = r.y_final[1];
+  }
+  return r.status;
+}
+
+// One DOPRI5 step of length h from y (RK form) followed by its continuous
+// extension at the fractions s_k of the step, as oracle_dop853_step_dense.
+int oracle_dopri5_step_dense(int rhs_kind, const double* y, size_t n_var, double h,
+                             double softening, const double* masses, const double* s_list,
+                             long n_s, double* yout, double* out) {
+  try {
+    Executor e = make_exe(0, 0, 1);
+    Rhs f = make_rhs(rhs_kind, n_var, softening, masses);
+    Tableau bt = dopri5();
+    const size_t n = n_var;
+    std::vector<double> ks(7 * n), tmp(n), rc;
+    f(y, ks.data(), e);
+    for (int stage = 1; stage < 7; ++stage) {
+      for (size_t i = 0; i < n; ++i) {
+        double acc = 0.0;
+        for (int j = 0; j < stage; ++j) acc += bt.A(stage - 1, j) * ks[j * n + i];
+        tmp[i] = y[i] + acc * h;
+      }
+      f(tmp.data(), ks.data() + stage * n, e);
+    }
+    for (size_t i = 0; i < n; ++i) {
+      double xs = 0.0;
+      for (int j = 0; j < 7; ++j) xs += bt.b[j] * ks[j * n + i];
+      yout[i] = y[i] + xs * h;
+    }
+    const double* kp[7];
+    for (int j = 0; j < 7; ++j) kp[j] = ks.data() + j * n;
+    detail::dopri5_prepare_dense(dopri5_dense(), n, h, y, yout, kp, rc);
+    for (long k = 0; k < n_s; ++k)
+      for (size_t i = 0; i < n; ++i)
+        out[k * n + i] = detail::dopri5_dense_eval(rc, n, i, s_list[k] * h, h, h);
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
 // One DOP853 step of length h from y (RK form: the twelve stages and the
 // 8th-order update of RKEmbeddedParallel.hpp:37-55) followed by its
 // continuous extension evaluated at the fractions s_k of the step

@@ -49,6 +49,8 @@ def lib():
                                              [_dp] * 4)
         L.oracle_dop853_step_dense.argtypes = [C.c_int, _dp, C.c_size_t, C.c_double,
                                                C.c_double, _dp, _dp, C.c_long, _dp, _dp]
+        L.oracle_dopri5_step_dense.argtypes = L.oracle_dop853_step_dense.argtypes
+        L.oracle_nr_dopr5.argtypes = L.oracle_nr_dop853_dense.argtypes
         L.oracle_result_count.argtypes = [C.c_void_p]
         L.oracle_result_count.restype = C.c_long
         L.oracle_result_times.argtypes = [C.c_void_p, _dp]
@@ -176,30 +178,48 @@ def nr_dop853(problem, y0, x1, x2, atol, rtol, h1, hmin=0.0, mu_eps=1e-3):
     return st, cnt.value, sel
 
 
-def nr_dop853_dense(problem, y0, x1, x2, atol, rtol, h1, intervals, hmin=0.0, mu_eps=1e-3,
-                    cap=100000):
-    """NR DOP853 with the DenseOutput policy: (status, xs, ys (count x 2), y_final)."""
+def _nr_call(fn, problem, y0, x1, x2, atol, rtol, h1, intervals, hmin, mu_eps, cap):
     cnt = C.c_long()
     xs, y0s, y1s, yf = np.zeros(cap), np.zeros(cap), np.zeros(cap), np.full(2, np.nan)
-    st = lib().oracle_nr_dop853_dense(problem, y0[0], y0[1], x1, x2, atol, rtol, h1, hmin,
-                                      mu_eps, intervals, cap, C.byref(cnt), _ptr(xs), _ptr(y0s),
-                                      _ptr(y1s), _ptr(yf))
+    st = fn(problem, y0[0], y0[1], x1, x2, atol, rtol, h1, hmin, mu_eps, intervals, cap,
+            C.byref(cnt), _ptr(xs), _ptr(y0s), _ptr(y1s), _ptr(yf))
     c = min(cnt.value, cap)
     return st, xs[:c], np.stack([y0s[:c], y1s[:c]], axis=1), yf
 
 
-def dop853_step_dense(rhs_kind, y, h, s_list, softening=0.0, masses=None):
-    """One RK-form DOP853 step of length h and its continuous extension at
-    the fractions s_list: (yout, dense (len(s_list) x n))."""
+def nr_dop853_dense(problem, y0, x1, x2, atol, rtol, h1, intervals, hmin=0.0, mu_eps=1e-3,
+                    cap=100000):
+    """NR DOP853 with the DenseOutput policy: (status, xs, ys (count x 2), y_final)."""
+    return _nr_call(lib().oracle_nr_dop853_dense, problem, y0, x1, x2, atol, rtol, h1,
+                    intervals, hmin, mu_eps, cap)
+
+
+def nr_dopr5(problem, y0, x1, x2, atol, rtol, h1, intervals=0, hmin=0.0, mu_eps=1e-3,
+             cap=100000):
+    """NR DOPRI5 with the RawOutput (intervals = 0) or DenseOutput policy:
+    (status, xs, ys (count x 2), y_final)."""
+    return _nr_call(lib().oracle_nr_dopr5, problem, y0, x1, x2, atol, rtol, h1, intervals,
+                    hmin, mu_eps, cap)
+
+
+def step_dense(tableau, rhs_kind, y, h, s_list, softening=0.0, masses=None):
+    """One RK-form step (DOP853 or DOPRI5) of length h and its continuous
+    extension at the fractions s_list: (yout, dense (len(s_list) x n))."""
+    fn = {"DOP853": lib().oracle_dop853_step_dense,
+          "DOPRI5": lib().oracle_dopri5_step_dense}[tableau]
     y = np.ascontiguousarray(y, dtype=np.float64)
     s_list = np.ascontiguousarray(s_list, dtype=np.float64)
     m = None if masses is None else np.ascontiguousarray(masses, dtype=np.float64)
     yout = np.zeros_like(y)
     out = np.zeros((s_list.size, y.size))
-    if lib().oracle_dop853_step_dense(rhs_kind, _ptr(y), y.size, h, softening, _ptr(m),
-                                      _ptr(s_list), s_list.size, _ptr(yout), _ptr(out)) != 0:
-        raise RuntimeError("oracle_dop853_step_dense failed")
+    if fn(rhs_kind, _ptr(y), y.size, h, softening, _ptr(m), _ptr(s_list), s_list.size,
+          _ptr(yout), _ptr(out)) != 0:
+        raise RuntimeError("oracle step_dense failed")
     return yout, out
+
+
+def dop853_step_dense(rhs_kind, y, h, s_list, softening=0.0, masses=None):
+    return step_dense("DOP853", rhs_kind, y, h, s_list, softening, masses)
 
 
 def energy(x, n, softening, masses=None):

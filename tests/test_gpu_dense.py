@@ -1,8 +1,10 @@
-"""DOP853 dense output of the device-resident solve against the CPU oracle.
+"""DOP853 / DOPRI5 dense output of the device-resident solve against the CPU
+oracle.
 
 The oracle's continuous extension is pinned by the reference's dense-output
-goldens (test_oracle_dense.py); here the device pass (three extra RHS
-evaluations per covering step, the polynomial evaluated per element) must
+goldens (test_oracle_dense.py); here the device pass (for DOP853 three extra
+RHS evaluations per covering step, for DOPRI5 none; the polynomial evaluated
+per element) must
 reproduce it bit for bit on the same solve: the written rows, the steps that
 wrote them and the solve itself, which dense output must not change.
 """
@@ -18,24 +20,25 @@ def bits(a):
     return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
 
 
-def oracle_dense(rhs_kind, x0, tf, tol, times, softening=0.0, masses=None):
-    return O.integrate("DOP853", rhs_kind, x0, 0.0, tf, tol, tol, softening=softening,
+def oracle_dense(rhs_kind, x0, tf, tol, times, softening=0.0, masses=None, tableau="DOP853"):
+    return O.integrate(tableau, rhs_kind, x0, 0.0, tf, tol, tol, softening=softening,
                        masses=masses, reduce=O.HIP_ORDER, pow_mode=O.POW_CORRECT,
                        store_states=False, threads=8, dense_times=times)
 
 
+@pytest.mark.parametrize("tableau", ["DOP853", "DOPRI5"])
 @pytest.mark.parametrize("n,kind,variant_eps", [(64, "cube", 1e-3), (300, "plummer", 1e-2),
                                                 (1000, "plummer", 1e-2)])
-def test_nbody_dense_bit_identical(exe, n, kind, variant_eps):
+def test_nbody_dense_bit_identical(exe, n, kind, variant_eps, tableau):
     import ode_prototyping_amd as oh
     from ode_prototyping_amd import scenarios
     x0, m = (scenarios.plummer(n, seed=4) if kind == "plummer"
              else scenarios.uniform_cube(n, seed=5))
     tf = 0.05 if n <= 300 else 0.02
     times = np.linspace(0.0, tf, 26)[1:-1]
-    print(f"dense nbody n={n} kind={kind}", flush=True)
-    ref = oracle_dense(O.RHS_NBODY, x0, tf, 1e-10, times, variant_eps, m)
-    res = oh.integrate("DOP853", x0, 0.0, tf, 1e-10, 1e-10,
+    print(f"dense nbody {tableau} n={n} kind={kind}", flush=True)
+    ref = oracle_dense(O.RHS_NBODY, x0, tf, 1e-10, times, variant_eps, m, tableau)
+    res = oh.integrate(tableau, x0, 0.0, tf, 1e-10, 1e-10,
                        oh.NBodyODE(variant_eps, m, "exact"), exe=exe, dense_times=times)
     assert res.status == "reached_end"
     assert (res.accepted, res.rejected, res.rhs_evals) == (ref.accepted, ref.rejected,
@@ -47,16 +50,17 @@ def test_nbody_dense_bit_identical(exe, n, kind, variant_eps):
     assert np.array_equal(bits(res.dense_states), bits(ref.dense_states))
 
 
+@pytest.mark.parametrize("tableau", ["DOP853", "DOPRI5"])
 @pytest.mark.parametrize("rhs", ["vanderpol", "exponential"])
-def test_simple_dense_bit_identical(exe, rhs):
+def test_simple_dense_bit_identical(exe, rhs, tableau):
     import ode_prototyping_amd as oh
     if rhs == "vanderpol":
         x0, ode, kind, tf = np.array([2.0, 0.0]), oh.VanDerPolODE(), O.RHS_VANDERPOL, 2.0
     else:
         x0, ode, kind, tf = np.arange(10.0), oh.ExponentialODE(), O.RHS_EXPONENTIAL, 3.0
     times = np.linspace(0.0, tf, 41)[1:]  # the last (tf) is never written
-    ref = oracle_dense(kind, x0, tf, 1e-10, times)
-    res = oh.integrate("DOP853", x0, 0.0, tf, 1e-10, 1e-10, ode, exe=exe, dense_times=times)
+    ref = oracle_dense(kind, x0, tf, 1e-10, times, tableau=tableau)
+    res = oh.integrate(tableau, x0, 0.0, tf, 1e-10, 1e-10, ode, exe=exe, dense_times=times)
     assert res.n_dense == ref.n_dense == len(times) - 1
     assert np.array_equal(bits(res.dense_states), bits(ref.dense_states))
     assert np.array_equal(bits(res.x_final), bits(ref.x_final))
@@ -115,8 +119,8 @@ def test_dense_output_argument_errors(exe):
     import ode_prototyping_amd as oh
     x0 = np.array([2.0, 0.0])
     ode = oh.VanDerPolODE()
-    with pytest.raises(oh.core.OdeHipError, match="DOP853"):
-        oh.integrate("DOPRI5", x0, 0.0, 1.0, 1e-8, 1e-8, ode, exe=exe, dense_times=[0.5])
+    with pytest.raises(oh.core.OdeHipError, match="DOP853 or DOPRI5"):
+        oh.integrate("RKF78", x0, 0.0, 1.0, 1e-8, 1e-8, ode, exe=exe, dense_times=[0.5])
     with pytest.raises(oh.core.OdeHipError, match="increasing"):
         oh.integrate("DOP853", x0, 0.0, 1.0, 1e-8, 1e-8, ode, exe=exe, dense_times=[0.5, 0.5])
     with pytest.raises(oh.core.OdeHipError, match="increasing"):
@@ -124,3 +128,26 @@ def test_dense_output_argument_errors(exe):
     # the context is still usable
     r = oh.integrate("DOP853", x0, 0.0, 1.0, 1e-8, 1e-8, ode, exe=exe, dense_times=[0.5])
     assert r.n_dense == 1
+
+
+def test_dopri5_dense_costs_no_evaluation(exe):
+    """DOPRI5's extension reads its seventh stage (f at the new state): the
+    solve, with or without k1 reuse, is the solve without dense output, and
+    no RHS evaluation is added."""
+    import ode_prototyping_amd as oh
+    from ode_prototyping_amd import scenarios
+    x0, m = scenarios.uniform_cube(150, seed=12)
+    ode = oh.NBodyODE(1e-3, m, "exact")
+    times = np.linspace(0.0, 0.05, 21)[1:]
+    runs = {}
+    for fsal in (False, True):
+        for dense in (False, True):
+            runs[fsal, dense] = oh.integrate("DOPRI5", x0, 0.0, 0.05, 1e-10, 1e-10, ode, exe=exe,
+                                             fsal=fsal, trace_cap=4096,
+                                             dense_times=times if dense else None)
+    for fsal in (False, True):
+        a, b = runs[fsal, False], runs[fsal, True]
+        assert np.array_equal(bits(a.x_final), bits(b.x_final))
+        assert np.array_equal(bits(a.trace), bits(b.trace))
+        assert a.rhs_evals == b.rhs_evals and b.dense_rhs_evals == 0 and b.n_dense == 19
+    assert np.array_equal(bits(runs[False, True].dense_states), bits(runs[True, True].dense_states))

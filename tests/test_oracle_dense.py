@@ -1,4 +1,4 @@
-"""DOP853 dense output in the CPU oracle (CPU only).
+"""DOP853 and DOPRI5 dense output in the CPU oracle (CPU only).
 
 Pins of the continuous extension (the reference's
 StepperDopr853::prepare_dense / dense_out, NumericalRecipes/
@@ -23,6 +23,12 @@ StepperDopr853.hpp:200-289, with the DenseOutput policy, Output.hpp:63-147):
   cos/sin to the tolerance;
 * the RK-form integrator's dense output agrees with its own accepted states
   at the accepted times.
+
+DOPRI5 (StepperDopr5::prepare_dense / dense_out, StepperDopr5.hpp:181-221,
+no extra stages): the NR DOPRI5 restatement reproduces both raw-output
+goldens bit for bit and the nonzero-start dense golden bit for bit; the two
+other dense goldens within their EXPECT_DOUBLE_EQ (see the test); the
+extension is 4th order (error ~h^5 inside a step).
 """
 import numpy as np
 import pytest
@@ -160,7 +166,116 @@ def test_rk_form_dense_output_agrees_with_accepted_states():
     assert np.max(np.abs(r.dense_states - states)) < 1e-12 * np.max(np.abs(states))
 
 
-def test_rk_form_dense_needs_dop853():
+def test_rk_form_dense_needs_dop853_or_dopri5():
     with pytest.raises(RuntimeError):
-        O.integrate("DOPRI5", O.RHS_VANDERPOL, [2.0, 0.0], 0.0, 1.0, 1e-8, 1e-8,
+        O.integrate("RKF78", O.RHS_VANDERPOL, [2.0, 0.0], 0.0, 1.0, 1e-8, 1e-8,
                     dense_times=[0.5])
+
+
+# ---------------------------------------------------------------- DOPRI5 --
+# NumericalRecipes/test/SimpleOscillatorTest.cpp and VanDerPolTest.cpp
+# (StepperDopr5 with RawOutput and DenseOutput).
+D5_OSC_Y0 = [1.0, 0.995004165274381, 0.9800665778342761, 0.9553364891156162, 0.9210609939762899,
+             0.8775825618689674, 0.8253356148906368, 0.7648421872376728, 0.6967067092917164,
+             0.6216099682354068, 0.540302305843194, 0.45359612136859212, 0.36235775441965129,
+             0.2674988286034845, 0.1699671428597921, 0.070737201637418756, -0.029199522309753027,
+             -0.12884449432018022, -0.22720209468205022, -0.32328956688476052,
+             -0.41614683651997625]
+D5_OSC_Y1 = [0.0, -0.09983341667072441, -0.19866933081341775, -0.29552020666230877,
+             -0.38941834234718053, -0.47942553861305215, -0.5646424733839779, -0.6442176872563701,
+             -0.7173560909168458, -0.7833269096128057, -0.8414709847747586, -0.8912073600416666,
+             -0.9320390859400902, -0.9635581853673072, -0.9854497299379401, -0.9974949865471938,
+             -0.9995736029766767, -0.99166481038394072, -0.97384763080764247,
+             -0.94630008760441731, -0.9092974267525671]
+D5_VDP_Y0 = [2.0, 1.9316114668840196, 1.8584936578161697, 1.779738838649533, 1.6936304673029232,
+             1.5973233455330451, 1.4853988650659193, 1.3445145377549466, 1.1027432107238968,
+             -1.9607574926907294, -1.8896363707127886, -1.8134015921664299, -1.7306276596111723,
+             -1.6390360829064694, -1.5345832043822263, -1.4085081362013523, -1.231625033046776,
+             1.9878333640723733, 1.9184567236794956, 1.844394686471839, 1.7644320190605265]
+D5_VDP_Y1 = [0.0, -0.70723585006735068, -0.75690138151347441, -0.82140897529267398,
+             -0.90658463142947066, -1.0280880445966889, -1.2278827867432773, -1.6569111260367424,
+             -4.4853706789511456, 0.68962769177543404, 0.73633857873973074, 0.79157138514815228,
+             0.86880585055426407, 0.96881359325147665, 1.1314276223440602, 1.428107104729363,
+             2.3390319866124534, -0.67412345520940009, -0.71620744619303756, -0.7682889892922099,
+             -0.83427005245677999]
+
+
+def run5(problem, intervals=0, x1=0.0):
+    if problem == "vanderpol":
+        return O.nr_dopr5(1, [2.0, 0.0], x1, 2.0, 1e-3, 1e-3, 0.01, intervals, mu_eps=1e-3)
+    return O.nr_dopr5(0, [1.0, 0.0], x1, 2.0, 1e-10, 1e-10, 0.01, intervals)
+
+
+@pytest.mark.parametrize("problem,count,x_mid,y_mid,y_last", [
+    ("oscillator", 49, 1.0004288786936162, (0.53994136718681807, -0.84170263152593661),
+     (-0.41614683651997603, -0.90929742675256664)),
+    ("vanderpol", 1177, 0.98436957642198519, (-1.9010597230846302, 0.72447110662737324),
+     (1.7644320190605265, -0.83427005245677999))])
+def test_nr_dopr5_raw_goldens(problem, count, x_mid, y_mid, y_last):
+    """SimpleOscillatorTest.cpp:40-63, VanDerPolTest.cpp:38-76: the step
+    sequence of the NR DOPRI5 stepper, bit for bit."""
+    st, xs, ys, yf = run5(problem)
+    assert st == 0 and len(xs) == count
+    m = count // 2
+    assert xs[0] == 0.0 and xs[m] == x_mid and xs[-1] == 2.0
+    assert tuple(ys[m]) == y_mid and tuple(ys[-1]) == y_last and tuple(yf) == y_last
+
+
+@pytest.mark.parametrize("problem,g0,g1", [("oscillator", D5_OSC_Y0, D5_OSC_Y1),
+                                           ("vanderpol", D5_VDP_Y0, D5_VDP_Y1)])
+def test_nr_dopr5_dense_consistency_goldens(problem, g0, g1):
+    """SimpleOscillatorTest.cpp:154-240, VanDerPolTest.cpp:167-250, to the
+    goldens' EXPECT_DOUBLE_EQ: these runs carry a slightly different
+    trajectory than the raw goldens of the same files (the oscillator's final
+    state differs from the raw run's by 4 ulp there, while the restatement's
+    equals the raw golden bit for bit), so they are met within 4 ulp, not
+    exactly."""
+    st, xs, ys, yf = run5(problem, 20)
+    assert st == 0 and len(xs) == 20
+    assert np.array_equal(xs, np.array(GOLD_X[:20]))
+    d = np.concatenate([ulp_diff(ys[:, 0], g0[:20]), ulp_diff(ys[:, 1], g1[:20]),
+                        ulp_diff(yf, [g0[-1], g1[-1]])])
+    assert np.max(d) <= 4.0
+
+
+def test_nr_dopr5_dense_nonzero_start_golden():
+    """SimpleOscillatorTest.cpp:242-280 (start at x = 1, five intervals): every
+    y value and the final state bit for bit; the last x is 2 - 2^-52 (the
+    golden writes it as 2.0, within its EXPECT_DOUBLE_EQ)."""
+    st, xs, ys, yf = run5("oscillator", 5, x1=1.0)
+    gx = [1.0, 1.2, 1.3999999999999999, 1.5999999999999999, 1.7999999999999998, 2.0]
+    g0 = [1.0, 0.9800665778342762, 0.92106099397628982, 0.82533561489063678, 0.6967067092917163,
+          0.54030230584326977]
+    g1 = [0.0, -0.19866933081341756, -0.38941834234718037, -0.56464247338397766,
+          -0.71735609091684549, -0.8414709847748435]
+    assert st == 0 and len(xs) == 6
+    assert np.array_equal(xs[:5], gx[:5]) and xs[5] == np.nextafter(2.0, 0.0)
+    assert np.array_equal(ys[:, 0], g0) and np.array_equal(ys[:, 1], g1)
+    assert yf[0] == 0.54030230584326955 and yf[1] == -0.84147098477484361
+
+
+def test_dopri5_extension_endpoints_and_order():
+    """DOPRI5's continuous extension is 4th order: inside a step of y' = y its
+    error falls by ~2^5 per halving of h; s = 0 and s = 1 reproduce the
+    step's ends."""
+    y = np.array([1.0, 2.0, -0.5])
+    s = np.array([0.0, 0.3, 0.5, 0.8, 1.0])
+    errs = []
+    for h in (0.2, 0.1):
+        yout, d = O.step_dense("DOPRI5", O.RHS_EXPONENTIAL, y, h, s)
+        assert np.array_equal(d[0], y)
+        assert np.max(ulp_diff(d[-1], yout)) <= 1.0
+        errs.append(np.max(np.abs(d[1:4] - np.exp(np.outer(s[1:4], h)) * y), axis=1))
+    ratio = errs[0] / errs[1]
+    assert np.all(ratio > 25) and np.all(ratio < 40), ratio
+
+
+def test_rk_form_dopri5_dense_output_agrees_with_accepted_states():
+    x0 = np.array([2.0, 0.0])
+    plain = O.integrate("DOPRI5", O.RHS_VANDERPOL, x0, 0.0, 2.0, 1e-8, 1e-8)
+    inner = plain.times[1:-1]
+    r = O.integrate("DOPRI5", O.RHS_VANDERPOL, x0, 0.0, 2.0, 1e-8, 1e-8, dense_times=inner)
+    assert np.array_equal(r.times, plain.times) and np.array_equal(r.x_final, plain.x_final)
+    assert r.n_dense == len(inner) and r.dense_rhs_evals == 0
+    states = np.array([plain.state(k) for k in range(1, len(plain.times) - 1)])
+    assert np.max(np.abs(r.dense_states - states)) < 1e-12 * np.max(np.abs(states))

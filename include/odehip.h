@@ -208,7 +208,8 @@ typedef struct {
                             across a rejection -- same results, fewer RHS
                             evaluations.  0: the reference's S evaluations
                             per attempt. */
-  int attempts_per_launch; /* attempts per captured graph; 0 = automatic */
+  int attempts_per_launch; /* attempts per captured graph; 0 = automatic;
+                              negative: no graph, attempts launched directly */
   double* times;         /* host, times_cap entries: t0 then t after each
                             accepted step (RawOutput::times()); may be NULL */
   long times_cap;

@@ -151,3 +151,28 @@ def test_dopri5_dense_costs_no_evaluation(exe):
         assert np.array_equal(bits(a.trace), bits(b.trace))
         assert a.rhs_evals == b.rhs_evals and b.dense_rhs_evals == 0 and b.n_dense == 19
     assert np.array_equal(bits(runs[False, True].dense_states), bits(runs[True, True].dense_states))
+
+
+@pytest.mark.parametrize("tableau", ["DOP853", "DOPRI5"])
+def test_dense_output_direct_launch_and_budget(exe, tableau):
+    """The same rows with attempts launched directly instead of from the
+    captured graph; and a solve stopped by an accepted-step budget writes
+    exactly the rows strictly before the time it reached."""
+    import ode_prototyping_amd as oh
+    from ode_prototyping_amd import scenarios
+    x0, m = scenarios.plummer(96, seed=21)
+    ode = oh.NBodyODE(1e-2, m, "exact")
+    times = np.linspace(0.0, 0.05, 31)[1:-1]
+    g = oh.integrate(tableau, x0, 0.0, 0.05, 1e-10, 1e-10, ode, exe=exe, dense_times=times)
+    d = oh.integrate(tableau, x0, 0.0, 0.05, 1e-10, 1e-10, ode, exe=exe, dense_times=times,
+                     attempts_per_launch=-1)
+    assert g.n_dense == d.n_dense == len(times)
+    assert np.array_equal(bits(g.dense_states), bits(d.dense_states))
+    assert np.array_equal(bits(g.x_final), bits(d.x_final))
+    half = g.accepted // 2
+    b = oh.integrate(tableau, x0, 0.0, 0.05, 1e-10, 1e-10, ode, exe=exe, dense_times=times,
+                     max_accepted=half)
+    assert b.status == "max_accepted" and b.t < 0.05
+    k = int(np.sum(times < b.t))
+    assert b.n_dense == k and 0 < k < len(times)
+    assert np.array_equal(bits(b.dense_states), bits(g.dense_states[:k]))

@@ -390,6 +390,15 @@ typedef struct {
   double* tune_ms;       /* host, ODEHIP_ENS_TILE_CONFIGS entries: ms per
                             right-hand side of every candidate (autotune
                             only; untried entries 0); may be NULL */
+  /* DOPRI5 dense output of every system at common times, as
+   * odehip_solve_options.dense_times: dense_times host, n_dense strictly
+   * increasing times after t0; dense_states host, n_dense x (total state
+   * length), each row packed like x0.  A system's row r is written when one
+   * of its accepted steps covers dense_times[r]; rows it does not reach
+   * (odehip_system_stats.n_dense) are NaN.  No extra RHS evaluations. */
+  const double* dense_times;
+  long n_dense;
+  double* dense_states;
 } odehip_ensemble_options;
 
 typedef struct {
@@ -397,6 +406,7 @@ typedef struct {
   int status;            /* ODEHIP_STATUS_* */
   int pad;
   double t, dt, dt_initial, last_err;
+  long n_dense;          /* dense-output rows written for this system */
 } odehip_system_stats;
 
 typedef struct {

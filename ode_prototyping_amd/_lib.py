@@ -57,14 +57,17 @@ class EnsembleOptions(C.Structure):
                 ("rtol", C.c_double), ("max_accepted", C.c_long), ("max_attempts", C.c_long),
                 ("fsal", C.c_int), ("rounds_per_launch", C.c_int), ("tile_config", C.c_int),
                 ("controller", C.c_int), ("mlp", C.c_void_p), ("trace", C.c_void_p),
-                ("trace_cap", C.c_long), ("tune_ms", C.c_void_p)]
+                ("trace_cap", C.c_long), ("tune_ms", C.c_void_p),
+                ("dense_times", C.c_void_p), ("n_dense", C.c_long),
+                ("dense_states", C.c_void_p)]
 
 
 class SystemStats(C.Structure):
     _fields_ = [("accepted", C.c_long), ("rejected", C.c_long), ("attempts", C.c_long),
                 ("rhs_evals", C.c_long), ("n_trace", C.c_long), ("learned_steps", C.c_long),
                 ("status", C.c_int), ("pad", C.c_int), ("t", C.c_double), ("dt", C.c_double),
-                ("dt_initial", C.c_double), ("last_err", C.c_double)]
+                ("dt_initial", C.c_double), ("last_err", C.c_double),
+                ("n_dense", C.c_long)]
 
 
 class EnsembleStats(C.Structure):

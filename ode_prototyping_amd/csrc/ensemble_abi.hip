@@ -6,6 +6,7 @@
 // a HIP graph and replayed until the device-side count of running systems
 // reaches zero; the host polls that count one replay behind.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <numeric>
@@ -156,11 +157,11 @@ size_t max_tiles(const Layout& L) {
 // Device arena: tables, per-system state, vectors.
 struct Plan {
   size_t sys, groups, groups_act, tiles, st, n_active, xb0, xb1, ks, ytmp, kfsal,
-      sc0, partA, partB, trace, total;
+      sc0, partA, partB, trace, dense_t, dense_out, total;
 };
 
 Plan plan(const Layout& L, size_t tiles_cap, int S, long trace_cap,
-          bool with_state) {
+          bool with_state, long n_dense = 0) {
   Plan p{};
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -186,6 +187,8 @@ Plan plan(const Layout& L, size_t tiles_cap, int S, long trace_cap,
     p.partA = take(L.groups.size() * sizeof(double));
     p.partB = take(L.groups.size() * sizeof(double));
     p.trace = take(ns * static_cast<size_t>(trace_cap) * 3 * sizeof(double));
+    p.dense_t = take(static_cast<size_t>(n_dense) * sizeof(double));
+    p.dense_out = take(static_cast<size_t>(n_dense) * v);
   }
   p.total = off;
   return p;
@@ -387,12 +390,28 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
     return fail(ctx, ODEHIP_ERR_INVALID, "bad controller");
   if (opt->controller == ODEHIP_CONTROLLER_LEARNED && !opt->mlp)
     return fail(ctx, ODEHIP_ERR_INVALID, "the learned controller needs a network (mlp)");
+  const long n_dense = opt->n_dense;
+  if (n_dense < 0) return fail(ctx, ODEHIP_ERR_INVALID, "negative n_dense");
+  if (n_dense > 0) {
+    if (tableau != ODEHIP_TABLEAU_DOPRI5)
+      return fail(ctx, ODEHIP_ERR_INVALID, "ensemble dense output needs DOPRI5");
+    if (!opt->dense_times || !opt->dense_states)
+      return fail(ctx, ODEHIP_ERR_INVALID, "dense_times / dense_states null");
+    double prev = opt->t0;
+    for (long r = 0; r < n_dense; ++r) {
+      const double tr = opt->dense_times[r];
+      if (!std::isfinite(tr) || !(tr > prev))
+        return fail(ctx, ODEHIP_ERR_INVALID,
+                    "dense_times must be finite and strictly increasing after t0");
+      prev = tr;
+    }
+  }
   Layout L;
   if ((rc = make_layout(ctx, n_systems, n_particles, &L))) return rc;
   const int S = kTableaus[tableau].S;
   const long trace_cap = opt->trace ? opt->trace_cap : 0;
   const size_t tcap = max_tiles(L);
-  const Plan p = plan(L, tcap, S, trace_cap, true);
+  const Plan p = plan(L, tcap, S, trace_cap, true, n_dense);
   EnsembleWorkspace* w = nullptr;
   if ((rc = workspace(ctx, p.total, &w))) return rc;
   const double eps2 = softening * softening;
@@ -409,6 +428,14 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
   b.eps2 = eps2;
   b.controller = opt->controller;
   b.mlp = opt->controller == ODEHIP_CONTROLLER_LEARNED ? mlp_infer_ptr(opt->mlp) : nullptr;
+  if (n_dense > 0) {
+    b.dense_t = reinterpret_cast<const double*>(w->dev + p.dense_t);
+    b.dense_out = reinterpret_cast<double*>(w->dev + p.dense_out);
+    b.n_dense = n_dense;
+    HIP_TRY(ctx, hipMemcpyAsync(w->dev + p.dense_t, opt->dense_times, n_dense * sizeof(double),
+                                hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, launch_fill(b.dense_out, NAN, static_cast<size_t>(n_dense) * L.nv, s));
+  }
 
   // tile configuration
   int cfg = opt->tile_config;
@@ -538,6 +565,10 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
     HIP_TRY(ctx, hipMemcpy(opt->trace, b.trace,
                            hs.size() * static_cast<size_t>(trace_cap) * 3 * sizeof(double),
                            hipMemcpyDeviceToHost));
+  if (n_dense > 0)
+    HIP_TRY(ctx, hipMemcpy(opt->dense_states, b.dense_out,
+                           static_cast<size_t>(n_dense) * L.nv * sizeof(double),
+                           hipMemcpyDeviceToHost));
   end_work(ctx);
   odehip_ensemble_stats agg{};
   agg.tile_config = cfg;
@@ -567,6 +598,7 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
       o.dt = h.dt;
       o.dt_initial = h.dt_initial;
       o.last_err = h.err;
+      o.n_dense = n_dense > 0 ? h.dense_next : 0;
     }
   }
   if (stats) *stats = agg;

@@ -29,6 +29,8 @@ __constant__ Tableau e_tab[ODEHIP_NUM_TABLEAUS] = {
     kTableaus[0], kTableaus[1], kTableaus[2],
     kTableaus[3], kTableaus[4], kTableaus[5]};
 
+__constant__ Dopri5Dense e_dense5 = kDopri5Dense;
+
 constexpr int kBlock = ODEHIP_REDUCE_GROUP;
 
 typedef const __attribute__((address_space(4))) double* uniform_ptr;
@@ -441,6 +443,25 @@ __global__ __launch_bounds__(64) void k_ens_controller_learned(EnsBufs b) {
   }
 }
 
+// DOPRI5 dense output of every system whose attempt of this round was
+// accepted and covered an output time (StepperDopr5::prepare_dense /
+// dense_out and DenseOutput::save, as the single-system pass): the gate, one
+// thread per system, then the rows, one workgroup per reduction group.  A
+// system that finished in this round still writes its last step's rows.
+__global__ void k_ens_dense_begin(EnsBufs b) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= b.n_sys) return;
+  dense_gate(b.st + s, b.dense_t, b.n_dense, 0);
+}
+
+__global__ __launch_bounds__(kBlock) void k_ens_dense_out5(EnsBufs b) {
+  const GroupPos p = group_pos(b);
+  const SolverState* st = b.st + p.sys;
+  if (!st->dense_go || !p.valid) return;
+  dopri5_dense_rows(e_dense5, st, b.ks, b.nv, p.e, st->cur ? b.xb0 : b.xb1,
+                    st->cur ? b.xb1 : b.xb0, b.dense_t, b.dense_out, b.nv);
+}
+
 __global__ void k_ens_init(EnsBufs b, double t0, double tf, long max_acc,
                            long max_att, int fsal, int controller) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -610,6 +631,10 @@ hipError_t ens_enqueue_round(const EnsBufs& b, hipStream_t s) {
                        b);
   } else {
     hipLaunchKernelGGL(k_ens_controller, dim3(sys_grid(b)), dim3(64), 0, s, b);
+  }
+  if (b.n_dense > 0) {
+    hipLaunchKernelGGL(k_ens_dense_begin, dim3(sys_grid(b)), dim3(64), 0, s, b);
+    hipLaunchKernelGGL(k_ens_dense_out5, dim3(groups_grid(b)), dim3(kBlock), 0, s, b);
   }
   return hipGetLastError();
 }

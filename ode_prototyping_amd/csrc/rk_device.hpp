@@ -271,6 +271,62 @@ __device__ inline void ctrl_attempt(SolverState* st, const Tableau& T,
     ctrl_step(st, accept, NAN);
 }
 
+// ------------------------------------------------------- dense output ----
+
+// DenseOutput::save's test (NumericalRecipes/Output.hpp:109) after an
+// attempt: if it was accepted and ended strictly after the next output
+// time(s), rows [dense_lo, dense_hi) are this step's and dense_go is set.
+// extra_rhs: RHS evaluations the pass costs (counted; with DOP853 f at the
+// step's end also becomes the next k1 when k1 reuse is on).
+__device__ inline void dense_gate(SolverState* st, const double* times, long n_dense,
+                                  int extra_rhs) {
+  int go = 0;
+  if (st->just_accepted && st->accepted != st->dense_seen) {
+    st->dense_seen = st->accepted;
+    const long lo = st->dense_next;
+    long hi = lo;
+    const double t = st->t;
+    while (hi < n_dense && t > times[hi]) ++hi;
+    if (hi > lo) {
+      go = 1;
+      st->dense_lo = lo;
+      st->dense_hi = hi;
+      st->dense_next = hi;
+      st->dense_steps += 1;
+      st->dense_rhs_evals += extra_rhs;
+      if (extra_rhs && st->fsal) st->k1_valid = 1;
+    }
+  }
+  st->dense_go = go;
+}
+
+// StepperDopr5::prepare_dense / dense_out (StepperDopr5.hpp:195-221) for
+// element e of a state whose stage vectors are nv apart in ks (stage 7 = f at
+// the new state): the step went from y to yout over st->dt_acc ending at
+// st->t; writes out[row * row_stride + e] for the pass's rows.
+__device__ inline void dopri5_dense_rows(const Dopri5Dense& D, const SolverState* st,
+                                         const double* ks, size_t nv, size_t e,
+                                         const double* y, const double* yout,
+                                         const double* times, double* out,
+                                         size_t row_stride) {
+  const double h = st->dt_acc;
+  const double yi = y[e];
+  const double ydiff = yout[e] - yi;
+  const double bspl = h * ks[e] - ydiff;
+  const double rc3 = ydiff - h * ks[6 * nv + e] - bspl;
+  double acc = D.d[0] * ks[static_cast<size_t>(D.idx[0]) * nv + e];
+  for (int m = 1; m < 6; ++m) acc = acc + D.d[m] * ks[static_cast<size_t>(D.idx[m]) * nv + e];
+  const double rc4 = h * acc;
+  const double t_end = st->t;
+  const long hi = st->dense_hi;
+  for (long row = st->dense_lo; row < hi; ++row) {
+    const double s = (times[row] - (t_end - h)) / h;
+    const double s1 = 1.0 - s;
+    out[static_cast<size_t>(row) * row_stride + e] =
+        yi + s * (ydiff + s1 * (bspl + s * (rc3 + s1 * rc4)));
+  }
+}
+
 // estimate_initial_step (RKEmbeddedParallel.hpp:183-185): d0, d1, dt0 from
 // the sums of squares of x0/sc and f0/sc.
 __device__ inline void ctrl_init_step_a(SolverState* st, double sa, double sb,

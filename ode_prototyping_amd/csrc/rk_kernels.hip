@@ -273,31 +273,11 @@ __device__ inline const double* dense_vec(const double* ks, const double* k13, i
   return ks + (j - 12) * n;
 }
 
-// DenseOutput::save's test (Output.hpp:109): the output times strictly
-// before the end of the step just accepted get rows [lo, hi) of this pass.
-// DOP853 (extra_rhs = 4): f at the step's end becomes the next attempt's k1
-// when k1 reuse is on.  DOPRI5 (extra_rhs = 0) needs no evaluation.
+// The gate of the dense pass (one thread).
 __global__ void k_dense_begin(SolverState* st, const double* times, long n_dense,
                               int extra_rhs) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  int go = 0;
-  if (st->just_accepted && st->accepted != st->dense_seen) {
-    st->dense_seen = st->accepted;
-    const long lo = st->dense_next;
-    long hi = lo;
-    const double t = st->t;
-    while (hi < n_dense && t > times[hi]) ++hi;
-    if (hi > lo) {
-      go = 1;
-      st->dense_lo = lo;
-      st->dense_hi = hi;
-      st->dense_next = hi;
-      st->dense_steps += 1;
-      st->dense_rhs_evals += extra_rhs;
-      if (extra_rhs && st->fsal) st->k1_valid = 1;
-    }
-  }
-  st->dense_go = go;
+  dense_gate(st, times, n_dense, extra_rhs);
 }
 
 // The input of extra stage 13 + e (StepperDopr853.hpp:239-262).
@@ -347,34 +327,17 @@ __global__ __launch_bounds__(kBlock) void k_dense_out(
   }
 }
 
-// DOPRI5: the five coefficients of element i (StepperDopr5::prepare_dense,
-// StepperDopr5.hpp:195-207) from stages 1, 3..7 -- stage 7 is f at the new
-// state, its input being the new state bit for bit -- and the rows of this
-// pass (dense_out, :210-221).
+// DOPRI5: the five coefficients of element i from stages 1, 3..7 -- stage 7
+// is f at the new state, its input being the new state bit for bit -- and the
+// rows of this pass.
 __global__ __launch_bounds__(kBlock) void k_dense_out5(
     const SolverState* st, const double* ks, const double* xb0, const double* xb1,
     const double* times, double* out, size_t n) {
   if (!st->dense_go) return;
   const size_t i = gid();
   if (i >= n) return;
-  const Dopri5Dense& D = c_dense5;
-  const double* y = st->cur ? xb0 : xb1;
-  const double* yout = st->cur ? xb1 : xb0;
-  const double h = st->dt_acc;
-  const double yi = y[i];
-  const double ydiff = yout[i] - yi;
-  const double bspl = h * ks[i] - ydiff;
-  const double rc3 = ydiff - h * ks[6 * n + i] - bspl;
-  double acc = D.d[0] * ks[static_cast<size_t>(D.idx[0]) * n + i];
-  for (int m = 1; m < 6; ++m) acc = acc + D.d[m] * ks[static_cast<size_t>(D.idx[m]) * n + i];
-  const double rc4 = h * acc;
-  const double t_end = st->t;
-  const long hi = st->dense_hi;
-  for (long row = st->dense_lo; row < hi; ++row) {
-    const double s = (times[row] - (t_end - h)) / h;
-    const double s1 = 1.0 - s;
-    out[static_cast<size_t>(row) * n + i] = yi + s * (ydiff + s1 * (bspl + s * (rc3 + s1 * rc4)));
-  }
+  dopri5_dense_rows(c_dense5, st, ks, n, i, st->cur ? xb0 : xb1, st->cur ? xb1 : xb0,
+                    times, out, n);
 }
 
 // Final state out: x_out = the x0 buffer.

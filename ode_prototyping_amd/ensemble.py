@@ -85,10 +85,18 @@ class EnsembleResult:
         self.tune_ms = tune_ms
         self.x_final = x_final
         self.trace = trace
+        self.dense_times = None
+        self.dense_states = None  # n_dense x packed state (NaN where not reached)
 
     def states(self):
         """Final states as a list of per-system arrays."""
         return unpack(self.x_final, self.sizes)
+
+    def dense_rows(self, k):
+        """System k's dense-output rows (n_dense[k] x 6 n_k)."""
+        off = 6 * int(self.sizes[:k].sum())
+        n = 6 * int(self.sizes[k])
+        return self.dense_states[:int(self.n_dense[k]), off:off + n]
 
     def __repr__(self):
         return (f"EnsembleResult(systems={self.sizes.size}, rounds={self.rounds}, "
@@ -99,7 +107,7 @@ class EnsembleResult:
 def solve_ensemble(tableau, sizes, x0, t0, tf, atol, rtol, softening, masses=None, *,
                    variant="exact", exe=None, max_accepted=0, max_attempts=0, fsal=False,
                    rounds_per_launch=0, tile_config=ENS_TILE_AUTO, controller="reference",
-                   mlp=None, trace_cap=0, x_out=None, fetch=True):
+                   mlp=None, trace_cap=0, x_out=None, fetch=True, dense_times=None):
     """Solve every system of the ensemble from t0 to tf (device-resident).
 
     ``x0``: packed host array or HipState; ``masses``: packed (n_k per system)
@@ -107,7 +115,9 @@ def solve_ensemble(tableau, sizes, x0, t0, tf, atol, rtol, softening, masses=Non
     ``mlp``: a trained :class:`ode_prototyping_amd.controller.StepMLP` for
     ``controller="learned"``.  Returns an :class:`EnsembleResult`; the final
     packed state is in ``x_final`` (host) unless ``fetch`` is False (then it
-    stays in ``x_out``).
+    stays in ``x_out``).  ``dense_times`` (DOPRI5): common output times after
+    t0; every system's state there, from its continuous extension, is in
+    ``dense_states`` (packed rows) and ``dense_rows(k)``.
     """
     if tableau not in TABLEAUS:
         raise ValueError(f"unknown tableau {tableau}")
@@ -123,13 +133,22 @@ def solve_ensemble(tableau, sizes, x0, t0, tf, atol, rtol, softening, masses=Non
     out = x_out if x_out is not None else HipState(exe, nv)
     trace = np.zeros((sz.size, max(int(trace_cap), 1), 3))
     tune = np.zeros(ENS_TILE_CONFIGS)
+    dt_arr = dense = None
+    if dense_times is not None and len(dense_times) > 0:
+        dt_arr = np.ascontiguousarray(dense_times, dtype=np.float64)
+        if dt_arr.size * nv * 8 > (16 << 30):
+            raise ValueError("the dense output would exceed 16 GiB")
+        dense = np.zeros((dt_arr.size, nv))
     opt = _lib.EnsembleOptions(float(t0), float(tf), float(atol), float(rtol),
                                int(max_accepted), int(max_attempts), 1 if fsal else 0,
                                int(rounds_per_launch), int(tile_config),
                                CONTROLLERS[controller],
                                mlp.handle if mlp is not None else None,
                                trace.ctypes.data if trace_cap > 0 else None, int(trace_cap),
-                               tune.ctypes.data)
+                               tune.ctypes.data,
+                               dt_arr.ctypes.data if dt_arr is not None else None,
+                               int(dt_arr.size) if dt_arr is not None else 0,
+                               dense.ctypes.data if dense is not None else None)
     agg = _lib.EnsembleStats()
     per = (_lib.SystemStats * sz.size)()
     exe.check(lib().odehip_ensemble_solve(exe.handle, TABLEAUS[tableau], int(sz.size), szp,
@@ -141,7 +160,11 @@ def solve_ensemble(tableau, sizes, x0, t0, tf, atol, rtol, softening, masses=Non
     tr = None
     if trace_cap > 0:
         tr = [trace[k, :per[k].n_trace].copy() for k in range(sz.size)]
-    return EnsembleResult(sz, list(per), agg, x_final, tr, tune)
+    res = EnsembleResult(sz, list(per), agg, x_final, tr, tune)
+    if dt_arr is not None:
+        res.dense_times = dt_arr
+        res.dense_states = dense
+    return res
 
 
 def ensemble_rhs(exe, sizes, x, softening, masses=None, variant="exact",

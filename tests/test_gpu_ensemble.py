@@ -224,3 +224,42 @@ def test_ensemble_rejects_bad_arguments(exe):
     with pytest.raises(OdeHipError):
         solve_ensemble("DOPRI5", [4], np.zeros(24), 0.0, 1.0, 1e-8, 1e-8, 1e-3,
                        controller="learned", exe=exe)
+
+
+def test_ensemble_dense_output_matches_single_system(exe):
+    """DOPRI5 dense output of every system at common times: each system's
+    rows equal the single-system solve's dense rows bit for bit (systems of
+    different sizes finish in different rounds, so the running-system lists
+    are rebuilt during the solve); rows a system does not reach are NaN."""
+    import ode_prototyping_amd as oh
+    from ode_prototyping_amd.ensemble import pack, solve_ensemble
+    sizes = [20, 90, 300, 33, 150, 7]
+    xs, ms = [], []
+    for k, n in enumerate(sizes):
+        x, m = oh.scenarios.uniform_cube(n, seed=60 + k)
+        xs.append(x)
+        ms.append(m)
+    tf = 0.02
+    times = np.linspace(0.0, tf, 11)[1:]  # the last (tf) is never written
+    res = solve_ensemble("DOPRI5", sizes, pack(xs), 0.0, tf, 1e-10, 1e-10, 1e-3, pack(ms),
+                         exe=exe, dense_times=times)
+    assert res.n_unfinished == 0 and np.all(res.n_dense == 9)
+    for k, n in enumerate(sizes):
+        one = oh.integrate("DOPRI5", xs[k], 0.0, tf, 1e-10, 1e-10,
+                           oh.NBodyODE(1e-3, ms[k], "exact"), exe=exe, dense_times=times)
+        assert one.n_dense == 9
+        assert np.array_equal(bits(res.dense_rows(k)), bits(one.dense_states))
+        assert np.array_equal(bits(res.states()[k]), bits(one.x_final))
+    assert np.all(np.isnan(res.dense_states[9]))
+    # an accepted-step budget stops systems at different times
+    b = solve_ensemble("DOPRI5", sizes, pack(xs), 0.0, tf, 1e-10, 1e-10, 1e-3, pack(ms),
+                       exe=exe, dense_times=times, max_accepted=6)
+    for k in range(len(sizes)):
+        nk = int(np.sum(times < b.t[k]))
+        assert b.n_dense[k] == nk
+        rows = b.dense_states[:, 6 * sum(sizes[:k]):6 * sum(sizes[:k + 1])]
+        assert not np.any(np.isnan(rows[:nk])) and np.all(np.isnan(rows[nk:]))
+        assert np.array_equal(bits(rows[:nk]), bits(res.dense_rows(k)[:nk]))
+    with pytest.raises(oh.core.OdeHipError, match="DOPRI5"):
+        solve_ensemble("DOP853", sizes, pack(xs), 0.0, tf, 1e-10, 1e-10, 1e-3, pack(ms),
+                       exe=exe, dense_times=times)

@@ -160,6 +160,20 @@ def ensemble_block(oh, exe):
             drift.append(abs((e1 - e0) / e0))
         block[name]["energy_rel_drift_64_systems"] = {"median": float(np.median(drift)),
                                                       "max": float(np.max(drift))}
+    # DOPRI5 dense output of every system at 20 common times (fast FP64 RHS)
+    times = np.linspace(0.0, tf, 21)[1:]
+    exe.synchronize()
+    t = time.perf_counter()
+    r = solve_ensemble(tab, sizes, x0d, 0.0, tf, tol, tol, eps, md, variant="fast", exe=exe,
+                       x_out=out_state, fetch=False, dense_times=times)
+    exe.synchronize()
+    secs = time.perf_counter() - t
+    block["fast"]["dense_output"] = {"seconds": secs,
+                                     "overhead": secs / block["fast"]["seconds"] - 1.0,
+                                     "rows_per_system": int(np.min(r.n_dense)),
+                                     "bytes_to_host": int(r.dense_states.nbytes)}
+    log(f"ensemble fast + dense output: {block['fast']['dense_output']}")
+    del r
     net.close()
     block["cpu_baseline"] = ensemble_cpu_baseline(sizes, x0, m, eps, tol, tab)
     log(f"ensemble cpu baseline: {block['cpu_baseline']}")
@@ -228,6 +242,61 @@ def controller_block(oh, exe):
     return block
 
 
+def outputs_block(oh, exe, x0_dev, atol, rtol, ode, n, eps, m, accepted, plain_secs, t_end):
+    """The output paths and the leapfrog integrator on the headline workload:
+    every accepted state kept (RawOutput), DOP853 dense output at 100 times,
+    the .bin trajectory write, and fixed-step leapfrog on the same RHS."""
+    import tempfile
+    tab = "DOP853"
+    block = {"workload": f"{n}-body Plummer sphere, eps={eps}, {tab}, rtol=atol=1e-10, "
+                         f"{accepted} accepted steps (plain FP64 solve: {plain_secs:.3f} s)"}
+
+    def timed(**kw):
+        exe.synchronize()
+        t = time.perf_counter()
+        r = oh.integrate(tab, x0_dev, 0.0, 100.0, atol, rtol, ode, exe=exe,
+                         max_accepted=accepted, **kw)
+        exe.synchronize()
+        return r, time.perf_counter() - t
+
+    raw = oh.RawOutput()
+    r, secs = timed(output=raw)
+    block["raw_output"] = {"seconds": secs, "overhead": secs / plain_secs - 1.0,
+                           "states": len(raw.times()),
+                           "bytes_to_host": len(raw.times()) * 6 * n * 8}
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "trajectory.bin")
+        t = time.perf_counter()
+        oh.nbody_io.write_output(path, raw, softening=eps)
+        wsecs = time.perf_counter() - t
+        size = os.path.getsize(path)
+    block["trajectory_file"] = {"seconds": wsecs, "bytes": size, "MB_per_s": size / wsecs / 1e6}
+    del raw
+    times = np.linspace(0.0, t_end, 102)[1:-1]
+    r, secs = timed(dense_times=times, fsal=True)
+    block["dense_output"] = {"seconds": secs, "overhead": secs / plain_secs - 1.0,
+                             "n_dense": r.n_dense, "dense_steps": r.dense_steps,
+                             "dense_rhs_evals": r.dense_rhs_evals,
+                             "rhs_evals": r.rhs_evals, "accepted": r.accepted}
+    # leapfrog: 100 fixed steps of dt = 1e-3 on the same RHS kernel
+    dt, steps = 1e-3, 100
+    e0 = nbody_energy(oh, exe, x0_dev, n, eps, m)
+    exe.synchronize()
+    t = time.perf_counter()
+    x_lf, lf = oh.leapfrog(x0_dev, 0.0, steps * dt - 0.5 * dt, dt, ode, exe=exe)
+    exe.synchronize()
+    secs = time.perf_counter() - t
+    e1 = nbody_energy(oh, exe, x_lf, n, eps, m)
+    block["leapfrog"] = {"dt": dt, "steps": lf["steps"], "t": lf["t"], "seconds": secs,
+                         "particle_steps_per_s": n * lf["steps"] / secs,
+                         "ms_per_step": 1000 * secs / lf["steps"],
+                         "rhs_evals": lf["rhs_evals"], "energy_rel_err": abs((e1 - e0) / e0)}
+    for k, v in block.items():
+        if k != "workload":
+            log(f"outputs {k}: {v}")
+    return block
+
+
 def precision_block(oh, exe):
     """Mixed precision on 65536 colliding Plummer bodies, DOP853."""
     n, eps, tol, tab, acc = 65536, 1e-2, 1e-10, "DOP853", 10
@@ -272,6 +341,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-controller", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-outputs", action="store_true")
     ap.add_argument("--profile-ensemble", action="store_true",
                     help="only run short ensemble solves (2048 systems, t in [0, 0.005]) in "
                          "each RHS variant (the workload of the committed ensemble profiles)")
@@ -436,6 +506,9 @@ def main():
                                        "final_state_rel_err_vs_plain": rel}
         log(f"chosen: {extras['chosen_mixed_fsal']}")
         out["variants"] = extras
+    if not args.no_outputs:
+        out["outputs"] = outputs_block(oh, exe, x0_dev, atol, rtol, ode, n, softening, m,
+                                       args.accepted, wall / steps, r.t)
 
     if not args.no_ensemble:
         out["ensemble"] = ensemble_block(oh, exe)

@@ -390,12 +390,14 @@ typedef struct {
   double* tune_ms;       /* host, ODEHIP_ENS_TILE_CONFIGS entries: ms per
                             right-hand side of every candidate (autotune
                             only; untried entries 0); may be NULL */
-  /* DOPRI5 dense output of every system at common times, as
+  /* Dense output (DOPRI5 or DOP853) of every system at common times, as
    * odehip_solve_options.dense_times: dense_times host, n_dense strictly
    * increasing times after t0; dense_states host, n_dense x (total state
    * length), each row packed like x0.  A system's row r is written when one
    * of its accepted steps covers dense_times[r]; rows it does not reach
-   * (odehip_system_stats.n_dense) are NaN.  No extra RHS evaluations. */
+   * (odehip_system_stats.n_dense) are NaN.  DOPRI5 adds no RHS evaluation;
+   * DOP853 adds, for the systems with a covering step, the single-system
+   * pass's evaluations. */
   const double* dense_times;
   long n_dense;
   double* dense_states;

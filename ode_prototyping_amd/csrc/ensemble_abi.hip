@@ -393,8 +393,8 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
   const long n_dense = opt->n_dense;
   if (n_dense < 0) return fail(ctx, ODEHIP_ERR_INVALID, "negative n_dense");
   if (n_dense > 0) {
-    if (tableau != ODEHIP_TABLEAU_DOPRI5)
-      return fail(ctx, ODEHIP_ERR_INVALID, "ensemble dense output needs DOPRI5");
+    if (tableau != ODEHIP_TABLEAU_DOPRI5 && tableau != ODEHIP_TABLEAU_DOP853)
+      return fail(ctx, ODEHIP_ERR_INVALID, "ensemble dense output needs DOPRI5 or DOP853");
     if (!opt->dense_times || !opt->dense_states)
       return fail(ctx, ODEHIP_ERR_INVALID, "dense_times / dense_states null");
     double prev = opt->t0;

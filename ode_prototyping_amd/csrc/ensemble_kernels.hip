@@ -30,6 +30,7 @@ __constant__ Tableau e_tab[ODEHIP_NUM_TABLEAUS] = {
     kTableaus[3], kTableaus[4], kTableaus[5]};
 
 __constant__ Dopri5Dense e_dense5 = kDopri5Dense;
+__constant__ Dop853Dense e_dense8 = kDop853Dense;
 
 constexpr int kBlock = ODEHIP_REDUCE_GROUP;
 
@@ -297,12 +298,14 @@ __global__ __launch_bounds__(64 * WPB) void k_ens_rhs(EnsBufs b,
   const EnsTile tile = b.tiles[slot];
   if (tile.sys < 0) return;
   const SolverState* st = b.st ? b.st + tile.sys : nullptr;
-  if (st && st->done) return;
+  // stage0 == 2: an evaluation of the dense-output pass, for the systems
+  // whose pass runs this round (finished or not)
+  if (st && (stage0 == 2 ? st->dense_go == 0 : st->done)) return;
   const EnsSys sy = b.sys[tile.sys];
   const int n = sy.n;
   const int tpl = n >= b.tpl2_min_n ? 2 : 1;
   double* k = k_out + sy.eoff;
-  if (st && stage0 && st->k1_valid) {
+  if (st && stage0 == 1 && st->k1_valid) {
     // k1 of this attempt is the FSAL copy (or kept after a rejection)
     if (st->k1_valid == 1) {
       const int lane = threadIdx.x & 63;
@@ -443,15 +446,33 @@ __global__ __launch_bounds__(64) void k_ens_controller_learned(EnsBufs b) {
   }
 }
 
-// DOPRI5 dense output of every system whose attempt of this round was
-// accepted and covered an output time (StepperDopr5::prepare_dense /
+// Dense output of every system whose attempt of this round was accepted and
+// covered an output time (StepperDopr5 / StepperDopr853 prepare_dense,
 // dense_out and DenseOutput::save, as the single-system pass): the gate, one
-// thread per system, then the rows, one workgroup per reduction group.  A
-// system that finished in this round still writes its last step's rows.
-__global__ void k_ens_dense_begin(EnsBufs b) {
+// thread per system; for DOP853 f at the new state and the three extra
+// stages, evaluated for those systems only; then the rows, one workgroup per
+// reduction group.  A system that finished in this round still writes its
+// last step's rows.
+__global__ void k_ens_dense_begin(EnsBufs b, int extra_rhs) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= b.n_sys) return;
-  dense_gate(b.st + s, b.dense_t, b.n_dense, 0);
+  dense_gate(b.st + s, b.dense_t, b.n_dense, extra_rhs);
+}
+
+__global__ __launch_bounds__(kBlock) void k_ens_dense_stage8(EnsBufs b, int x) {
+  const GroupPos p = group_pos(b);
+  const SolverState* st = b.st + p.sys;
+  if (!st->dense_go || !p.valid) return;
+  b.ytmp[p.e] = dop853_dense_stage(e_dense8, x, b.ks, b.kfsal, b.nv, p.e,
+                                   st->cur ? b.xb0 : b.xb1, st->dt_acc);
+}
+
+__global__ __launch_bounds__(kBlock) void k_ens_dense_out8(EnsBufs b) {
+  const GroupPos p = group_pos(b);
+  const SolverState* st = b.st + p.sys;
+  if (!st->dense_go || !p.valid) return;
+  dop853_dense_rows(e_dense8, st, b.ks, b.kfsal, b.nv, p.e, st->cur ? b.xb0 : b.xb1,
+                    st->cur ? b.xb1 : b.xb0, b.dense_t, b.dense_out, b.nv);
 }
 
 __global__ __launch_bounds__(kBlock) void k_ens_dense_out5(EnsBufs b) {
@@ -632,9 +653,17 @@ hipError_t ens_enqueue_round(const EnsBufs& b, hipStream_t s) {
   } else {
     hipLaunchKernelGGL(k_ens_controller, dim3(sys_grid(b)), dim3(64), 0, s, b);
   }
-  if (b.n_dense > 0) {
-    hipLaunchKernelGGL(k_ens_dense_begin, dim3(sys_grid(b)), dim3(64), 0, s, b);
+  if (b.n_dense > 0 && b.tableau == ODEHIP_TABLEAU_DOPRI5) {
+    hipLaunchKernelGGL(k_ens_dense_begin, dim3(sys_grid(b)), dim3(64), 0, s, b, 0);
     hipLaunchKernelGGL(k_ens_dense_out5, dim3(groups_grid(b)), dim3(kBlock), 0, s, b);
+  } else if (b.n_dense > 0) {
+    hipLaunchKernelGGL(k_ens_dense_begin, dim3(sys_grid(b)), dim3(64), 0, s, b, 4);
+    if ((e = ens_launch_rhs(b, nullptr, b.kfsal, 2, s)) != hipSuccess) return e;
+    for (int x = 0; x < 3; ++x) {
+      hipLaunchKernelGGL(k_ens_dense_stage8, dim3(groups_grid(b)), dim3(kBlock), 0, s, b, x);
+      if ((e = ens_launch_rhs(b, b.ytmp, b.ks + (x + 1) * b.nv, 2, s)) != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_ens_dense_out8, dim3(groups_grid(b)), dim3(kBlock), 0, s, b);
   }
   return hipGetLastError();
 }

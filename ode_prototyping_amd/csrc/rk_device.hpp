@@ -300,6 +300,57 @@ __device__ inline void dense_gate(SolverState* st, const double* times, long n_d
   st->dense_go = go;
 }
 
+// DOP853's stage vector j of a dense pass: 0..11 the accepted attempt's
+// stages (nv apart in ks), 12 = f at the step's end (k13), 13..15 the extra
+// stages, kept in the slots of stages 1..3, which the extension never reads.
+__device__ inline const double* dop853_dense_vec(const double* ks, const double* k13, int j,
+                                                 size_t nv) {
+  if (j < 12) return ks + static_cast<size_t>(j) * nv;
+  if (j == 12) return k13;
+  return ks + static_cast<size_t>(j - 12) * nv;
+}
+
+// Element e of the input of extra stage 13 + x (StepperDopr853.hpp:239-262):
+// y + h (w_0 k_{i0} + ... + w_7 k_{i7}), the sum left to right.
+__device__ inline double dop853_dense_stage(const Dop853Dense& D, int x, const double* ks,
+                                            const double* k13, size_t nv, size_t e,
+                                            const double* y, double h) {
+  double acc = D.extra_w[x][0] * dop853_dense_vec(ks, k13, D.extra_idx[x][0], nv)[e];
+  for (int m = 1; m < 8; ++m)
+    acc = acc + D.extra_w[x][m] * dop853_dense_vec(ks, k13, D.extra_idx[x][m], nv)[e];
+  return y[e] + h * acc;
+}
+
+// StepperDopr853::prepare_dense / dense_out (StepperDopr853.hpp:214-237,
+// 263-289) for element e: the eight polynomial coefficients from 14 stage
+// vectors, then out[row * row_stride + e] for the pass's rows.
+__device__ inline void dop853_dense_rows(const Dop853Dense& D, const SolverState* st,
+                                         const double* ks, const double* k13, size_t nv,
+                                         size_t e, const double* y, const double* yout,
+                                         const double* times, double* out,
+                                         size_t row_stride) {
+  const double h = st->dt_acc;
+  const double yi = y[e];
+  const double ydiff = yout[e] - yi;
+  const double bspl = h * ks[e] - ydiff;
+  const double rc3 = ydiff - h * k13[e] - bspl;
+  double rc[4];
+  for (int r = 0; r < 4; ++r) {
+    double acc = D.d_w[r][0] * dop853_dense_vec(ks, k13, D.d_idx[0], nv)[e];
+    for (int m = 1; m < 12; ++m)
+      acc = acc + D.d_w[r][m] * dop853_dense_vec(ks, k13, D.d_idx[m], nv)[e];
+    rc[r] = h * acc;
+  }
+  const double t_end = st->t;
+  const long hi = st->dense_hi;
+  for (long row = st->dense_lo; row < hi; ++row) {
+    const double s = (times[row] - (t_end - h)) / h;
+    const double s1 = 1.0 - s;
+    out[static_cast<size_t>(row) * row_stride + e] =
+        yi + s * (ydiff + s1 * (bspl + s * (rc3 + s1 * (rc[0] + s * (rc[1] + s1 * (rc[2] + s * rc[3]))))));
+  }
+}
+
 // StepperDopr5::prepare_dense / dense_out (StepperDopr5.hpp:195-221) for
 // element e of a state whose stage vectors are nv apart in ks (stage 7 = f at
 // the new state): the step went from y to yout over st->dt_acc ending at

@@ -263,16 +263,6 @@ __global__ void k_save_state(const SolverState* st, const double* xb0,
 // the buffer st->cur does not name to the one it names, over st->dt_acc,
 // ending at st->t.
 
-// Stage vector j of the pass: 0..11 the accepted attempt's stages, 12 = f at
-// the step's end (k13), 13..15 the extra stages, kept in the slots of stages
-// 1..3, which the extension never reads.
-__device__ inline const double* dense_vec(const double* ks, const double* k13, int j,
-                                          size_t n) {
-  if (j < 12) return ks + j * n;
-  if (j == 12) return k13;
-  return ks + (j - 12) * n;
-}
-
 // The gate of the dense pass (one thread).
 __global__ void k_dense_begin(SolverState* st, const double* times, long n_dense,
                               int extra_rhs) {
@@ -280,22 +270,17 @@ __global__ void k_dense_begin(SolverState* st, const double* times, long n_dense
   dense_gate(st, times, n_dense, extra_rhs);
 }
 
-// The input of extra stage 13 + e (StepperDopr853.hpp:239-262).
-__global__ void k_dense_stage(const SolverState* st, int e, double* ytmp,
+// The input of extra stage 13 + x (StepperDopr853.hpp:239-262).
+__global__ void k_dense_stage(const SolverState* st, int x, double* ytmp,
                               const double* ks, const double* k13,
                               const double* xb0, const double* xb1, size_t n) {
   if (!st->dense_go) return;
   const size_t i = gid();
   if (i >= n) return;
-  const Dop853Dense& D = c_dense;
-  const double* y = st->cur ? xb0 : xb1;
-  double acc = D.extra_w[e][0] * dense_vec(ks, k13, D.extra_idx[e][0], n)[i];
-  for (int m = 1; m < 8; ++m) acc = acc + D.extra_w[e][m] * dense_vec(ks, k13, D.extra_idx[e][m], n)[i];
-  ytmp[i] = y[i] + st->dt_acc * acc;
+  ytmp[i] = dop853_dense_stage(c_dense, x, ks, k13, n, i, st->cur ? xb0 : xb1, st->dt_acc);
 }
 
-// The eight polynomial coefficients of element i (StepperDopr853.hpp:214-237,
-// 263-276) and the rows of this pass (dense_out, :278-289).
+// The eight polynomial coefficients of element i and the rows of this pass.
 __global__ __launch_bounds__(kBlock) void k_dense_out(
     const SolverState* st, const double* ks, const double* k13,
     const double* xb0, const double* xb1, const double* times, double* out,
@@ -303,28 +288,8 @@ __global__ __launch_bounds__(kBlock) void k_dense_out(
   if (!st->dense_go) return;
   const size_t i = gid();
   if (i >= n) return;
-  const Dop853Dense& D = c_dense;
-  const double* y = st->cur ? xb0 : xb1;
-  const double* yout = st->cur ? xb1 : xb0;
-  const double h = st->dt_acc;
-  const double yi = y[i];
-  const double ydiff = yout[i] - yi;
-  const double bspl = h * ks[i] - ydiff;
-  const double rc3 = ydiff - h * k13[i] - bspl;
-  double rc[4];
-  for (int r = 0; r < 4; ++r) {
-    double acc = D.d_w[r][0] * dense_vec(ks, k13, D.d_idx[0], n)[i];
-    for (int m = 1; m < 12; ++m) acc = acc + D.d_w[r][m] * dense_vec(ks, k13, D.d_idx[m], n)[i];
-    rc[r] = h * acc;
-  }
-  const double t_end = st->t;
-  const long hi = st->dense_hi;
-  for (long row = st->dense_lo; row < hi; ++row) {
-    const double s = (times[row] - (t_end - h)) / h;
-    const double s1 = 1.0 - s;
-    out[static_cast<size_t>(row) * n + i] =
-        yi + s * (ydiff + s1 * (bspl + s * (rc3 + s1 * (rc[0] + s * (rc[1] + s1 * (rc[2] + s * rc[3]))))));
-  }
+  dop853_dense_rows(c_dense, st, ks, k13, n, i, st->cur ? xb0 : xb1, st->cur ? xb1 : xb0,
+                    times, out, n);
 }
 
 // DOPRI5: the five coefficients of element i from stages 1, 3..7 -- stage 7

@@ -115,7 +115,7 @@ def solve_ensemble(tableau, sizes, x0, t0, tf, atol, rtol, softening, masses=Non
     ``mlp``: a trained :class:`ode_prototyping_amd.controller.StepMLP` for
     ``controller="learned"``.  Returns an :class:`EnsembleResult`; the final
     packed state is in ``x_final`` (host) unless ``fetch`` is False (then it
-    stays in ``x_out``).  ``dense_times`` (DOPRI5): common output times after
+    stays in ``x_out``).  ``dense_times`` (DOPRI5, DOP853): common output times after
     t0; every system's state there, from its continuous extension, is in
     ``dense_states`` (packed rows) and ``dense_rows(k)``.
     """

@@ -226,11 +226,14 @@ def test_ensemble_rejects_bad_arguments(exe):
                        controller="learned", exe=exe)
 
 
-def test_ensemble_dense_output_matches_single_system(exe):
-    """DOPRI5 dense output of every system at common times: each system's
-    rows equal the single-system solve's dense rows bit for bit (systems of
-    different sizes finish in different rounds, so the running-system lists
-    are rebuilt during the solve); rows a system does not reach are NaN."""
+@pytest.mark.parametrize("tableau,fsal", [("DOPRI5", False), ("DOP853", False),
+                                          ("DOP853", True)])
+def test_ensemble_dense_output_matches_single_system(exe, tableau, fsal):
+    """Dense output of every system at common times: each system's rows equal
+    the single-system solve's dense rows bit for bit (systems of different
+    sizes finish in different rounds, so the running-system lists are
+    rebuilt during the solve), and so do its final state and counters; rows
+    a system does not reach are NaN."""
     import ode_prototyping_amd as oh
     from ode_prototyping_amd.ensemble import pack, solve_ensemble
     sizes = [20, 90, 300, 33, 150, 7]
@@ -241,25 +244,28 @@ def test_ensemble_dense_output_matches_single_system(exe):
         ms.append(m)
     tf = 0.02
     times = np.linspace(0.0, tf, 11)[1:]  # the last (tf) is never written
-    res = solve_ensemble("DOPRI5", sizes, pack(xs), 0.0, tf, 1e-10, 1e-10, 1e-3, pack(ms),
-                         exe=exe, dense_times=times)
+    res = solve_ensemble(tableau, sizes, pack(xs), 0.0, tf, 1e-10, 1e-10, 1e-3, pack(ms),
+                         exe=exe, dense_times=times, fsal=fsal)
     assert res.n_unfinished == 0 and np.all(res.n_dense == 9)
     for k, n in enumerate(sizes):
-        one = oh.integrate("DOPRI5", xs[k], 0.0, tf, 1e-10, 1e-10,
-                           oh.NBodyODE(1e-3, ms[k], "exact"), exe=exe, dense_times=times)
+        one = oh.integrate(tableau, xs[k], 0.0, tf, 1e-10, 1e-10,
+                           oh.NBodyODE(1e-3, ms[k], "exact"), exe=exe, dense_times=times,
+                           fsal=fsal)
         assert one.n_dense == 9
         assert np.array_equal(bits(res.dense_rows(k)), bits(one.dense_states))
         assert np.array_equal(bits(res.states()[k]), bits(one.x_final))
+        assert (res.accepted[k], res.rejected[k], res.rhs_evals[k]) == (
+            one.accepted, one.rejected, one.rhs_evals)
     assert np.all(np.isnan(res.dense_states[9]))
     # an accepted-step budget stops systems at different times
-    b = solve_ensemble("DOPRI5", sizes, pack(xs), 0.0, tf, 1e-10, 1e-10, 1e-3, pack(ms),
-                       exe=exe, dense_times=times, max_accepted=6)
+    b = solve_ensemble(tableau, sizes, pack(xs), 0.0, tf, 1e-10, 1e-10, 1e-3, pack(ms),
+                       exe=exe, dense_times=times, max_accepted=6, fsal=fsal)
     for k in range(len(sizes)):
         nk = int(np.sum(times < b.t[k]))
         assert b.n_dense[k] == nk
         rows = b.dense_states[:, 6 * sum(sizes[:k]):6 * sum(sizes[:k + 1])]
         assert not np.any(np.isnan(rows[:nk])) and np.all(np.isnan(rows[nk:]))
         assert np.array_equal(bits(rows[:nk]), bits(res.dense_rows(k)[:nk]))
-    with pytest.raises(oh.core.OdeHipError, match="DOPRI5"):
-        solve_ensemble("DOP853", sizes, pack(xs), 0.0, tf, 1e-10, 1e-10, 1e-3, pack(ms),
+    with pytest.raises(oh.core.OdeHipError, match="DOPRI5 or DOP853"):
+        solve_ensemble("RKF78", sizes, pack(xs), 0.0, tf, 1e-10, 1e-10, 1e-3, pack(ms),
                        exe=exe, dense_times=times)

@@ -1,6 +1,7 @@
 """Summarise a rocprofv3 kernel trace as a text table.
 
-Usage: python profiles/make_summary.py <run_results.db | kernel_trace.csv> "<command>"
+Usage: python -m ode_prototyping_amd.profile_summary <run_results.db | kernel_trace.csv> \
+       "<command>"
 Prints per-kernel dispatch count, average / min / max duration and share, and
 the average over the launches that did most work (>= 50 % of the longest:
 the launches of a solve in which every system is still running -- a finished

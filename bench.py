@@ -341,7 +341,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-controller", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
-    ap.add_argument("--no-outputs", action="store_true")
+    ap.add_argument("--no-outputs", action="store_true",
+                    help="skip the output-path block (also skipped by --no-extras)")
     ap.add_argument("--profile-ensemble", action="store_true",
                     help="only run short ensemble solves (2048 systems, t in [0, 0.005]) in "
                          "each RHS variant (the workload of the committed ensemble profiles)")
@@ -506,7 +507,7 @@ def main():
                                        "final_state_rel_err_vs_plain": rel}
         log(f"chosen: {extras['chosen_mixed_fsal']}")
         out["variants"] = extras
-    if not args.no_outputs:
+    if not args.no_extras and not args.no_outputs:
         out["outputs"] = outputs_block(oh, exe, x0_dev, atol, rtol, ode, n, softening, m,
                                        args.accepted, wall / steps, r.t)
 

@@ -351,7 +351,7 @@ def main():
                          "(the workload of the committed counter profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): this build "
+        print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): this benchmark "
               "measures the single-GPU path; multi-GPU sharding is design-only", file=sys.stderr)
         sys.exit(2)
 

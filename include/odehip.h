@@ -301,8 +301,9 @@ int odehip_nbody_momentum(odehip_context* ctx, const double* x, size_t n,
 
 /* ------------------------------------------------- learned step controller
  * A small network proposes the next step ratio (no reference counterpart;
- * the reference's controller is RKEmbeddedParallel.hpp:64-79, which stays
- * the judge: a step is accepted exactly when its error norm is <= 1; the
+ * the reference's controller is RKEmbeddedParallel.hpp:64-79, whose test
+ * still decides acceptance: a step is accepted exactly when its error norm
+ * is <= 1; the
  * proposed ratio is capped at 1 right after a rejection, the reference's
  * clamps and clip to tf apply, and the PI controller replaces the network
  * before three attempts of history exist, after two rejections in a row, or

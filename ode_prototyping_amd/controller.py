@@ -5,7 +5,7 @@ The reference's controller (``RKEmbeddedParallel.hpp:64-79``) sizes the next
 step as ``dt (0.38 / err)^(1/k)``, i.e. it assumes the error constant of the
 last step holds for the next.  The learned controller replaces that proposal
 with a network that sees the last three attempts (error norms, step sizes,
-accept flags) and the system size; the embedded error estimate still judges
+accept flags) and the system size; the embedded error estimate still decides on
 every step and the PI controller takes over before three attempts of history
 exist, after two rejections in a row and on a non-finite output (see
 ``include/odehip.h``).

@@ -84,7 +84,7 @@ def _small_net(exe):
     return StepMLP(exe, np.zeros(10, np.float32), np.ones(10, np.float32), params=p)
 
 
-def test_learned_controller_keeps_the_judge(exe):
+def test_learned_controller_keeps_the_acceptance_test(exe):
     """A network that always proposes dt * 1.3: the error norm still decides
     acceptance, the PI controller sizes attempts 1-3 and after two
     rejections in a row, and the run reaches tf."""

@@ -12,6 +12,13 @@ struct EnsembleWorkspace;
 void destroy_ensemble_workspace(EnsembleWorkspace* w);
 }  // namespace odehip
 
+struct odehip_context;
+namespace odehip {
+// Grows the context's scratch buffer to at least n doubles (kept across
+// calls; synchronises the stream before replacing it).
+int ensure_scratch(odehip_context* ctx, size_t n);
+}  // namespace odehip
+
 struct GraphKey {
   int tableau = -1, rhs_kind = -1, variant = -1, k = 0, fsal = -1;
   size_t n_var = 0;

@@ -59,9 +59,12 @@ extern "C" int odehip_leapfrog(odehip_context* ctx, const odehip_ode* ode,
   const size_t n_var = ode->n_var;
   const size_t n_dof = n_var / 2;
   const long keep = std::min<long>(states_cap, n_steps + 1);
-  double* buf = nullptr;
+  // the context's scratch buffer (kept across calls: nothing is allocated
+  // per call once it is large enough)
   const size_t need = 2 * n_var + (states ? static_cast<size_t>(keep) * n_var : 0);
-  HIP_TRY(ctx, hipMalloc(&buf, need * sizeof(double)));
+  int rc = ensure_scratch(ctx, need);
+  if (rc) return rc;
+  double* buf = ctx->scratch;
   double* k0 = buf;           // [vel | acc] of the current positions
   double* k1 = buf + n_var;   // of the new positions
   double* sv = buf + 2 * n_var;
@@ -98,7 +101,6 @@ extern "C" int odehip_leapfrog(odehip_context* ctx, const odehip_ode* ode,
     e = hipMemcpyAsync(states, sv, static_cast<size_t>(keep) * n_var * sizeof(double),
                        hipMemcpyDeviceToHost, s);
   hipError_t se = hipStreamSynchronize(s);
-  (void)hipFree(buf);
   if (e != hipSuccess) return hip_fail(ctx, e, "leapfrog");
   if (se != hipSuccess) return hip_fail(ctx, se, "leapfrog sync");
   if (times)

@@ -21,9 +21,7 @@
 
 using namespace odehip;
 
-namespace {
-
-int ensure_scratch(odehip_context* ctx, size_t n) {
+int odehip::ensure_scratch(odehip_context* ctx, size_t n) {
   if (ctx->scratch_cap >= n) return ODEHIP_OK;
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   if (ctx->scratch) HIP_TRY(ctx, hipFree(ctx->scratch));
@@ -34,6 +32,8 @@ int ensure_scratch(odehip_context* ctx, size_t n) {
   ctx->scratch_cap = cap;
   return ODEHIP_OK;
 }
+
+namespace {
 
 int ensure_ws(odehip_context* ctx, size_t n) {
   if (ctx->ws_cap >= n) return ODEHIP_OK;

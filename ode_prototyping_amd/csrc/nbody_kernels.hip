@@ -88,6 +88,144 @@ constexpr int kExactWaves = 16;
 constexpr int kExactJW = 3;  // partners per wave per tile (48-partner tiles, 144 KB LDS)
 constexpr int kExactTJ = kExactWaves * kExactJW;
 
+// The kExactJW partners one wave evaluates in one tile (wave-uniform, held
+// in SGPRs).
+struct PartnerSet {
+  double x[kExactJW], y[kExactJW], z[kExactJW], m[kExactJW];
+};
+
+// Partners jb .. jb + kExactJW - 1, all < n.
+template <bool kMass>
+__device__ __forceinline__ void load_partner_set(PartnerSet& p, uniform_ptr yu,
+                                                 uniform_ptr massu, int jb) {
+  const uniform_ptr pp = yu + 3 * jb;
+#pragma unroll
+  for (int jj = 0; jj < kExactJW; ++jj) {
+    p.x[jj] = pp[3 * jj];
+    p.y[jj] = pp[3 * jj + 1];
+    p.z[jj] = pp[3 * jj + 2];
+    p.m[jj] = kMass ? massu[jb + jj] : 1.0;
+  }
+}
+
+// The same with indices past the end clamped to n - 1: their terms are
+// computed but never folded.
+template <bool kMass>
+__device__ __forceinline__ void load_partner_set_clamped(PartnerSet& p, uniform_ptr yu,
+                                                         uniform_ptr massu, int jb, int n) {
+  if (jb + kExactJW <= n) {
+    load_partner_set<kMass>(p, yu, massu, jb);
+    return;
+  }
+#pragma unroll
+  for (int jj = 0; jj < kExactJW; ++jj) {
+    const int j = min(jb + jj, n - 1);
+    p.x[jj] = yu[3 * j];
+    p.y[jj] = yu[3 * j + 1];
+    p.z[jj] = yu[3 * j + 2];
+    p.m[jj] = kMass ? massu[j] : 1.0;
+  }
+}
+
+// This wave's kExactJW terms of one tile for its 64 targets, stored to
+// dst[jj][coordinate][lane].
+template <bool kMass, bool kShared>
+__device__ __forceinline__ void exact_tile_terms(const PartnerSet& p, double xi, double yi,
+                                                 double zi, double softening_sq,
+                                                 double (*dst)[3][64], int lane) {
+  double ax[kExactJW], ay[kExactJW], az[kExactJW];
+#pragma unroll
+  for (int jj = 0; jj < kExactJW; ++jj) {
+    const double dx = p.x[jj] - xi, dy = p.y[jj] - yi, dz = p.z[jj] - zi;
+    if (kShared)
+      exact_term_shared_rcp<kMass>(dx, dy, dz, p.m[jj], softening_sq, ax[jj], ay[jj], az[jj]);
+    else
+      exact_term_ieee<kMass>(dx, dy, dz, p.m[jj], softening_sq, ax[jj], ay[jj], az[jj]);
+  }
+#pragma unroll
+  for (int jj = 0; jj < kExactJW; ++jj) {
+    dst[jj][0][lane] = ax[jj];
+    dst[jj][1][lane] = ay[jj];
+    dst[jj][2][lane] = az[jj];
+  }
+}
+
+// Adds the terms of partners base .. base + cnt - 1 (one coordinate; src
+// points at partner 0's entry for this lane) to acc in partner order,
+// skipping the target's own index.  The skip adds +0 instead of the term:
+// acc starts at +0 and a round-to-nearest sum is -0 only if both operands
+// are, so acc is never -0 and acc + +0 is acc bit for bit -- no per-lane
+// branch.
+__device__ __forceinline__ double exact_fold(double acc, const double* src, int base, int cnt,
+                                             int i0, int i) {
+  if (cnt == kExactTJ && (base + kExactTJ <= i0 || base >= i0 + 64)) {
+#pragma unroll
+    for (int m = 0; m < kExactTJ; ++m) acc += src[m * 192];
+  } else {
+    for (int m = 0; m < cnt; ++m) {
+      const double v = src[m * 192];
+      acc += base + m != i ? v : 0.0;
+    }
+  }
+  return acc;
+}
+
+// All tiles of one workgroup; returns the folded sum of the calling lane
+// (waves 0-2: coordinate = wave).  Partner data is loaded one tile ahead
+// (scalar loads hidden behind the current tile's arithmetic).  Tiles whose
+// partners and next partners are all in range run in a loop unrolled by two
+// over two partner sets, with no bounds tests and no register copies: the
+// scalar instructions per tile stay few, since the scalar unit is shared by
+// the 16 waves that all leave the same barrier together.
+template <bool kMass, bool kShared>
+__device__ __forceinline__ double exact_tiles(double (*terms)[kExactTJ][3][64], uniform_ptr yu,
+                                              uniform_ptr massu, int n, double softening_sq,
+                                              double xi, double yi, double zi, int wave, int lane,
+                                              int i0, int i) {
+  const int n_tiles = (n + kExactTJ - 1) / kExactTJ;
+  const int n_full = n / kExactTJ;
+  const int jw = wave * kExactJW;
+  double acc = 0.0;
+  PartnerSet pa, pb;
+  // tile 0: nothing to fold yet
+  load_partner_set_clamped<kMass>(pa, yu, massu, jw, n);
+  if (n_tiles > 1) load_partner_set_clamped<kMass>(pb, yu, massu, kExactTJ + jw, n);
+  exact_tile_terms<kMass, kShared>(pa, xi, yi, zi, softening_sq, terms[0] + jw, lane);
+  __syncthreads();
+  int t = 1;
+  // Steady state: tiles t - 1 .. t + 2 are all full; tile t's partners are
+  // in pb.  Tile t is odd, so its terms go to buffer 1 and tile t + 1's to
+  // buffer 0.
+  for (; t + 2 < n_full; t += 2) {
+    if (wave < 3) acc = exact_fold(acc, &terms[0][0][wave][lane], (t - 1) * kExactTJ,
+                                   kExactTJ, i0, i);
+    load_partner_set<kMass>(pa, yu, massu, (t + 1) * kExactTJ + jw);
+    exact_tile_terms<kMass, kShared>(pb, xi, yi, zi, softening_sq, terms[1] + jw, lane);
+    __syncthreads();
+    if (wave < 3) acc = exact_fold(acc, &terms[1][0][wave][lane], t * kExactTJ, kExactTJ,
+                                   i0, i);
+    load_partner_set<kMass>(pb, yu, massu, (t + 2) * kExactTJ + jw);
+    exact_tile_terms<kMass, kShared>(pa, xi, yi, zi, softening_sq, terms[0] + jw, lane);
+    __syncthreads();
+  }
+  // The remaining tiles (partial last tile included), then the last fold.
+  for (; t <= n_tiles; ++t) {
+    if (wave < 3) {
+      const int base = (t - 1) * kExactTJ;
+      acc = exact_fold(acc, &terms[(t - 1) & 1][0][wave][lane], base, min(kExactTJ, n - base),
+                       i0, i);
+    }
+    if (t < n_tiles) {
+      pa = pb;
+      if (t + 1 < n_tiles)
+        load_partner_set_clamped<kMass>(pb, yu, massu, (t + 1) * kExactTJ + jw, n);
+      exact_tile_terms<kMass, kShared>(pa, xi, yi, zi, softening_sq, terms[t & 1] + jw, lane);
+    }
+    __syncthreads();
+  }
+  return acc;
+}
+
 template <bool kMass, bool kSharedRcp>
 __global__ __launch_bounds__(64 * kExactWaves) void k_nbody_exact(NBodyArgs a) {
   if (prologue_skip(a)) return;
@@ -97,7 +235,6 @@ __global__ __launch_bounds__(64 * kExactWaves) void k_nbody_exact(NBodyArgs a) {
   double* __restrict__ k = a.k;
   const uniform_ptr massu = as_uniform(a.mass);
   const int n = a.n;
-  const double softening_sq = a.eps2;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i0 = blockIdx.x * 64;
@@ -106,87 +243,14 @@ __global__ __launch_bounds__(64 * kExactWaves) void k_nbody_exact(NBodyArgs a) {
   const double xi = valid_i ? y[3 * i] : 0.0;
   const double yi = valid_i ? y[3 * i + 1] : 0.0;
   const double zi = valid_i ? y[3 * i + 2] : 0.0;
-  double acc = 0.0;
-  const int n_tiles = (n + kExactTJ - 1) / kExactTJ;
-  // Partner coordinates of the next tile are loaded one tile ahead (scalar
-  // loads), so their latency hides behind the current tile's arithmetic.
-  double px[kExactJW], py[kExactJW], pz[kExactJW], pm[kExactJW];
   // the shared-reciprocal form is used when every position coordinate passes
   // coord_shared_ok (one pass over the 3N coordinates per workgroup)
   const bool shared_ok = kSharedRcp && all_positions_shared_ok(y, n);
-  auto load_partners = [&](int t) {
-    const int jb = t * kExactTJ + wave * kExactJW;
-    if (jb + kExactJW <= n) {
-      const uniform_ptr pp = yu + 3 * jb;
-#pragma unroll
-      for (int jj = 0; jj < kExactJW; ++jj) {
-        px[jj] = pp[3 * jj];
-        py[jj] = pp[3 * jj + 1];
-        pz[jj] = pp[3 * jj + 2];
-        pm[jj] = kMass ? massu[jb + jj] : 1.0;
-      }
-    } else {
-#pragma unroll
-      for (int jj = 0; jj < kExactJW; ++jj) {
-        // partners past the end are clamped to a valid index: their terms
-        // are computed but never folded
-        const int j = min(jb + jj, n - 1);
-        px[jj] = yu[3 * j];
-        py[jj] = yu[3 * j + 1];
-        pz[jj] = yu[3 * j + 2];
-        pm[jj] = kMass ? massu[j] : 1.0;
-      }
-    }
-  };
-  load_partners(0);
-  for (int t = 0; t <= n_tiles; ++t) {
-    // Fold the previous tile first: the in-order adds of the three folding
-    // waves then overlap the other waves' arithmetic on the same SIMD.
-    if (t > 0 && wave < 3) {
-      const int base = (t - 1) * kExactTJ;
-      const int cnt = min(kExactTJ, n - base);
-      const double* src = &terms[(t - 1) & 1][0][wave][lane];
-      if (cnt == kExactTJ && (base + kExactTJ <= i0 || base >= i0 + 64)) {
-#pragma unroll
-        for (int m = 0; m < kExactTJ; ++m) acc += src[m * 192];
-      } else {
-        for (int m = 0; m < cnt; ++m) {
-          const double v = src[m * 192];
-          if (base + m != i) acc += v;
-        }
-      }
-    }
-    if (t < n_tiles) {
-      double dx[kExactJW], dy[kExactJW], dz[kExactJW], mj[kExactJW];
-#pragma unroll
-      for (int jj = 0; jj < kExactJW; ++jj) {
-        dx[jj] = px[jj] - xi;
-        dy[jj] = py[jj] - yi;
-        dz[jj] = pz[jj] - zi;
-        mj[jj] = pm[jj];
-      }
-      if (t + 1 < n_tiles) load_partners(t + 1);
-      double ax[kExactJW], ay[kExactJW], az[kExactJW];
-      if (kSharedRcp && shared_ok) {
-#pragma unroll
-        for (int jj = 0; jj < kExactJW; ++jj)
-          exact_term_shared_rcp<kMass>(dx[jj], dy[jj], dz[jj], mj[jj], softening_sq,
-                                       ax[jj], ay[jj], az[jj]);
-      } else {
-#pragma unroll
-        for (int jj = 0; jj < kExactJW; ++jj)
-          exact_term_ieee<kMass>(dx[jj], dy[jj], dz[jj], mj[jj], softening_sq, ax[jj],
-                                 ay[jj], az[jj]);
-      }
-#pragma unroll
-      for (int jj = 0; jj < kExactJW; ++jj) {
-        terms[t & 1][wave * kExactJW + jj][0][lane] = ax[jj];
-        terms[t & 1][wave * kExactJW + jj][1][lane] = ay[jj];
-        terms[t & 1][wave * kExactJW + jj][2][lane] = az[jj];
-      }
-    }
-    __syncthreads();
-  }
+  double acc;
+  if (kSharedRcp && shared_ok)
+    acc = exact_tiles<kMass, true>(terms, yu, massu, n, a.eps2, xi, yi, zi, wave, lane, i0, i);
+  else
+    acc = exact_tiles<kMass, false>(terms, yu, massu, n, a.eps2, xi, yi, zi, wave, lane, i0, i);
   if (wave < 3 && valid_i) k[3L * n + 3 * i + wave] = acc;
   copy_velocities(y, k, n, i0);
 }

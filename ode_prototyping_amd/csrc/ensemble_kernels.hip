@@ -74,14 +74,83 @@ __device__ inline void tile_store(double* __restrict__ k, int n, int i0,
 
 // exact: every target sums its partners' terms in increasing j, skipping
 // j == i (NBodyODE.hpp:89-98 with the order of the single-threaded loop).
-// Terms of U partners are computed together; the shared-reciprocal form is
-// used when all of the system's coordinates pass coord_shared_ok.
-template <bool kMass, bool kSharedRcp, int TPL>
-__device__ inline void exact_tile(const double* __restrict__ y,
-                                  const double* __restrict__ mass,
-                                  double* __restrict__ k, int n, int i0,
-                                  double eps2) {
+// Partners j0 .. j1 - 1 in groups of U computed together; kSelf: the range
+// may hold this wave's own targets, whose term is then not added.
+template <bool kMass, bool kShared, bool kSelf, int TPL>
+__device__ __forceinline__ void exact_sweep(uniform_ptr yu, uniform_ptr mu, int j0, int j1,
+                                            const int* ic, const double* xi, const double* yi,
+                                            const double* zi, double eps2, double* ax,
+                                            double* ay, double* az) {
   constexpr int U = TPL == 1 ? 4 : 2;
+  double px[U], py[U], pz[U], pm[U];
+  auto load = [&](int j) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      px[u] = yu[3 * (j + u)];
+      py[u] = yu[3 * (j + u) + 1];
+      pz[u] = yu[3 * (j + u) + 2];
+      pm[u] = kMass ? mu[j + u] : 1.0;
+    }
+  };
+  int j = j0;
+  if (j + U <= j1) load(j);
+  for (; j + U <= j1; j += U) {
+    double cx[U], cy[U], cz[U], cm[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      cx[u] = px[u];
+      cy[u] = py[u];
+      cz[u] = pz[u];
+      cm[u] = pm[u];
+    }
+    if (j + 2 * U <= j1) load(j + U);  // next partners, one group ahead
+    double tx[U][TPL], ty[U][TPL], tz[U][TPL];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int t = 0; t < TPL; ++t) {
+        const double dx = cx[u] - xi[t], dy = cy[u] - yi[t], dz = cz[u] - zi[t];
+        if (kShared)
+          exact_term_shared_rcp<kMass>(dx, dy, dz, cm[u], eps2, tx[u][t], ty[u][t], tz[u][t]);
+        else
+          exact_term_ieee<kMass>(dx, dy, dz, cm[u], eps2, tx[u][t], ty[u][t], tz[u][t]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int t = 0; t < TPL; ++t) {
+        if (kSelf && j + u == ic[t]) continue;
+        ax[t] += tx[u][t];
+        ay[t] += ty[u][t];
+        az[t] += tz[u][t];
+      }
+    }
+  }
+  for (; j < j1; ++j) {
+    const double cx = yu[3 * j], cy = yu[3 * j + 1], cz = yu[3 * j + 2];
+    const double cm = kMass ? mu[j] : 1.0;
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+      double tx, ty, tz;
+      exact_term_ieee<kMass>(cx - xi[t], cy - yi[t], cz - zi[t], cm, eps2, tx, ty, tz);
+      if (kSelf && j == ic[t]) continue;
+      ax[t] += tx;
+      ay[t] += ty;
+      az[t] += tz;
+    }
+  }
+}
+
+// The shared-reciprocal form is used when all of the system's coordinates
+// pass coord_shared_ok.  Partners are swept in three ranges in order: before
+// the wave's targets, the targets' own range (the only one that needs the
+// j == i test) and after it.
+template <bool kMass, bool kSharedRcp, int TPL>
+__device__ __forceinline__ void exact_tile(const double* __restrict__ y,
+                                           const double* __restrict__ mass,
+                                           double* __restrict__ k, int n, int i0,
+                                           double eps2) {
   const int lane = threadIdx.x & 63;
   const uniform_ptr yu = as_uniform(y);
   const uniform_ptr mu = as_uniform(mass);
@@ -96,7 +165,6 @@ __device__ inline void exact_tile(const double* __restrict__ y,
     zi[t] = y[3 * ic[t] + 2];
     ax[t] = ay[t] = az[t] = 0.0;
   }
-  double px[U], py[U], pz[U], pm[U];
   // the shared-reciprocal form when every position coordinate of the system
   // passes coord_shared_ok (one pass of the wave over its 3n coordinates)
   bool shared_ok = kSharedRcp;
@@ -104,83 +172,21 @@ __device__ inline void exact_tile(const double* __restrict__ y,
     for (int e = lane; e < 3 * n; e += 64) shared_ok = shared_ok & coord_shared_ok(y[e]);
     shared_ok = __all(shared_ok);
   }
-  auto load = [&](int j) {
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      px[u] = yu[3 * (j + u)];
-      py[u] = yu[3 * (j + u) + 1];
-      pz[u] = yu[3 * (j + u) + 2];
-      pm[u] = kMass ? mu[j + u] : 1.0;
-    }
-  };
-  int j = 0;
-  if (U <= n) load(0);
-  for (; j + U <= n; j += U) {
-    double cx[U], cy[U], cz[U], cm[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      cx[u] = px[u];
-      cy[u] = py[u];
-      cz[u] = pz[u];
-      cm[u] = pm[u];
-    }
-    if (j + 2 * U <= n) load(j + U);  // next partners, one group ahead
-    double dx[U][TPL], dy[U][TPL], dz[U][TPL];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-#pragma unroll
-      for (int t = 0; t < TPL; ++t) {
-        dx[u][t] = cx[u] - xi[t];
-        dy[u][t] = cy[u] - yi[t];
-        dz[u][t] = cz[u] - zi[t];
-      }
-    }
-    double tx[U][TPL], ty[U][TPL], tz[U][TPL];
-    if (kSharedRcp && shared_ok) {
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-#pragma unroll
-        for (int t = 0; t < TPL; ++t)
-          exact_term_shared_rcp<kMass>(dx[u][t], dy[u][t], dz[u][t], cm[u], eps2,
-                                       tx[u][t], ty[u][t], tz[u][t]);
-    } else {
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-#pragma unroll
-        for (int t = 0; t < TPL; ++t)
-          exact_term_ieee<kMass>(dx[u][t], dy[u][t], dz[u][t], cm[u], eps2,
-                                 tx[u][t], ty[u][t], tz[u][t]);
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-#pragma unroll
-      for (int t = 0; t < TPL; ++t) {
-        const bool self = (j + u == ic[t]);
-        ax[t] = self ? ax[t] : ax[t] + tx[u][t];
-        ay[t] = self ? ay[t] : ay[t] + ty[u][t];
-        az[t] = self ? az[t] : az[t] + tz[u][t];
-      }
-    }
-  }
-  for (; j < n; ++j) {
-    const double cx = yu[3 * j], cy = yu[3 * j + 1], cz = yu[3 * j + 2];
-    const double cm = kMass ? mu[j] : 1.0;
-#pragma unroll
-    for (int t = 0; t < TPL; ++t) {
-      double tx, ty, tz;
-      exact_term_ieee<kMass>(cx - xi[t], cy - yi[t], cz - zi[t], cm, eps2, tx, ty,
-                             tz);
-      const bool self = (j == ic[t]);
-      ax[t] = self ? ax[t] : ax[t] + tx;
-      ay[t] = self ? ay[t] : ay[t] + ty;
-      az[t] = self ? az[t] : az[t] + tz;
-    }
+  // targets of this wave: i0 .. s1 - 1 (lanes past n are clamped to n - 1)
+  const int s1 = min(n, i0 + 64 * TPL);
+  if (kSharedRcp && shared_ok) {
+    exact_sweep<kMass, true, false, TPL>(yu, mu, 0, i0, ic, xi, yi, zi, eps2, ax, ay, az);
+    exact_sweep<kMass, true, true, TPL>(yu, mu, i0, s1, ic, xi, yi, zi, eps2, ax, ay, az);
+    exact_sweep<kMass, true, false, TPL>(yu, mu, s1, n, ic, xi, yi, zi, eps2, ax, ay, az);
+  } else {
+    exact_sweep<kMass, false, false, TPL>(yu, mu, 0, i0, ic, xi, yi, zi, eps2, ax, ay, az);
+    exact_sweep<kMass, false, true, TPL>(yu, mu, i0, s1, ic, xi, yi, zi, eps2, ax, ay, az);
+    exact_sweep<kMass, false, false, TPL>(yu, mu, s1, n, ic, xi, yi, zi, eps2, ax, ay, az);
   }
   tile_store<TPL>(k, n, i0, ax, ay, az);
   tile_velocities<TPL>(y, k, n, i0);
 }
 
-// fast: refined hardware rsqrt, FMA accumulation in increasing j.
 template <bool kMass, bool kGuard, int TPL>
 __device__ inline void fast_tile(const double* __restrict__ y,
                                  const double* __restrict__ mass,

@@ -257,24 +257,141 @@ __global__ __launch_bounds__(64 * kExactWaves) void k_nbody_exact(NBodyArgs a) {
 
 // The same exact sum with 32-target tiles: twice the workgroups, for N too
 // small to fill the chip with 64-target tiles.  Lane l evaluates target l % 32
-// against the kE32JW partners of slot l / 32 of its wave; the partner
-// coordinates of both slots arrive through scalar loads and each lane selects
-// its own.  Two waves fold a tile's terms in partner order: wave 0 the x
-// (lanes 0-31) and y (lanes 32-63) chains, wave 1 the z chains.
+// against the kE32JW partners of slot l / 32 of its wave; each lane fetches
+// its slot's partners with vector loads (the two half-waves read different
+// addresses, one tile ahead), so no per-lane selection between wave-uniform
+// values is needed.  Two waves fold a tile's terms in partner order: wave 0
+// the x (lanes 0-31) and y (lanes 32-63) chains, wave 1 the z chains.
 constexpr int kE32Waves = 16;
 constexpr int kE32JW = 2;                       // partners per slot per tile
 constexpr int kE32TJ = 2 * kE32JW * kE32Waves;  // partners per tile (96 KB LDS)
+
+// This lane's kE32JW partners (x, y, z, m), starting at index jb.
+typedef double E32Partners[kE32JW][4];
+
+// Partners jb .. jb + kE32JW - 1, all < n.
+template <bool kMass>
+__device__ __forceinline__ void e32_load(E32Partners& q, const double* __restrict__ y,
+                                         const double* __restrict__ mass, int jb) {
+#pragma unroll
+  for (int jj = 0; jj < kE32JW; ++jj) {
+    q[jj][0] = y[3 * (jb + jj)];
+    q[jj][1] = y[3 * (jb + jj) + 1];
+    q[jj][2] = y[3 * (jb + jj) + 2];
+    q[jj][3] = kMass ? mass[jb + jj] : 1.0;
+  }
+}
+
+// The same with indices past the end clamped to n - 1 (their terms are
+// computed but never folded).
+template <bool kMass>
+__device__ __forceinline__ void e32_load_clamped(E32Partners& q, const double* __restrict__ y,
+                                                 const double* __restrict__ mass, int jb,
+                                                 int n) {
+#pragma unroll
+  for (int jj = 0; jj < kE32JW; ++jj) {
+    const int j = min(jb + jj, n - 1);
+    q[jj][0] = y[3 * j];
+    q[jj][1] = y[3 * j + 1];
+    q[jj][2] = y[3 * j + 2];
+    q[jj][3] = kMass ? mass[j] : 1.0;
+  }
+}
+
+// This lane's terms of one tile, stored to dst[m0 + jj][coordinate][tgt].
+template <bool kMass, bool kShared>
+__device__ __forceinline__ void e32_terms(const E32Partners& q, double xi, double yi, double zi,
+                                          double softening_sq, double (*dst)[3][32], int m0,
+                                          int tgt) {
+  double ax[kE32JW], ay[kE32JW], az[kE32JW];
+#pragma unroll
+  for (int jj = 0; jj < kE32JW; ++jj) {
+    const double dx = q[jj][0] - xi, dy = q[jj][1] - yi, dz = q[jj][2] - zi;
+    if (kShared)
+      exact_term_shared_rcp<kMass>(dx, dy, dz, q[jj][3], softening_sq, ax[jj], ay[jj], az[jj]);
+    else
+      exact_term_ieee<kMass>(dx, dy, dz, q[jj][3], softening_sq, ax[jj], ay[jj], az[jj]);
+  }
+#pragma unroll
+  for (int jj = 0; jj < kE32JW; ++jj) {
+    dst[m0 + jj][0][tgt] = ax[jj];
+    dst[m0 + jj][1][tgt] = ay[jj];
+    dst[m0 + jj][2][tgt] = az[jj];
+  }
+}
+
+// exact_fold for the 32-target layout (partner stride 96 doubles).
+__device__ __forceinline__ double e32_fold(double acc, const double* src, int base, int cnt,
+                                           int i0, int i) {
+  if (cnt == kE32TJ && (base + kE32TJ <= i0 || base >= i0 + 32)) {
+#pragma unroll
+    for (int m = 0; m < kE32TJ; ++m) acc += src[m * 96];
+  } else {
+    for (int m = 0; m < cnt; ++m) {
+      const double v = src[m * 96];
+      acc += base + m != i ? v : 0.0;
+    }
+  }
+  return acc;
+}
+
+// All tiles of one workgroup (the structure of exact_tiles: peeled first
+// tile, a two-tile steady-state loop over two register partner sets with no
+// bounds tests, then the remaining tiles).  Returns the folded sum of the
+// calling lane (meaningful on the folding lanes).
+template <bool kMass, bool kShared>
+__device__ __forceinline__ double e32_tiles(double (*terms)[kE32TJ][3][32],
+                                            const double* __restrict__ y,
+                                            const double* __restrict__ mass, int n,
+                                            double softening_sq, double xi, double yi, double zi,
+                                            int wave, int slot, int tgt, int i0, int i, int fc) {
+  const int n_tiles = (n + kE32TJ - 1) / kE32TJ;
+  const int n_full = n / kE32TJ;
+  const int m0 = 2 * kE32JW * wave + slot * kE32JW;  // this lane's first partner in a tile
+  double acc = 0.0;
+  E32Partners qa, qb;
+  e32_load_clamped<kMass>(qa, y, mass, m0, n);
+  if (n_tiles > 1) e32_load_clamped<kMass>(qb, y, mass, kE32TJ + m0, n);
+  e32_terms<kMass, kShared>(qa, xi, yi, zi, softening_sq, terms[0], m0, tgt);
+  __syncthreads();
+  int t = 1;
+  for (; t + 2 < n_full; t += 2) {
+    if (wave < 2) acc = e32_fold(acc, &terms[0][0][fc][tgt], (t - 1) * kE32TJ, kE32TJ, i0, i);
+    e32_load<kMass>(qa, y, mass, (t + 1) * kE32TJ + m0);
+    e32_terms<kMass, kShared>(qb, xi, yi, zi, softening_sq, terms[1], m0, tgt);
+    __syncthreads();
+    if (wave < 2) acc = e32_fold(acc, &terms[1][0][fc][tgt], t * kE32TJ, kE32TJ, i0, i);
+    e32_load<kMass>(qb, y, mass, (t + 2) * kE32TJ + m0);
+    e32_terms<kMass, kShared>(qa, xi, yi, zi, softening_sq, terms[0], m0, tgt);
+    __syncthreads();
+  }
+  for (; t <= n_tiles; ++t) {
+    if (wave < 2) {
+      const int base = (t - 1) * kE32TJ;
+      acc = e32_fold(acc, &terms[(t - 1) & 1][0][fc][tgt], base, min(kE32TJ, n - base), i0, i);
+    }
+    if (t < n_tiles) {
+      E32Partners qc;
+#pragma unroll
+      for (int jj = 0; jj < kE32JW; ++jj)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) qc[jj][c] = qb[jj][c];
+      if (t + 1 < n_tiles) e32_load_clamped<kMass>(qb, y, mass, (t + 1) * kE32TJ + m0, n);
+      e32_terms<kMass, kShared>(qc, xi, yi, zi, softening_sq, terms[t & 1], m0, tgt);
+    }
+    __syncthreads();
+  }
+  return acc;
+}
 
 template <bool kMass, bool kSharedRcp>
 __global__ __launch_bounds__(64 * kE32Waves) void k_nbody_exact32(NBodyArgs a) {
   if (prologue_skip(a)) return;
   __shared__ double terms[2][kE32TJ][3][32];
   const double* __restrict__ y = select_input(a);
-  const uniform_ptr yu = as_uniform(y);
   double* __restrict__ k = a.k;
-  const uniform_ptr massu = as_uniform(a.mass);
+  const double* __restrict__ mass = a.mass;
   const int n = a.n;
-  const double softening_sq = a.eps2;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int tgt = lane & 31;
@@ -284,71 +401,17 @@ __global__ __launch_bounds__(64 * kE32Waves) void k_nbody_exact32(NBodyArgs a) {
   const int ic = i < n ? i : n - 1;
   const double xi = y[3 * ic], yi = y[3 * ic + 1], zi = y[3 * ic + 2];
   // fold role: wave 0 lanes 0-31 -> x, 32-63 -> y; wave 1 lanes 0-31 -> z
+  // (wave 1 lanes 32-63 repeat the z fold and discard it)
   const int fc = wave == 0 ? slot : 2;
   const bool folds = (wave == 0) || (wave == 1 && slot == 0);
-  double acc = 0.0;
-  const int n_tiles = (n + kE32TJ - 1) / kE32TJ;
-  // partner jj of slot s in this wave: tile index 2 kE32JW wave + s kE32JW + jj
-  double pa[kE32JW][4], pb[kE32JW][4];
   const bool shared_ok = kSharedRcp && all_positions_shared_ok(y, n);
-  auto load_partners = [&](int t) {
-#pragma unroll
-    for (int jj = 0; jj < kE32JW; ++jj) {
-      const int ja = min(t * kE32TJ + 2 * kE32JW * wave + jj, n - 1);
-      const int jb = min(t * kE32TJ + 2 * kE32JW * wave + kE32JW + jj, n - 1);
-      pa[jj][0] = yu[3 * ja];
-      pa[jj][1] = yu[3 * ja + 1];
-      pa[jj][2] = yu[3 * ja + 2];
-      pa[jj][3] = kMass ? massu[ja] : 1.0;
-      pb[jj][0] = yu[3 * jb];
-      pb[jj][1] = yu[3 * jb + 1];
-      pb[jj][2] = yu[3 * jb + 2];
-      pb[jj][3] = kMass ? massu[jb] : 1.0;
-    }
-  };
-  load_partners(0);
-  for (int t = 0; t <= n_tiles; ++t) {
-    if (t > 0 && wave < 2) {
-      const int base = (t - 1) * kE32TJ;
-      const int cnt = min(kE32TJ, n - base);
-      const double* src = &terms[(t - 1) & 1][0][fc][tgt];
-      if (folds) {
-        if (cnt == kE32TJ && (base + kE32TJ <= i0 || base >= i0 + 32)) {
-#pragma unroll
-          for (int m = 0; m < kE32TJ; ++m) acc += src[m * 96];
-        } else {
-          for (int m = 0; m < cnt; ++m) {
-            const double v = src[m * 96];
-            if (base + m != i) acc += v;
-          }
-        }
-      }
-    }
-    if (t < n_tiles) {
-      double dx[kE32JW], dy[kE32JW], dz[kE32JW], pm[kE32JW];
-#pragma unroll
-      for (int jj = 0; jj < kE32JW; ++jj) {
-        dx[jj] = (slot ? pb[jj][0] : pa[jj][0]) - xi;
-        dy[jj] = (slot ? pb[jj][1] : pa[jj][1]) - yi;
-        dz[jj] = (slot ? pb[jj][2] : pa[jj][2]) - zi;
-        pm[jj] = slot ? pb[jj][3] : pa[jj][3];
-      }
-      if (t + 1 < n_tiles) load_partners(t + 1);
-#pragma unroll
-      for (int jj = 0; jj < kE32JW; ++jj) {
-        double ax, ay, az;
-        if (kSharedRcp && shared_ok)
-          exact_term_shared_rcp<kMass>(dx[jj], dy[jj], dz[jj], pm[jj], softening_sq, ax, ay, az);
-        else
-          exact_term_ieee<kMass>(dx[jj], dy[jj], dz[jj], pm[jj], softening_sq, ax, ay, az);
-        const int mm = 2 * kE32JW * wave + slot * kE32JW + jj;
-        terms[t & 1][mm][0][tgt] = ax;
-        terms[t & 1][mm][1][tgt] = ay;
-        terms[t & 1][mm][2][tgt] = az;
-      }
-    }
-    __syncthreads();
-  }
+  double acc;
+  if (kSharedRcp && shared_ok)
+    acc = e32_tiles<kMass, true>(terms, y, mass, n, a.eps2, xi, yi, zi, wave, slot, tgt, i0, i,
+                                 fc);
+  else
+    acc = e32_tiles<kMass, false>(terms, y, mass, n, a.eps2, xi, yi, zi, wave, slot, tgt, i0, i,
+                                  fc);
   if (folds && i < n) k[3L * n + 3 * i + fc] = acc;
   // dxdt[0:3n] = x[3n:6n] for this tile's 32 particles
   const int e = threadIdx.x;

@@ -7,7 +7,8 @@
 // atomic-free gather: a workgroup owns a tile of 64 target particles (one per
 // lane) and streams the partner particles j; partner coordinates are
 // wave-uniform, so they arrive through scalar loads into SGPRs and cost no
-// VGPR bandwidth or LDS.
+// VGPR bandwidth or LDS (the 32-target exact kernel, whose half-waves take
+// different partners, uses vector loads instead).
 //
 //  * exact:  the reference's FP64 expression (((dx^2 + dy^2) + dz^2) + eps^2,
 //            correctly rounded sqrt, den = dist * dist_sq, three correctly

@@ -40,6 +40,27 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (and matrix) peak
 FLOP_PER_INTERACTION = 20
+# FP64 VALU wave-instructions per 64 directed interactions of the FP64 RHS
+# kernels (SQ_INSTS_VALU x 64 / N^2 at N=16384, profiles/nbody_rhs_n16384_valu_counters.txt)
+# and how many of them are quarter-rate transcendentals (v_rsq_f64, v_rcp_f64).
+VALU_PER_64_INTERACTIONS = {"exact": (41.04, 2), "fast": (18.02, 1)}
+SIMDS = 1024  # 256 CUs x 4 SIMDs
+
+
+def instruction_bound(variant, n, clock_mhz, kernel_ms):
+    """Issue-time bound of an FP64 RHS launch: every 64-lane FP64 instruction
+    occupies its SIMD for 4 cycles, a quarter-rate one for 16, at the clock the
+    live FMA probe measured."""
+    if variant not in VALU_PER_64_INTERACTIONS:
+        return None
+    total, quarter = VALU_PER_64_INTERACTIONS[variant]
+    cycles = (total - quarter) * 4 + quarter * 16
+    bound_ms = n * n / 64 / SIMDS * cycles / (clock_mhz * 1e3)
+    return {"valu_per_64_interactions": total, "quarter_rate": quarter,
+            "cycles_per_64_interactions": cycles, "bound_ms": bound_ms,
+            "frac": bound_ms / kernel_ms,
+            "source": "SQ_INSTS_VALU per interaction from the committed counter profile; "
+                      "4 cycles per FP64 wave64 instruction, 16 for v_rsq_f64 / v_rcp_f64"}
 
 
 def log(msg):
@@ -471,6 +492,7 @@ def main():
                      "kernel_ms": rhs_ms,
                      "sustained_fp64_tflops": sus_tflops, "clock_mhz": sus_mhz,
                      "frac_of_sustained": achieved / sus_tflops,
+                     "instruction_bound": instruction_bound(args.variant, n, sus_mhz, rhs_ms),
                      "algorithmic": f"{FLOP_PER_INTERACTION} FLOP x N(N-1) = "
                                     f"{FLOP_PER_INTERACTION * interactions:.4g} FLOP/launch"},
     }

@@ -89,7 +89,10 @@ typedef struct odehip_context odehip_context;
  * Replaces the CUDA executor object (src/par_exec/CudaExecutor.cuh:80-126),
  * which used the legacy default stream; a context owns a HIP stream and the
  * scratch the reductions and the device-resident controller need, so no call
- * allocates per step (CudaExecutor.cuh:103-107,121-122 does). */
+ * allocates per step (CudaExecutor.cuh:103-107,121-122 does).  A context
+ * belongs to one device: every call on it first makes that device current on
+ * the calling thread, and device buffers and networks passed to it must live
+ * on that device (a network from another device is rejected). */
 int odehip_create(int device, odehip_context** out);
 int odehip_destroy(odehip_context* ctx);
 const char* odehip_last_error(const odehip_context* ctx);

@@ -85,6 +85,23 @@ inline int hip_fail(odehip_context* ctx, hipError_t e, const char* where) {
     if (_e != hipSuccess) return hip_fail((ctx), _e, #expr); \
   } while (0)
 
+// Makes the context's device current on the calling thread: allocations,
+// copies and launches go to the current device, so a thread may hold contexts
+// on several devices.  A null context is left to the entry point's checks.
+inline int use_device(odehip_context* ctx) {
+  if (!ctx) return ODEHIP_OK;
+  int cur = -1;
+  if (hipGetDevice(&cur) == hipSuccess && cur == ctx->device) return ODEHIP_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  return ODEHIP_OK;
+}
+
+#define ODEHIP_USE_DEVICE(ctx)            \
+  do {                                    \
+    const int _rc = use_device(ctx);      \
+    if (_rc != ODEHIP_OK) return _rc;     \
+  } while (0)
+
 // Work stream for graph capture: the context's own stream, ordered after the
 // caller's stream when an external (possibly legacy default) stream is set.
 inline hipStream_t begin_work(odehip_context* ctx) {

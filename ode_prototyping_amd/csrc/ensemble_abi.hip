@@ -330,6 +330,7 @@ int odehip_ensemble_rhs(odehip_context* ctx, int n_systems,
                         double softening, const double* masses,
                         int tile_config, const double* x, double* dxdt,
                         double* ms_out) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx) return ODEHIP_ERR_INVALID;
   if (!x || !dxdt) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   int rc = check_common(ctx, nbody_variant, softening);
@@ -375,6 +376,7 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
                           const double* x0, double* x_out,
                           odehip_ensemble_stats* stats,
                           odehip_system_stats* per_system) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx) return ODEHIP_ERR_INVALID;
   if (!opt || !x0 || !x_out) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   if (tableau < 0 || tableau >= ODEHIP_NUM_TABLEAUS)
@@ -390,6 +392,8 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
     return fail(ctx, ODEHIP_ERR_INVALID, "bad controller");
   if (opt->controller == ODEHIP_CONTROLLER_LEARNED && !opt->mlp)
     return fail(ctx, ODEHIP_ERR_INVALID, "the learned controller needs a network (mlp)");
+  if (opt->controller == ODEHIP_CONTROLLER_LEARNED && mlp_device(opt->mlp) != ctx->device)
+    return fail(ctx, ODEHIP_ERR_INVALID, "the network lives on another device");
   const long n_dense = opt->n_dense;
   if (n_dense < 0) return fail(ctx, ODEHIP_ERR_INVALID, "negative n_dense");
   if (n_dense > 0) {

@@ -38,6 +38,7 @@ extern "C" int odehip_leapfrog(odehip_context* ctx, const odehip_ode* ode,
                                double t0, double tf, double dt, double* x,
                                double* states, double* times, long states_cap,
                                odehip_leapfrog_stats* stats) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx) return ODEHIP_ERR_INVALID;
   if (!ode || !x) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   if (ode->rhs_kind != ODEHIP_RHS_NBODY || ode->n_var == 0 || ode->n_var % 6)

@@ -162,6 +162,8 @@ __device__ inline void mlp_tile_forward(const MlpInfer* M, MlpTileLds& t) {
 
 // The published inference copy of a network handle (mlp_kernels.hip).
 const MlpInfer* mlp_infer_ptr(const odehip_mlp* m);
+// Device the network was created on (its buffers live there).
+int mlp_device(const odehip_mlp* m);
 
 // Stores the normalised features of row `row` (0..15) of a tile.
 __device__ inline void mlp_tile_set_row(const MlpInfer* M, MlpTileLds& t,

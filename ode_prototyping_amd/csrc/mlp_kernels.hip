@@ -37,6 +37,7 @@ struct odehip_mlp {
 namespace odehip {
 
 const MlpInfer* mlp_infer_ptr(const odehip_mlp* m) { return m ? m->infer : nullptr; }
+int mlp_device(const odehip_mlp* m) { return m ? m->device : -1; }
 
 namespace {
 
@@ -351,6 +352,7 @@ extern "C" {
 
 int odehip_mlp_create(odehip_context* ctx, const float* params, uint64_t seed,
                       const float* mean, const float* stdv, odehip_mlp** out) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx) return ODEHIP_ERR_INVALID;
   if (!out || !mean || !stdv) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   float inv[kMlpIn];
@@ -425,8 +427,11 @@ int odehip_mlp_destroy(odehip_mlp* m) {
 
 int odehip_mlp_get_params(odehip_context* ctx, const odehip_mlp* m,
                           float* params_out) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx) return ODEHIP_ERR_INVALID;
   if (!m || !params_out) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  if (m->device != ctx->device)
+    return fail(ctx, ODEHIP_ERR_INVALID, "the network lives on another device");
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   HIP_TRY(ctx, hipMemcpy(params_out, m->w, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
   return ODEHIP_OK;
@@ -435,8 +440,11 @@ int odehip_mlp_get_params(odehip_context* ctx, const odehip_mlp* m,
 int odehip_mlp_loss_grad(odehip_context* ctx, odehip_mlp* m, const float* x,
                          const float* t, long n, float* grad_out,
                          double* loss_out) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx) return ODEHIP_ERR_INVALID;
   if (!m || !x || !t || !grad_out) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  if (m->device != ctx->device)
+    return fail(ctx, ODEHIP_ERR_INVALID, "the network lives on another device");
   if (n <= 0 || n > (1L << 30)) return fail(ctx, ODEHIP_ERR_INVALID, "bad sample count");
   hipStream_t s = begin_work(ctx);
   HIP_TRY(ctx, hipMemsetAsync(m->loss_acc, 0, sizeof(double), s));
@@ -456,8 +464,11 @@ int odehip_mlp_loss_grad(odehip_context* ctx, odehip_mlp* m, const float* x,
 int odehip_mlp_train(odehip_context* ctx, odehip_mlp* m, const float* x,
                      const float* t, long n, const odehip_mlp_train_options* o,
                      double* epoch_loss) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx) return ODEHIP_ERR_INVALID;
   if (!m || !x || !t || !o) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  if (m->device != ctx->device)
+    return fail(ctx, ODEHIP_ERR_INVALID, "the network lives on another device");
   if (n <= 0 || n > (1L << 34)) return fail(ctx, ODEHIP_ERR_INVALID, "bad sample count");
   if (o->epochs < 0 || o->batch <= 0 || !(o->lr > 0.0f))
     return fail(ctx, ODEHIP_ERR_INVALID, "bad training options");
@@ -495,8 +506,11 @@ int odehip_mlp_train(odehip_context* ctx, odehip_mlp* m, const float* x,
 
 int odehip_mlp_infer(odehip_context* ctx, const odehip_mlp* m, const float* x,
                      long n, float* y) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx) return ODEHIP_ERR_INVALID;
   if (!m || !x || !y) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  if (m->device != ctx->device)
+    return fail(ctx, ODEHIP_ERR_INVALID, "the network lives on another device");
   if (n <= 0 || n > (1L << 34)) return fail(ctx, ODEHIP_ERR_INVALID, "bad sample count");
   hipStream_t s = begin_work(ctx);
   const long blocks = (n + 15) / 16;

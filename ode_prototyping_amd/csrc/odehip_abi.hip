@@ -196,6 +196,7 @@ int odehip_set_stream(odehip_context* ctx, void* hip_stream) {
 }
 
 int odehip_synchronize(odehip_context* ctx) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx) return ODEHIP_ERR_INVALID;
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->own));
@@ -203,12 +204,14 @@ int odehip_synchronize(odehip_context* ctx) {
 }
 
 int odehip_timer_start(odehip_context* ctx) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx) return ODEHIP_ERR_INVALID;
   HIP_TRY(ctx, hipEventRecord(ctx->t_start, ctx->stream));
   return ODEHIP_OK;
 }
 
 int odehip_timer_stop(odehip_context* ctx, double* elapsed_ms) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx || !elapsed_ms) return ODEHIP_ERR_INVALID;
   HIP_TRY(ctx, hipEventRecord(ctx->t_stop, ctx->stream));
   HIP_TRY(ctx, hipEventSynchronize(ctx->t_stop));
@@ -219,6 +222,7 @@ int odehip_timer_stop(odehip_context* ctx, double* elapsed_ms) {
 }
 
 int odehip_probe_fp64(odehip_context* ctx, double* tflops, double* clock_mhz) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx || !tflops || !clock_mhz) return ODEHIP_ERR_INVALID;
   const int blocks = 1024, threads = 1024, iters = 20000;
   const size_t need = static_cast<size_t>(blocks) * threads + 2;
@@ -249,6 +253,7 @@ int odehip_probe_fp64(odehip_context* ctx, double* tflops, double* clock_mhz) {
 
 // ------------------------------------------------------------------ memory --
 int odehip_alloc(odehip_context* ctx, size_t n, double** out) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx || !out) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   *out = nullptr;
   hipError_t e = hipMalloc(out, std::max<size_t>(n, 1) * sizeof(double));
@@ -260,6 +265,7 @@ int odehip_alloc(odehip_context* ctx, size_t n, double** out) {
 }
 
 int odehip_free(odehip_context* ctx, double* p) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx) return ODEHIP_ERR_INVALID;
   if (!p) return ODEHIP_OK;
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -268,6 +274,7 @@ int odehip_free(odehip_context* ctx, double* p) {
 }
 
 int odehip_copy_h2d(odehip_context* ctx, double* dst, const double* src, size_t n) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx || (n && (!dst || !src))) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   HIP_TRY(ctx, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -275,6 +282,7 @@ int odehip_copy_h2d(odehip_context* ctx, double* dst, const double* src, size_t 
 }
 
 int odehip_copy_d2h(odehip_context* ctx, double* dst, const double* src, size_t n) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx || (n && (!dst || !src))) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   HIP_TRY(ctx, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -282,12 +290,14 @@ int odehip_copy_d2h(odehip_context* ctx, double* dst, const double* src, size_t 
 }
 
 int odehip_copy_d2d(odehip_context* ctx, double* dst, const double* src, size_t n) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx || (n && (!dst || !src))) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   HIP_TRY(ctx, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
   return ODEHIP_OK;
 }
 
 int odehip_fill(odehip_context* ctx, double* dst, double v, size_t n) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx || (n && !dst)) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   HIP_TRY(ctx, launch_fill(dst, v, n, ctx->stream));
   return ODEHIP_OK;
@@ -297,6 +307,7 @@ int odehip_fill(odehip_context* ctx, double* dst, double v, size_t n) {
 int odehip_nbody_rhs(odehip_context* ctx, const double* x, double* dxdt,
                      size_t n, double softening, const double* masses,
                      int variant) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx || !x || !dxdt) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   if (n == 0 || n > (1u << 26)) return fail(ctx, ODEHIP_ERR_INVALID, "bad particle count");
   if (variant < 0 || variant > 2) return fail(ctx, ODEHIP_ERR_INVALID, "bad variant");
@@ -313,6 +324,7 @@ int odehip_nbody_rhs(odehip_context* ctx, const double* x, double* dxdt,
 int odehip_nbody_acc(odehip_context* ctx, const double* pos, double* acc,
                      size_t n, double softening, const double* masses,
                      int variant) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx || !pos || !acc) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   if (n == 0 || n > (1u << 26)) return fail(ctx, ODEHIP_ERR_INVALID, "bad particle count");
   // The RHS kernels read a full [pos | vel] state and write [vel | acc]; stage
@@ -331,6 +343,7 @@ int odehip_nbody_acc(odehip_context* ctx, const double* pos, double* acc,
 
 int odehip_ode_rhs(odehip_context* ctx, int kind, const double* x, double* dxdt,
                    size_t n_var) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx || !x || !dxdt) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   if (kind == ODEHIP_RHS_NBODY) {
     if (n_var % 6) return fail(ctx, ODEHIP_ERR_INVALID, "n_var % 6 != 0");
@@ -347,6 +360,7 @@ int odehip_ode_rhs(odehip_context* ctx, int kind, const double* x, double* dxdt,
 int odehip_rk_stage(odehip_context* ctx, int tableau, int stage, double dt,
                     double* temp, const double* ks, const double* x0,
                     size_t n_var) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx || !temp || !ks || !x0) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   if (!valid_tableau(tableau)) return fail(ctx, ODEHIP_ERR_INVALID, "bad tableau");
   if (stage < 1 || stage >= kTableaus[tableau].S)
@@ -359,6 +373,7 @@ int odehip_rk_update_state_and_error(odehip_context* ctx, int tableau,
                                      double dt, double* x, double* err,
                                      const double* ks, const double* x0,
                                      size_t n_var) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx || !x || !err || !ks || !x0) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   if (!valid_tableau(tableau)) return fail(ctx, ODEHIP_ERR_INVALID, "bad tableau");
   HIP_TRY(ctx, launch_rk_update_host_dt(tableau, dt, x, err, ks, x0, n_var, ctx->stream));
@@ -369,6 +384,7 @@ int odehip_rk_update_error_target(odehip_context* ctx, double* sc,
                                   const double* atol, const double* rtol,
                                   const double* x, const double* x0,
                                   size_t n_var) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx || !sc || !atol || !rtol || !x || !x0)
     return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   HIP_TRY(ctx, launch_rk_target(sc, atol, rtol, x, x0, n_var, ctx->stream));
@@ -377,6 +393,7 @@ int odehip_rk_update_error_target(odehip_context* ctx, double* sc,
 
 int odehip_rk_norm(odehip_context* ctx, const double* v, const double* sc,
                    size_t n_var, double* host_result) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx || !v || !sc || !host_result) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   if (n_var == 0) return fail(ctx, ODEHIP_ERR_INVALID, "n_var is 0");
   const int ng = n_groups_for(n_var);
@@ -393,6 +410,7 @@ int odehip_rk_norm(odehip_context* ctx, const double* v, const double* sc,
 int odehip_rk_compute_error_target(odehip_context* ctx, double* sc,
                                    const double* x0, const double* atol,
                                    const double* rtol, size_t n_var) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx || !sc || !x0 || !atol || !rtol) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   HIP_TRY(ctx, launch_init_target(sc, x0, atol, rtol, n_var, ctx->stream));
   return ODEHIP_OK;
@@ -400,6 +418,7 @@ int odehip_rk_compute_error_target(odehip_context* ctx, double* sc,
 
 int odehip_rk_euler_step(odehip_context* ctx, double* x1, const double* x0,
                          const double* f0, double dt0, size_t n_var) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx || !x1 || !x0 || !f0) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   HIP_TRY(ctx, launch_euler_host_dt(x1, x0, f0, dt0, n_var, ctx->stream));
   return ODEHIP_OK;
@@ -407,6 +426,7 @@ int odehip_rk_euler_step(odehip_context* ctx, double* x1, const double* x0,
 
 int odehip_rk_difference(odehip_context* ctx, double* df, const double* f0,
                          size_t n_var) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx || !df || !f0) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   HIP_TRY(ctx, launch_difference(df, f0, n_var, ctx->stream));
   return ODEHIP_OK;
@@ -416,6 +436,7 @@ int odehip_rk_initial_step(odehip_context* ctx, int tableau, int rhs_kind,
                            const double* x0, const double* atol,
                            const double* rtol, size_t n_var, double softening,
                            const double* masses, int variant, double* host_dt) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx || !x0 || !atol || !rtol || !host_dt)
     return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   if (!valid_tableau(tableau)) return fail(ctx, ODEHIP_ERR_INVALID, "bad tableau");
@@ -439,6 +460,7 @@ int odehip_rk_initial_step(odehip_context* ctx, int tableau, int rhs_kind,
 int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
                     const odehip_solve_options* opt, double* x,
                     odehip_solve_stats* stats) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx) return ODEHIP_ERR_INVALID;
   if (!opt || !x) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   if (!valid_tableau(tableau)) return fail(ctx, ODEHIP_ERR_INVALID, "bad tableau");
@@ -455,6 +477,8 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
       (!opt->mlp || ode->rhs_kind != ODEHIP_RHS_NBODY))
     return fail(ctx, ODEHIP_ERR_INVALID,
                 "the learned controller needs a network and an N-body problem");
+  if (opt->controller == ODEHIP_CONTROLLER_LEARNED && mlp_device(opt->mlp) != ctx->device)
+    return fail(ctx, ODEHIP_ERR_INVALID, "the network lives on another device");
   const long times_cap = opt->times ? opt->times_cap : 0;
   const long trace_cap = opt->trace ? opt->trace_cap : 0;
   const long states_cap = opt->states ? opt->states_cap : 0;
@@ -617,6 +641,7 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
 int odehip_nbody_energy(odehip_context* ctx, const double* x, size_t n,
                         double softening, const double* masses,
                         double* host_out) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx || !x || !host_out) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   if (n == 0) return fail(ctx, ODEHIP_ERR_INVALID, "n is 0");
   const size_t ng = static_cast<size_t>(n_groups_for(n));
@@ -638,6 +663,7 @@ int odehip_nbody_energy(odehip_context* ctx, const double* x, size_t n,
 
 int odehip_nbody_momentum(odehip_context* ctx, const double* x, size_t n,
                           const double* masses, double* host_out) {
+  ODEHIP_USE_DEVICE(ctx);
   if (!ctx || !x || !host_out) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   if (n == 0) return fail(ctx, ODEHIP_ERR_INVALID, "n is 0");
   const size_t ng = static_cast<size_t>(n_groups_for(n));

@@ -484,6 +484,8 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
   auto capture = [&](const EnsBufs& rb, int k) -> int {
     if (w->exec && w->graph_k == k && std::memcmp(&w->graph_bufs, &rb, sizeof rb) == 0)
       return ODEHIP_OK;
+    // a replay of the cached graph may still run (e.g. after a failed call)
+    HIP_TRY(ctx, hipStreamSynchronize(s));
     if (w->exec) (void)hipGraphExecDestroy(w->exec);
     if (w->graph) (void)hipGraphDestroy(w->graph);
     w->exec = nullptr;
@@ -493,8 +495,11 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
     for (int r = 0; r < k && ce == hipSuccess; ++r) ce = ens_enqueue_round(rb, s);
     hipGraph_t g = nullptr;
     hipError_t ee = hipStreamEndCapture(s, &g);
-    if (ce != hipSuccess) return hip_fail(ctx, ce, "capture round");
-    if (ee != hipSuccess) return hip_fail(ctx, ee, "hipStreamEndCapture");
+    if (ce != hipSuccess || ee != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return ce != hipSuccess ? hip_fail(ctx, ce, "capture round")
+                              : hip_fail(ctx, ee, "hipStreamEndCapture");
+    }
     w->graph = g;
     HIP_TRY(ctx, hipGraphInstantiate(&w->exec, g, nullptr, nullptr, 0));
     w->graph_bufs = rb;

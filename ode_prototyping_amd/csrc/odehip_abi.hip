@@ -547,6 +547,8 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
     key.caps[2] = states_cap;
     key.caps[3] = n_dense;
     if (!ctx->exec || !(key == ctx->key)) {
+      // a replay of the cached graph may still run (e.g. after a failed call)
+      HIP_TRY(ctx, hipStreamSynchronize(s));
       if (ctx->exec) {
         (void)hipGraphExecDestroy(ctx->exec);
         ctx->exec = nullptr;
@@ -560,8 +562,11 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
       for (int a = 0; a < K && ce == hipSuccess; ++a) ce = enqueue_attempt(b, s);
       hipGraph_t g = nullptr;
       hipError_t ee = hipStreamEndCapture(s, &g);
-      if (ce != hipSuccess) return hip_fail(ctx, ce, "capture attempt");
-      if (ee != hipSuccess) return hip_fail(ctx, ee, "hipStreamEndCapture");
+      if (ce != hipSuccess || ee != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return ce != hipSuccess ? hip_fail(ctx, ce, "capture attempt")
+                                : hip_fail(ctx, ee, "hipStreamEndCapture");
+      }
       ctx->graph = g;
       HIP_TRY(ctx, hipGraphInstantiate(&ctx->exec, g, nullptr, nullptr, 0));
       ctx->key = key;

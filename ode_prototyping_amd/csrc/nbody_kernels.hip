@@ -264,8 +264,8 @@ __global__ __launch_bounds__(64 * kExactWaves) void k_nbody_exact(NBodyArgs a) {
 // values is needed.  Two waves fold a tile's terms in partner order: wave 0
 // the x (lanes 0-31) and y (lanes 32-63) chains, wave 1 the z chains.
 constexpr int kE32Waves = 16;
-constexpr int kE32JW = 2;                       // partners per slot per tile
-constexpr int kE32TJ = 2 * kE32JW * kE32Waves;  // partners per tile (96 KB LDS)
+constexpr int kE32JW = 3;                       // partners per slot per tile
+constexpr int kE32TJ = 2 * kE32JW * kE32Waves;  // partners per tile (144 KB LDS)
 
 // This lane's kE32JW partners (x, y, z, m), starting at index jb.
 typedef double E32Partners[kE32JW][4];

@@ -72,6 +72,9 @@ constexpr EnsTileConfig kEnsTileConfigs[ODEHIP_ENS_TILE_CONFIGS] = {
     {4, 256, 0},        {4, 256, 1},        {1, kEnsNoTpl2, 0}, {1, kEnsNoTpl2, 1},
     {1, 1024, 0},       {1, 1024, 1},       {1, 256, 0},  {1, 256, 1}};
 constexpr int kEnsDefaultTileConfig = 1;
+// Waves that give each of the 1024 SIMDs (256 CUs x 4) four, the depth that
+// hides the RHS arithmetic's latency.
+constexpr long kEnsFillWaves = 4096;
 
 // y == null: each system's current x0 buffer (solve); else a packed state.
 // st == null in b: plain evaluation of every system.

@@ -281,8 +281,33 @@ bool tune_cache_hit(const EnsembleWorkspace* w, const Layout& L, int variant,
          std::memcmp(&w->tuned_eps2, &eps2, sizeof eps2) == 0 && w->tuned_sizes == L.sizes;
 }
 
+// Tiles that hold targets (empty padding tiles not counted): the waves a
+// full-ensemble RHS launch puts to work under `cfg`.
+long real_tiles(const Layout& L, const EnsTileConfig& cfg) {
+  long t = 0;
+  for (const auto& sy : L.sys) {
+    const int step = 64 * (sy.n >= cfg.tpl2_min_n ? 2 : 1);
+    t += (sy.n + step - 1) / step;
+  }
+  return t;
+}
+
+// The same configuration with 64-target tiles for every system.
+int without_tpl2(int cfg) {
+  for (int c = 0; c < ODEHIP_ENS_TILE_CONFIGS; ++c)
+    if (kEnsTileConfigs[c].wpb == kEnsTileConfigs[cfg].wpb &&
+        kEnsTileConfigs[c].xcd == kEnsTileConfigs[cfg].xcd &&
+        kEnsTileConfigs[c].tpl2_min_n == kEnsNoTpl2)
+      return c;
+  return cfg;
+}
+
 // Times one right-hand side of the state `y` into `k` for every candidate
-// configuration (one warm-up, best of two) and keeps the fastest.
+// configuration (one warm-up, best of two).  Among the candidates within 3 %
+// of the fastest it keeps the one with the most target tiles (then the
+// fastest of those): on the full ensemble 128-target tiles often tie with
+// 64-target ones, but they halve the waves once compaction leaves a few
+// large systems running.
 int autotune(odehip_context* ctx, EnsembleWorkspace* w, const Plan& p,
              const Layout& L, EnsBufs b, const double* y, double* k,
              hipStream_t s) {
@@ -307,6 +332,15 @@ int autotune(odehip_context* ctx, EnsembleWorkspace* w, const Plan& p,
     w->tuned_ms[c] = ms_best;
     if (ms_best < best) {
       best = ms_best;
+      best_cfg = c;
+    }
+  }
+  long most = -1;
+  for (int c = 0; c < ODEHIP_ENS_TILE_CONFIGS; ++c) {
+    if (!(w->tuned_ms[c] <= 1.03 * best)) continue;
+    const long t = real_tiles(L, kEnsTileConfigs[c]);
+    if (t > most || (t == most && w->tuned_ms[c] < w->tuned_ms[best_cfg])) {
+      most = t;
       best_cfg = c;
     }
   }
@@ -507,6 +541,7 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
     return ODEHIP_OK;
   };
   EnsBufs rb = b;
+  int run_cfg = cfg;  // the tile shape of the running systems (cfg until compaction)
   int K = rounds_per_replay(L.work);
   if ((rc = capture(rb, K))) return rc;
 
@@ -545,7 +580,17 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
             act.push_back(L.groups[L.sys[k].gbase + g]);
         }
         if (n_live == 0) break;
-        const std::vector<EnsTile> tiles = make_tiles(L, kEnsTileConfigs[cfg], &live);
+        // once the running systems' tiles no longer give every SIMD a wave,
+        // split their 128-target tiles (same results, twice the waves)
+        std::vector<EnsTile> tiles = make_tiles(L, kEnsTileConfigs[run_cfg], &live);
+        if (kEnsTileConfigs[run_cfg].tpl2_min_n != kEnsNoTpl2) {
+          long real = 0;
+          for (const EnsTile& t : tiles) real += t.sys >= 0 ? 1 : 0;
+          if (real < kEnsFillWaves) {
+            run_cfg = without_tpl2(run_cfg);
+            tiles = make_tiles(L, kEnsTileConfigs[run_cfg], &live);
+          }
+        }
         HIP_TRY(ctx, hipMemcpy(w->dev + p.groups_act, act.data(), act.size() * sizeof(int2),
                                hipMemcpyHostToDevice));
         HIP_TRY(ctx, hipMemcpy(w->dev + p.tiles, tiles.data(), tiles.size() * sizeof(EnsTile),
@@ -553,7 +598,7 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
         w->table_cfg = -2;  // the tile region now holds the running systems' tiles
         rb.groups = reinterpret_cast<const int2*>(w->dev + p.groups_act);
         rb.n_groups = static_cast<int>(act.size());
-        set_tile_cfg(&rb, cfg, static_cast<int>(tiles.size()));
+        set_tile_cfg(&rb, run_cfg, static_cast<int>(tiles.size()));
         K = rounds_per_replay(sum_n2);
         if ((rc = capture(rb, K))) return rc;
         compact_below = n_live / 2;

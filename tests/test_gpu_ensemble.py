@@ -101,8 +101,11 @@ def assert_system_matches(res, k, ref, x_final, trace=True):
         assert np.array_equal(tr[:, 2].astype(int), ref.trace_acc)
 
 
+# tile configurations: auto; 64-target tiles; 128-target tiles from 1024 bodies
+# (none here); 128-target tiles from 256 bodies (the 257-body system), which
+# compaction splits into 64-target ones once systems finish
 @pytest.mark.parametrize("tableau,tf,cfg", [("DOPRI5", 0.05, -1), ("DOP853", 0.03, 7),
-                                            ("RKF45", 0.02, 3)])
+                                            ("RKF45", 0.02, 3), ("DOPRI5", 0.05, 11)])
 def test_ensemble_solve_matches_oracle(exe, ens, tableau, tf, cfg):
     from ode_prototyping_amd.ensemble import pack, solve_ensemble
     xs, ms = ens

@@ -51,6 +51,21 @@ def test_ensemble_rhs_exact_bitwise(exe, ens, cfg, with_mass):
     assert ms_ > 0.0
 
 
+@pytest.mark.parametrize("cfg", [-1, 1, 6, 11])
+def test_ensemble_rhs_exact_bitwise_many_systems(exe, cfg):
+    """600 systems of 16 to 300 bodies (about 1700 tiles): more workgroups
+    than the 256 CUs hold at once, XCD placement on and off, 128-target tiles
+    for the systems of at least 256 bodies in configuration 11."""
+    from ode_prototyping_amd.ensemble import ensemble_rhs, pack, unpack
+    rng = np.random.default_rng(600)
+    sizes = [int(v) for v in rng.integers(16, 301, size=600)]
+    xs, ms = make_ensemble(sizes, seed0=1000)
+    d, _ = ensemble_rhs(exe, sizes, pack(xs), 1e-3, pack(ms), "exact", tile_config=cfg)
+    for k, part in enumerate(unpack(d, sizes)):
+        ref = O.rhs(O.RHS_NBODY, xs[k], 1e-3, ms[k])
+        assert np.array_equal(bits(part), bits(ref)), (cfg, k, sizes[k])
+
+
 def test_ensemble_rhs_exact_zero_softening(exe):
     """eps = 0: the IEEE term path (no shared reciprocal) and the skipped
     diagonal (0/0 would be NaN)."""

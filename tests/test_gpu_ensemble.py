@@ -136,6 +136,29 @@ def test_ensemble_solve_matches_oracle(exe, ens, tableau, tf, cfg):
     assert res.rounds == res.attempts.max()
 
 
+def test_ensemble_solve_many_systems_with_compaction(exe):
+    """120 systems of 16 to 160 bodies: they finish at different rounds, so
+    the running set is compacted (tables rebuilt, graph re-captured) several
+    times; every system still matches its own oracle solve bit for bit."""
+    from ode_prototyping_amd.ensemble import pack, solve_ensemble
+    rng = np.random.default_rng(120)
+    sizes = [int(v) for v in rng.integers(16, 161, size=120)]
+    xs, ms = make_ensemble(sizes, seed0=2000)
+    # two rounds per graph replay: the host sees the running count often
+    # enough to compact at every halving
+    res = solve_ensemble("DOPRI5", sizes, pack(xs), 0.0, 0.1, 1e-10, 1e-10, 1e-3, pack(ms),
+                         variant="exact", exe=exe, trace_cap=4096, rounds_per_launch=2)
+    assert res.n_unfinished == 0
+    # finishing times spread enough for the running set to halve repeatedly
+    # (attempts range from about 5 to 300)
+    a = np.sort(res.attempts)
+    assert a[len(a) // 2] < a[-1] // 8 and a[3 * len(a) // 4] < a[-1] // 4
+    finals = res.states()
+    for k in range(len(sizes)):
+        ref = oracle_solve("DOPRI5", xs[k], 0.1, 1e-10, 1e-3, ms[k])
+        assert_system_matches(res, k, ref, finals[k])
+
+
 def test_ensemble_matches_single_solves_fsal(exe, ens):
     """FSAL on, max_accepted budget: each system as odehip_rk_solve alone."""
     import ode_prototyping_amd as oh

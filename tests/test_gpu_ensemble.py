@@ -159,6 +159,23 @@ def test_ensemble_solve_many_systems_with_compaction(exe):
         assert_system_matches(res, k, ref, finals[k])
 
 
+def test_ensemble_dense_output_across_compaction(exe):
+    """DOPRI5 dense rows of the 120-system compaction case at 25 common
+    times: every system's rows equal its oracle solve's rows bit for bit,
+    written before and after the running set is rebuilt."""
+    from ode_prototyping_amd.ensemble import pack, solve_ensemble
+    rng = np.random.default_rng(120)
+    sizes = [int(v) for v in rng.integers(16, 161, size=120)]
+    xs, ms = make_ensemble(sizes, seed0=2000)
+    times = np.linspace(0.0, 0.1, 26)[1:-1]
+    res = solve_ensemble("DOPRI5", sizes, pack(xs), 0.0, 0.1, 1e-10, 1e-10, 1e-3, pack(ms),
+                         variant="exact", exe=exe, rounds_per_launch=2, dense_times=times)
+    assert res.n_unfinished == 0 and np.all(res.n_dense == len(times))
+    for k in range(len(sizes)):
+        ref = oracle_solve("DOPRI5", xs[k], 0.1, 1e-10, 1e-3, ms[k], dense_times=times)
+        assert np.array_equal(bits(res.dense_rows(k)), bits(ref.dense_states)), k
+
+
 def test_ensemble_matches_single_solves_fsal(exe, ens):
     """FSAL on, max_accepted budget: each system as odehip_rk_solve alone."""
     import ode_prototyping_amd as oh

@@ -5,8 +5,8 @@ Host-side Python mirror of the reference's interfaces
 counterpart of ``src/par_exec/CudaExecutor.cuh``), device state containers
 (:class:`HipState`, ``src/state/CudaState.cuh``), ODE functors
 (:class:`NBodyODE`, :class:`ExponentialODE`, :class:`VanDerPolODE`,
-``src/ode``), output policies (:class:`RawOutput`, ``src/output/RawOutput.hpp``; DOP853
-:class:`DenseOutput`, ``NumericalRecipes/Output.hpp``)
+``src/ode``), output policies (:class:`RawOutput`, ``src/output/RawOutput.hpp``;
+DOP853 / DOPRI5 :class:`DenseOutput`, ``NumericalRecipes/Output.hpp``)
 and the integrator (:func:`integrate`, ``RKEmbeddedParallel<BT>::integrate``,
 ``src/integrator/RKEmbeddedParallel/RKEmbeddedParallel.hpp:13-81``), plus
 the leapfrog integrator (:func:`leapfrog`, ``LeapfrogParallel.hpp``), batched

@@ -15,9 +15,10 @@ ensembles (:mod:`.ensemble`), the learned step-size controller
 (:mod:`.nbody_io`).  All numerical work runs in libodehip.so through its C ABI
 (include/odehip.h).
 """
-from ._lib import (CONTROLLERS, NBODY_VARIANTS, REDUCE_GROUP, STAGES, STATUS, TABLEAUS, lib)
+from ._lib import (CONTROLLERS, MIXED_NEAR_DEFAULT, NBODY_VARIANTS, REDUCE_GROUP, STAGES,
+                   STATUS, TABLEAUS, lib)
 from .core import (DenseOutput, ExponentialODE, HipExecutor, HipState, MostRecentOutput,
-                   NBodyODE, RawOutput, SolveResult, VanDerPolODE, integrate,
+                   NBodyODE, OdeHipError, RawOutput, SolveResult, VanDerPolODE, integrate,
                    integrate_host_driven, leapfrog)
 from . import nbody_io, scenarios
 

@@ -68,6 +68,17 @@ class HipExecutor:
         self.check(lib().odehip_timer_stop(self._h, C.byref(ms)), "timer_stop")
         return ms.value
 
+    @property
+    def near_field(self):
+        """Near-field coefficient c of the mixed RHS: pairs with r^2 < c / n in FP64."""
+        c = C.c_double()
+        self.check(lib().odehip_get_near_field(self._h, C.byref(c)), "get_near_field")
+        return c.value
+
+    @near_field.setter
+    def near_field(self, coefficient):
+        self.check(lib().odehip_set_near_field(self._h, float(coefficient)), "set_near_field")
+
     def probe_fp64(self):
         """Sustained FP64 FMA TFLOP/s and shader clock (MHz) of the device."""
         tf, mhz = C.c_double(), C.c_double()

@@ -76,6 +76,15 @@ class HipExecutor {
       throw Error(std::string(what) + ": " + odehip_last_error(ctx_));
   }
   void synchronize() const { check(odehip_synchronize(ctx_), "synchronize"); }
+  // Mixed RHS near field: pairs with r^2 < coefficient / n are FP64.
+  void set_near_field(double coefficient) const {
+    check(odehip_set_near_field(ctx_, coefficient), "set_near_field");
+  }
+  double near_field() const {
+    double c = 0.0;
+    check(odehip_get_near_field(ctx_, &c), "get_near_field");
+    return c;
+  }
 
   static odehip_context* shared_context() {
     static odehip_context* ctx = [] {

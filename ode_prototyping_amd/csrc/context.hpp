@@ -23,6 +23,7 @@ struct GraphKey {
   int tableau = -1, rhs_kind = -1, variant = -1, k = 0, fsal = -1;
   size_t n_var = 0;
   double eps2 = 0.0;
+  float near2 = 0.0f;
   const void* mass = nullptr;
   const void* atol = nullptr;
   const void* rtol = nullptr;
@@ -36,6 +37,7 @@ struct GraphKey {
     return tableau == o.tableau && rhs_kind == o.rhs_kind &&
            variant == o.variant && k == o.k && fsal == o.fsal &&
            n_var == o.n_var && std::memcmp(&eps2, &o.eps2, sizeof eps2) == 0 &&
+           std::memcmp(&near2, &o.near2, sizeof near2) == 0 &&
            mass == o.mass && atol == o.atol && rtol == o.rtol && ws == o.ws &&
            controller == o.controller && mlp == o.mlp &&
            std::memcmp(caps, o.caps, sizeof caps) == 0;
@@ -47,6 +49,8 @@ struct odehip_context {
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;  // current stream (own or external)
   std::string err;
+  // mixed RHS: pairs with r^2 < near_c / n are computed in FP64
+  double near_c = ODEHIP_MIXED_NEAR_DEFAULT;
   // solve workspace
   double* ws = nullptr;
   size_t ws_cap = 0;

@@ -61,7 +61,16 @@ struct NBodyArgs {
   int stage0;           // 1: this launch computes k1 of an attempt;
                         // 2: a dense-output evaluation (runs iff st->dense_go)
   double eps2;
+  float near2;          // mixed variant: pairs with FP32 r^2 below this are
+                        // computed in FP64 (0: none)
 };
+
+// Near-field threshold of the mixed RHS for a system of n bodies: pairs with
+// r^2 < coefficient / n take the FP64 term.  With masses ~1/n this selects the
+// pairs whose single term is a sizeable part of a body's acceleration.
+__host__ __device__ inline float mixed_near2(double coefficient, int n) {
+  return n > 0 ? static_cast<float>(coefficient / n) : 0.0f;
+}
 
 // ---- launchers (nbody_kernels.hip) ----
 hipError_t launch_nbody_rhs(const NBodyArgs& a, int variant, hipStream_t s);
@@ -128,6 +137,7 @@ struct SolveBufs {
   int nbody_variant;
   int n_particles;
   double eps2;
+  float near2;            // mixed RHS near-field threshold (NBodyArgs::near2)
   const double* mass;
   int controller;
   const struct MlpInfer* mlp;  // learned controller's network, or null

@@ -74,6 +74,7 @@ extern "C" int odehip_leapfrog(odehip_context* ctx, const odehip_ode* ode,
   a.mass = ode->masses;
   a.n = static_cast<int>(n_var / 6);
   a.eps2 = ode->softening * ode->softening;
+  a.near2 = mixed_near2(ctx->near_c, a.n);
   a.y = x;
   auto rhs = [&](double* k) {
     a.k = k;

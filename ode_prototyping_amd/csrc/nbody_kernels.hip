@@ -26,7 +26,8 @@
 //            in a fixed order (deterministic, not bit-identical).
 //  * mixed:  displacement computed in FP64 and rounded to FP32, r^-3 and the
 //            pair force in FP32 (hardware rsqrt), per-wave partial sums in
-//            FP32 flushed into FP64 accumulators every 32 partners.
+//            FP32 flushed into FP64 accumulators every 32 partners; near pairs
+//            (r^2 below a per-launch threshold) in FP64, summed in FP64.
 #include "internal.hpp"
 #include "nbody_terms.hpp"
 
@@ -488,6 +489,7 @@ __global__ __launch_bounds__(64 * kMixedWaves) void k_nbody_mixed(NBodyArgs a) {
   const uniform_ptr mass = as_uniform(a.mass);
   const int n = a.n;
   const float eps2f = static_cast<float>(a.eps2);
+  const float near2 = a.near2;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i0 = blockIdx.x * 64;
@@ -505,17 +507,36 @@ __global__ __launch_bounds__(64 * kMixedWaves) void k_nbody_mixed(NBodyArgs a) {
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     for (; j + 8 <= jstop; j += 8) {
       const uniform_ptr pp = yu + 3 * j;
+      // bit u: some lane has partner j + u in its near field
+      uint32_t near = 0;
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        const float mj = kMass ? static_cast<float>(mass[j + u]) : 1.0f;
-        mixed_term<kMass, kGuard>(pp[3 * u], pp[3 * u + 1], pp[3 * u + 2], mj, j + u, i,
-                                  xi, yi, zi, eps2f, fx, fy, fz);
+        const double mj = kMass ? mass[j + u] : 1.0;
+        const bool far = mixed_term<kMass, kGuard>(pp[3 * u], pp[3 * u + 1], pp[3 * u + 2], mj,
+                                                   j + u, i, xi, yi, zi, eps2f, near2, fx, fy,
+                                                   fz);
+        near |= ballot_not(far) ? (1u << u) : 0u;
+      }
+      fx = opaque(fx);
+      fy = opaque(fy);
+      fz = opaque(fz);
+      if (near) {
+        const double xo = opaque(xi), yo = opaque(yi), zo = opaque(zi);
+        do {
+          const int u = __builtin_ctz(near);
+          near &= near - 1;
+          const double mj = kMass ? mass[j + u] : 1.0;
+          mixed_near_term<kMass, kGuard>(pp[3 * u], pp[3 * u + 1], pp[3 * u + 2], mj, j + u,
+                                         i, xo, yo, zo, eps2f, a.eps2, near2, ax, ay, az);
+        } while (near);
       }
     }
     for (; j < jstop; ++j) {
-      const float mj = kMass ? static_cast<float>(mass[j]) : 1.0f;
-      mixed_term<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j, i,
-                                xi, yi, zi, eps2f, fx, fy, fz);
+      const double mj = kMass ? mass[j] : 1.0;
+      if (!mixed_term<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j, i, xi,
+                                     yi, zi, eps2f, near2, fx, fy, fz))
+        mixed_near_term<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j, i,
+                                       xi, yi, zi, eps2f, a.eps2, near2, ax, ay, az);
     }
     ax += static_cast<double>(fx);
     ay += static_cast<double>(fy);

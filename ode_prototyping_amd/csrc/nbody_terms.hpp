@@ -118,22 +118,73 @@ __device__ inline void fast_term(double xj, double yj, double zj, double mj,
   az = __builtin_fma(dz, w, az);
 }
 
+// Mixed-precision pair term.  The displacement is formed in FP64 and rounded
+// to FP32; r^-3 (hardware rsqrt) and the term are FP32 and go into the FP32
+// partial sums (fx, fy, fz).  A near pair -- FP32 r^2 below near2 -- is left
+// out here and the function returns false (true: a far pair, done); the
+// caller then adds its FP64 term with mixed_near_term.  In a close encounter one partner dominates a body's
+// acceleration, and FP32's ~1e-7 relative error on that one term would set
+// the accuracy of the whole solve.  near2 = 0 gives the all-FP32 form.
 template <bool kMass, bool kGuard>
-__device__ inline void mixed_term(double xj, double yj, double zj, float mj,
+__device__ inline bool mixed_term(double xj, double yj, double zj, double mj,
                                   int j, int i, double xi, double yi,
-                                  double zi, float eps2f, float& ax, float& ay,
-                                  float& az) {
+                                  double zi, float eps2f, float near2,
+                                  float& fx, float& fy, float& fz) {
   const float dx = static_cast<float>(xj - xi);
   const float dy = static_cast<float>(yj - yi);
   const float dz = static_cast<float>(zj - zi);
   const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2f)));
   const float rinv = __builtin_amdgcn_rsqf(r2);
   float w = (rinv * rinv) * rinv;
-  if (kMass) w *= mj;
+  if (kMass) w *= static_cast<float>(mj);
   if (kGuard) w = (j == i) ? 0.0f : w;
-  ax = __builtin_fmaf(dx, w, ax);
-  ay = __builtin_fmaf(dy, w, ay);
-  az = __builtin_fmaf(dz, w, az);
+  const bool far = !(r2 < near2);
+  w = far ? w : 0.0f;
+  fx = __builtin_fmaf(dx, w, fx);
+  fy = __builtin_fmaf(dy, w, fy);
+  fz = __builtin_fmaf(dz, w, fz);
+  return far;
+}
+
+// Mask of the active lanes of the wave for which c is false (from the
+// compare result kept in a scalar register pair; OR-ing such masks runs on
+// the scalar unit).
+__device__ inline uint64_t ballot_not(bool c) {
+  return ~__builtin_amdgcn_ballot_w64(c) & __builtin_amdgcn_ballot_w64(true);
+}
+
+// Hides a value's origin from the compiler, so that expressions of it are
+// not merged with the same expressions elsewhere (no code is emitted).  The
+// near-pair pass recomputes displacements instead of keeping every partner's
+// FP64 displacement of the far-pair pass alive across it.
+__device__ inline double opaque(double v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+// The same for the FP32 partial sums just before the near-pair branch: they
+// are complete there, so the far-pair pass is not sunk past the branch (which
+// would keep every partner's displacement and r^2 live across it).
+__device__ inline float opaque(float v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+// The FP64 term of a pair that mixed_term left out (the same FP32 test picks
+// it), added into the FP64 sums.  Callers batch partners and enter this only
+// when some lane has a near pair among them, so the loop over far pairs stays
+// one straight-line block; they pass the target through opaque().
+template <bool kMass, bool kGuard>
+__device__ inline void mixed_near_term(double xj, double yj, double zj,
+                                       double mj, int j, int i, double xi,
+                                       double yi, double zi, float eps2f,
+                                       double eps2, float near2, double& ax,
+                                       double& ay, double& az) {
+  const float dx = static_cast<float>(xj - xi);
+  const float dy = static_cast<float>(yj - yi);
+  const float dz = static_cast<float>(zj - zi);
+  const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2f)));
+  if (r2 < near2)
+    fast_term<kMass, kGuard>(xj, yj, zj, mj, j, i, xi, yi, zi, eps2, ax, ay, az);
 }
 
 }  // namespace odehip

@@ -124,6 +124,7 @@ int prepare_bufs(odehip_context* ctx, int tableau, const odehip_ode* ode,
   b->nbody_variant = ode->nbody_variant;
   b->n_particles = static_cast<int>(n / 6);
   b->eps2 = ode->softening * ode->softening;
+  b->near2 = mixed_near2(ctx->near_c, b->n_particles);
   b->mass = ode->masses;
   return ODEHIP_OK;
 }
@@ -192,6 +193,20 @@ const char* odehip_last_error(const odehip_context* ctx) {
 int odehip_set_stream(odehip_context* ctx, void* hip_stream) {
   if (!ctx) return ODEHIP_ERR_INVALID;
   ctx->stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->own;
+  return ODEHIP_OK;
+}
+
+int odehip_set_near_field(odehip_context* ctx, double coefficient) {
+  if (!ctx) return ODEHIP_ERR_INVALID;
+  if (!(coefficient >= 0.0) || !std::isfinite(coefficient))
+    return fail(ctx, ODEHIP_ERR_INVALID, "near-field coefficient must be finite and >= 0");
+  ctx->near_c = coefficient;
+  return ODEHIP_OK;
+}
+
+int odehip_get_near_field(const odehip_context* ctx, double* coefficient) {
+  if (!ctx || !coefficient) return ODEHIP_ERR_INVALID;
+  *coefficient = ctx->near_c;
   return ODEHIP_OK;
 }
 
@@ -317,6 +332,7 @@ int odehip_nbody_rhs(odehip_context* ctx, const double* x, double* dxdt,
   a.mass = masses;
   a.n = static_cast<int>(n);
   a.eps2 = softening * softening;
+  a.near2 = mixed_near2(ctx->near_c, a.n);
   HIP_TRY(ctx, launch_nbody_rhs(a, variant, ctx->stream));
   return ODEHIP_OK;
 }
@@ -533,6 +549,7 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
     key.fsal = opt->fsal ? 1 : 0;
     key.n_var = n;
     key.eps2 = b.eps2;
+    key.near2 = b.near2;
     key.mass = ode->masses;
     key.atol = opt->atol;
     key.rtol = opt->rtol;

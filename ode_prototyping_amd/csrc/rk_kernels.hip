@@ -444,6 +444,7 @@ hipError_t enqueue_rhs(const SolveBufs& b, const double* y_fixed, double* k,
     a.n = b.n_particles;
     a.stage0 = stage0;
     a.eps2 = b.eps2;
+    a.near2 = b.near2;
     return launch_nbody_rhs(a, b.nbody_variant, s);
   }
   return launch_simple_rhs(b.rhs_kind, y_fixed, b.xb0, b.xb1, b.st, k, b.kfsal,

@@ -197,6 +197,29 @@ def test_ensemble_matches_single_solves_fsal(exe, ens):
     assert np.all(res.accepted <= 25)
 
 
+@pytest.mark.parametrize("cfg", [0, 5, 10])
+def test_ensemble_rhs_mixed_all_near_is_fp64(exe, ens, cfg):
+    """A near field covering every pair turns the ensemble's mixed RHS into
+    FP64 terms (the fast variant's bar); the default one keeps the FP32 bar."""
+    import ode_prototyping_amd as oh
+    from ode_prototyping_amd.ensemble import ensemble_rhs, pack, unpack
+    xs, ms = ens
+    exe.near_field = 1e30
+    try:
+        d, _ = ensemble_rhs(exe, SIZES, pack(xs), 1e-3, pack(ms), "mixed", tile_config=cfg)
+    finally:
+        exe.near_field = oh.MIXED_NEAR_DEFAULT
+    for k, part in enumerate(unpack(d, SIZES)):
+        n = SIZES[k]
+        ref = O.rhs(O.RHS_NBODY, xs[k], 1e-3, ms[k])
+        pos = xs[k][:3 * n].reshape(n, 3)
+        r = pos[None, :, :] - pos[:, None, :]
+        w = ms[k][None, :] / (np.sum(r * r, axis=2) + 1e-6) ** 1.5
+        scale = np.sum(np.abs(r) * w[:, :, None], axis=1).ravel() + 1e-300
+        err = np.abs(part[3 * n:] - ref[3 * n:]) / scale
+        assert err.max() < 2e-14, (n, err.max())
+
+
 @pytest.mark.parametrize("variant", ["fast", "mixed"])
 def test_ensemble_matches_single_solves_variants(exe, variant):
     """The fast and mixed right-hand sides of the ensemble sum partners in

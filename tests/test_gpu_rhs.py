@@ -3,7 +3,7 @@
 Bar: the exact variant is bit-identical to the oracle (itself bit-identical
 to the reference's NBodyODE with SingleThreadedExecutor); the fast FP64
 variant within 5e-14 of the largest acceleration component; the mixed
-variant within 2e-6.
+variant within 2e-6, and its near-field pairs to FP64 accuracy.
 """
 import numpy as np
 import pytest
@@ -92,6 +92,57 @@ def test_mixed_rhs_close(exe, n):
     assert np.array_equal(got[:3 * n], want[:3 * n])
     scale = np.max(np.abs(acc_w))
     assert np.max(np.abs(acc_g - acc_w)) <= 2e-6 * scale
+
+
+@pytest.mark.parametrize("n", [2, 65, 1000])
+@pytest.mark.parametrize("eps", [1e-2, 0.0])
+def test_mixed_rhs_all_near_is_fp64(exe, n, eps):
+    """With a near field that covers every pair, the mixed RHS computes every
+    term in FP64: the fast variant's bar (5e-14), not FP32's."""
+    import ode_prototyping_amd as oh
+    x, m = ic(n, seed=13)
+    exe.near_field = 1e30
+    try:
+        got = gpu_rhs(exe, x, eps, m, "mixed")
+    finally:
+        exe.near_field = oh.MIXED_NEAR_DEFAULT
+    want = O.rhs(O.RHS_NBODY, x, eps, m, threads=8)
+    scale = np.max(np.abs(want[3 * n:]))
+    assert np.max(np.abs(got[3 * n:] - want[3 * n:])) <= 5e-14 * scale
+
+
+def test_mixed_rhs_close_pair_in_fp64(exe):
+    """A close pair (distance 1e-3, softening 1e-4) among 500 bodies: its two
+    accelerations are dominated by the pair term, which the default near field
+    computes in FP64 (rel. error <= 1e-10); all-FP32 (near field 0) only meets
+    the FP32 bar."""
+    import ode_prototyping_amd as oh
+    n = 500
+    x, m = ic(n, seed=17)
+    x[3:6] = x[0:3] + np.array([6e-4, -5e-4, 6e-4])  # |d| ~ 1e-3
+    want = O.rhs(O.RHS_NBODY, x, 1e-4, m, threads=8)
+    assert exe.near_field == oh.MIXED_NEAR_DEFAULT
+    got = gpu_rhs(exe, x, 1e-4, m, "mixed")
+    pair = want[3 * n:3 * n + 6]
+    rel = np.max(np.abs(got[3 * n:3 * n + 6] - pair)) / np.max(np.abs(pair))
+    assert rel <= 1e-10, rel
+    exe.near_field = 0.0
+    try:
+        got32 = gpu_rhs(exe, x, 1e-4, m, "mixed")
+    finally:
+        exe.near_field = oh.MIXED_NEAR_DEFAULT
+    rel32 = np.max(np.abs(got32[3 * n:3 * n + 6] - pair)) / np.max(np.abs(pair))
+    assert rel32 <= 2e-6
+    scale = np.max(np.abs(want[3 * n:]))
+    assert np.max(np.abs(got[3 * n:] - want[3 * n:])) <= 2e-6 * scale
+
+
+def test_near_field_setter_rejects_bad_values(exe):
+    import ode_prototyping_amd as oh
+    for bad in (-1.0, float("nan"), float("inf")):
+        with pytest.raises(oh.OdeHipError):
+            exe.near_field = bad
+    assert exe.near_field == oh.MIXED_NEAR_DEFAULT
 
 
 def test_nbody_acc_matches_rhs(exe):

@@ -329,23 +329,28 @@ def precision_block(oh, exe):
                          f"closing at 0.3, eps={eps}, {tab}, rtol=atol={tol}, {acc} accepted "
                          "steps"}
     finals = {}
-    for var in ("exact", "mixed"):
+    for name in ("exact", "mixed", "mixed_near_field"):
+        var = "mixed" if name == "mixed_near_field" else name
+        exe.near_field = (oh.MIXED_NEAR_CLOSE if name == "mixed_near_field"
+                          else oh.MIXED_NEAR_DEFAULT)
         o = oh.NBodyODE(eps, m, var)
         t = time.perf_counter()
         r = oh.integrate(tab, x0d, 0.0, 100.0, atol, atol, o, exe=exe, max_accepted=acc)
         secs = time.perf_counter() - t
-        finals[var] = r.x_final
+        finals[name] = r.x_final
         e1 = nbody_energy(oh, exe, r.x_final, n, eps, m)
-        block[var] = {"seconds": secs, "particle_steps_per_s": n * r.accepted / secs,
-                      "accepted": r.accepted, "rejected": r.rejected, "t": r.t,
-                      "rhs_evals": r.rhs_evals, "energy_rel_err": abs((e1 - e0) / e0)}
-        log(f"precision {var}: {block[var]}")
+        block[name] = {"seconds": secs, "particle_steps_per_s": n * r.accepted / secs,
+                       "accepted": r.accepted, "rejected": r.rejected, "t": r.t,
+                       "rhs_evals": r.rhs_evals, "energy_rel_err": abs((e1 - e0) / e0),
+                       "near_field": exe.near_field if var == "mixed" else None}
+        log(f"precision {name}: {block[name]}")
+    exe.near_field = oh.MIXED_NEAR_DEFAULT
     # the exact path equals the FP64 oracle bit for bit; compare at the same time
     o = oh.NBodyODE(eps, m, "exact")
-    t_mixed = block["mixed"]["t"]
-    ref = oh.integrate(tab, x0d, 0.0, t_mixed, atol, atol, o, exe=exe).x_final
-    block["mixed"]["final_state_rel_err_vs_fp64_same_t"] = float(
-        np.max(np.abs(finals["mixed"] - ref)) / np.max(np.abs(ref)))
+    for name in ("mixed", "mixed_near_field"):
+        ref = oh.integrate(tab, x0d, 0.0, block[name]["t"], atol, atol, o, exe=exe).x_final
+        block[name]["final_state_rel_err_vs_fp64_same_t"] = float(
+            np.max(np.abs(finals[name] - ref)) / np.max(np.abs(ref)))
     return block
 
 
@@ -499,20 +504,25 @@ def main():
 
     if not args.no_extras:
         extras = {}
-        for var in ("exact", "fast", "mixed"):
-            if var == args.variant:
+        for name in ("exact", "fast", "mixed", "mixed_near_field"):
+            var = "mixed" if name == "mixed_near_field" else name
+            if name == args.variant:
                 continue
+            exe.near_field = (oh.MIXED_NEAR_CLOSE if name == "mixed_near_field"
+                              else oh.MIXED_NEAR_DEFAULT)
             o2 = oh.NBodyODE(softening, m, var)
             t = time.perf_counter()
             r2 = solve(o2)
             exe.synchronize()
             secs = time.perf_counter() - t
             kms = rhs_kernel_ms(exe, oh, xs, ks, o2, 5)
-            extras[var] = {"particle_steps_per_s": n * r2.accepted / secs,
-                           "ms_per_solve": 1000 * secs, "rhs_evals_per_solve": r2.rhs_evals,
-                           "rhs_kernel_ms": kms,
-                           "rhs_tflops": FLOP_PER_INTERACTION * interactions / (kms * 1e-3) / 1e12}
-            log(f"{var}: {extras[var]}")
+            extras[name] = {"particle_steps_per_s": n * r2.accepted / secs,
+                            "near_field": exe.near_field if var == "mixed" else None,
+                            "ms_per_solve": 1000 * secs, "rhs_evals_per_solve": r2.rhs_evals,
+                            "rhs_kernel_ms": kms,
+                            "rhs_tflops": FLOP_PER_INTERACTION * interactions / (kms * 1e-3) / 1e12}
+            log(f"{name}: {extras[name]}")
+        exe.near_field = oh.MIXED_NEAR_DEFAULT
         # chosen path on this workload: mixed-precision RHS + k1 reuse after rejections
         o3 = oh.NBodyODE(softening, m, "mixed")
         t = time.perf_counter()

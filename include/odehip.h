@@ -51,13 +51,17 @@ extern "C" {
 #define ODEHIP_NBODY_FAST 1  /* FP64 with a refined hardware rsqrt and
                                 parallel partial sums (~1 ulp per term) */
 #define ODEHIP_NBODY_MIXED 2 /* displacement rounded to FP32, pair force in
-                                FP32, accumulation in FP64; near pairs
-                                (r^2 < c / n, odehip_set_near_field) in FP64 */
-/* Default near-field coefficient c of the mixed variant: with masses ~1/n a
-   pair closer than sqrt(c / n) is one whose single term is a sizeable part of
-   a body's acceleration, where FP32's ~1e-7 relative error would dominate the
-   solve's error at tight tolerances. */
-#define ODEHIP_MIXED_NEAR_DEFAULT 20.0
+                                FP32, accumulation in FP64; in single-system
+                                calls near pairs (r^2 < c / n,
+                                odehip_set_near_field) in FP64 */
+/* Near-field coefficient c of the mixed variant: with masses ~1/n a pair
+   closer than sqrt(c / n) is one whose single term is a sizeable part of a
+   body's acceleration, where FP32's ~1e-7 relative error would dominate the
+   solve's error at tight tolerances.  Off by default (all pairs FP32, the
+   fastest form); ODEHIP_MIXED_NEAR_CLOSE is the setting for systems with
+   close encounters. */
+#define ODEHIP_MIXED_NEAR_DEFAULT 0.0
+#define ODEHIP_MIXED_NEAR_CLOSE 20.0
 
 /* Solve exit status (the reference only ever exits with REACHED_END). */
 /* Step-size controllers.  REFERENCE is RKEmbeddedParallel.hpp:64-79
@@ -109,8 +113,10 @@ int odehip_set_stream(odehip_context* ctx, void* hip_stream);
  * counterpart: the reference has only FP64 pair terms, NBodyODE.hpp:89-98).
  * Pairs with FP32 r^2 < c / n are computed and summed in FP64; c = 0 gives
  * all pairs in FP32.  c must be finite and >= 0; the default is
- * ODEHIP_MIXED_NEAR_DEFAULT.  Applies to every later mixed-variant call
- * (RHS, solve, ensemble, leapfrog) on the context. */
+ * ODEHIP_MIXED_NEAR_DEFAULT.  Applies to every later single-system
+ * mixed-variant call (RHS, acc, solve, leapfrog) on the context; the
+ * ensemble entry points (odehip_ensemble_*) compute every mixed pair in
+ * FP32 and ignore it. */
 int odehip_set_near_field(odehip_context* ctx, double coefficient);
 int odehip_get_near_field(const odehip_context* ctx, double* coefficient);
 int odehip_synchronize(odehip_context* ctx);

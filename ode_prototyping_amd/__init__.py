@@ -15,8 +15,8 @@ ensembles (:mod:`.ensemble`), the learned step-size controller
 (:mod:`.nbody_io`).  All numerical work runs in libodehip.so through its C ABI
 (include/odehip.h).
 """
-from ._lib import (CONTROLLERS, MIXED_NEAR_DEFAULT, NBODY_VARIANTS, REDUCE_GROUP, STAGES,
-                   STATUS, TABLEAUS, lib)
+from ._lib import (CONTROLLERS, MIXED_NEAR_CLOSE, MIXED_NEAR_DEFAULT, NBODY_VARIANTS,
+                   REDUCE_GROUP, STAGES, STATUS, TABLEAUS, lib)
 from .core import (DenseOutput, ExponentialODE, HipExecutor, HipState, MostRecentOutput,
                    NBodyODE, OdeHipError, RawOutput, SolveResult, VanDerPolODE, integrate,
                    integrate_host_driven, leapfrog)

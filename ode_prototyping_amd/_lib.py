@@ -17,7 +17,8 @@ TABLEAUS = {"HE21": 0, "RKF45": 1, "DOPRI5": 2, "DVERK": 3, "RKF78": 4, "DOP853"
 STAGES = {"HE21": 2, "RKF45": 6, "DOPRI5": 7, "DVERK": 8, "RKF78": 13, "DOP853": 12}
 RHS_NBODY, RHS_EXPONENTIAL, RHS_VANDERPOL = 0, 1, 2
 NBODY_VARIANTS = {"exact": 0, "fast": 1, "mixed": 2}
-MIXED_NEAR_DEFAULT = 20.0  # ODEHIP_MIXED_NEAR_DEFAULT
+MIXED_NEAR_DEFAULT = 0.0  # ODEHIP_MIXED_NEAR_DEFAULT
+MIXED_NEAR_CLOSE = 20.0  # ODEHIP_MIXED_NEAR_CLOSE
 CONTROLLERS = {"reference": 0, "pi": 1, "learned": 2}
 STATUS = {-1: "running", 0: "reached_end", 1: "max_accepted", 2: "nonfinite", 3: "max_attempts"}
 REDUCE_GROUP = 256

@@ -389,7 +389,6 @@ int odehip_ensemble_rhs(odehip_context* ctx, int n_systems,
   b.mass = masses;
   b.variant = nbody_variant;
   b.eps2 = eps2;
-  b.near_c = ctx->near_c;
   set_tile_cfg(&b, cfg, n_tiles);
   HIP_TRY(ctx, hipEventRecord(w->t0, s));
   HIP_TRY(ctx, ens_launch_rhs(b, x, dxdt, 0, s));
@@ -465,7 +464,6 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
   b.tableau = tableau;
   b.variant = nbody_variant;
   b.eps2 = eps2;
-  b.near_c = ctx->near_c;
   b.controller = opt->controller;
   b.mlp = opt->controller == ODEHIP_CONTROLLER_LEARNED ? mlp_infer_ptr(opt->mlp) : nullptr;
   if (n_dense > 0) {

@@ -232,13 +232,12 @@ __device__ inline void fast_tile(const double* __restrict__ y,
 }
 
 // mixed: FP64 displacement rounded to FP32, FP32 pair force, partial sums
-// in FP32 flushed into FP64 every 32 partners; near pairs (FP32 r^2 < near2)
-// in FP64, summed in FP64.
+// in FP32 flushed into FP64 every 32 partners.
 template <bool kMass, bool kGuard, int TPL>
 __device__ inline void mixed_tile(const double* __restrict__ y,
                                   const double* __restrict__ mass,
                                   double* __restrict__ k, int n, int i0,
-                                  double eps2, float near2) {
+                                  double eps2) {
   constexpr int kFlush = 32;
   constexpr int U = 8;
   const int lane = threadIdx.x & 63;
@@ -264,56 +263,22 @@ __device__ inline void mixed_tile(const double* __restrict__ y,
     for (int t = 0; t < TPL; ++t) fx[t] = fy[t] = fz[t] = 0.0f;
     for (; j + U <= jstop; j += U) {
       const uniform_ptr pp = yu + 3 * j;
-      // bit u: some lane has partner j + u in the near field of one of its targets
-      uint32_t near = 0;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const double mj = kMass ? mu[j + u] : 1.0;
-        uint64_t lanes = 0;
+        const float mj = kMass ? static_cast<float>(mu[j + u]) : 1.0f;
 #pragma unroll
         for (int t = 0; t < TPL; ++t)
-          lanes |= ballot_not(mixed_term<kMass, kGuard>(pp[3 * u], pp[3 * u + 1],
-                                                        pp[3 * u + 2], mj, j + u, ic[t], xi[t],
-                                                        yi[t], zi[t], eps2f, near2, fx[t],
-                                                        fy[t], fz[t]));
-        near |= lanes ? (1u << u) : 0u;
-      }
-#pragma unroll
-      for (int t = 0; t < TPL; ++t) {
-        fx[t] = opaque(fx[t]);
-        fy[t] = opaque(fy[t]);
-        fz[t] = opaque(fz[t]);
-      }
-      if (near) {
-        double xo[TPL], yo[TPL], zo[TPL];
-#pragma unroll
-        for (int t = 0; t < TPL; ++t) {
-          xo[t] = opaque(xi[t]);
-          yo[t] = opaque(yi[t]);
-          zo[t] = opaque(zi[t]);
-        }
-        do {
-          const int u = __builtin_ctz(near);
-          near &= near - 1;
-          const double mj = kMass ? mu[j + u] : 1.0;
-#pragma unroll
-          for (int t = 0; t < TPL; ++t)
-            mixed_near_term<kMass, kGuard>(pp[3 * u], pp[3 * u + 1], pp[3 * u + 2], mj,
-                                           j + u, ic[t], xo[t], yo[t], zo[t], eps2f, eps2,
-                                           near2, ax[t], ay[t], az[t]);
-        } while (near);
+          mixed_term_fp32<kMass, kGuard>(pp[3 * u], pp[3 * u + 1], pp[3 * u + 2], mj,
+                                         j + u, ic[t], xi[t], yi[t], zi[t], eps2f, fx[t],
+                                         fy[t], fz[t]);
       }
     }
     for (; j < jstop; ++j) {
-      const double mj = kMass ? mu[j] : 1.0;
+      const float mj = kMass ? static_cast<float>(mu[j]) : 1.0f;
 #pragma unroll
       for (int t = 0; t < TPL; ++t)
-        if (!mixed_term<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j, ic[t],
-                                       xi[t], yi[t], zi[t], eps2f, near2, fx[t], fy[t],
-                                       fz[t]))
-          mixed_near_term<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j,
-                                         ic[t], xi[t], yi[t], zi[t], eps2f, eps2, near2,
-                                         ax[t], ay[t], az[t]);
+        mixed_term_fp32<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j,
+                                       ic[t], xi[t], yi[t], zi[t], eps2f, fx[t], fy[t], fz[t]);
     }
 #pragma unroll
     for (int t = 0; t < TPL; ++t) {
@@ -375,11 +340,10 @@ __global__ __launch_bounds__(64 * WPB) void k_ens_rhs(EnsBufs b,
     else
       fast_tile<kMass, kSpecial, 1>(y, m, k, n, tile.i0, b.eps2);
   } else {
-    const float near2 = mixed_near2(b.near_c, n);
     if (tpl == 2)
-      mixed_tile<kMass, kSpecial, 2>(y, m, k, n, tile.i0, b.eps2, near2);
+      mixed_tile<kMass, kSpecial, 2>(y, m, k, n, tile.i0, b.eps2);
     else
-      mixed_tile<kMass, kSpecial, 1>(y, m, k, n, tile.i0, b.eps2, near2);
+      mixed_tile<kMass, kSpecial, 1>(y, m, k, n, tile.i0, b.eps2);
   }
 }
 

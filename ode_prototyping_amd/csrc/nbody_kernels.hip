@@ -479,7 +479,8 @@ __global__ __launch_bounds__(64 * kFastWaves) void k_nbody_fast(NBodyArgs a) {
 constexpr int kMixedWaves = 16;
 constexpr int kMixedFlush = 32;
 
-template <bool kMass, bool kGuard>
+// kNear: near pairs (a.near2 > 0) in FP64; without it every pair is FP32.
+template <bool kMass, bool kGuard, bool kNear>
 __global__ __launch_bounds__(64 * kMixedWaves) void k_nbody_mixed(NBodyArgs a) {
   if (prologue_skip(a)) return;
   __shared__ double red[kMixedWaves][3][64];
@@ -505,7 +506,23 @@ __global__ __launch_bounds__(64 * kMixedWaves) void k_nbody_mixed(NBodyArgs a) {
   while (j < jend) {
     const int jstop = min(jend, j + kMixedFlush);
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    for (; j + 8 <= jstop; j += 8) {
+    if constexpr (!kNear) {
+      for (; j + 8 <= jstop; j += 8) {
+        const uniform_ptr pp = yu + 3 * j;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const float mj = kMass ? static_cast<float>(mass[j + u]) : 1.0f;
+          mixed_term_fp32<kMass, kGuard>(pp[3 * u], pp[3 * u + 1], pp[3 * u + 2], mj, j + u,
+                                         i, xi, yi, zi, eps2f, fx, fy, fz);
+        }
+      }
+      for (; j < jstop; ++j) {
+        const float mj = kMass ? static_cast<float>(mass[j]) : 1.0f;
+        mixed_term_fp32<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j, i, xi,
+                                       yi, zi, eps2f, fx, fy, fz);
+      }
+    }
+    for (; kNear && j + 8 <= jstop; j += 8) {
       const uniform_ptr pp = yu + 3 * j;
       // bit u: some lane has partner j + u in its near field
       uint32_t near = 0;
@@ -531,7 +548,7 @@ __global__ __launch_bounds__(64 * kMixedWaves) void k_nbody_mixed(NBodyArgs a) {
         } while (near);
       }
     }
-    for (; j < jstop; ++j) {
+    for (; kNear && j < jstop; ++j) {
       const double mj = kMass ? mass[j] : 1.0;
       if (!mixed_term<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j, i, xi,
                                      yi, zi, eps2f, near2, fx, fy, fz))
@@ -691,17 +708,17 @@ hipError_t launch_nbody_rhs(const NBodyArgs& a, int variant, hipStream_t s) {
       break;
     case ODEHIP_NBODY_MIXED: {
       const bool guardf = !(static_cast<float>(a.eps2) >= 1e-12f);
-      if (mass) {
-        if (guardf)
-          hipLaunchKernelGGL((k_nbody_mixed<true, true>), dim3(blocks), dim3(64 * kMixedWaves), 0, s, a);
-        else
-          hipLaunchKernelGGL((k_nbody_mixed<true, false>), dim3(blocks), dim3(64 * kMixedWaves), 0, s, a);
-      } else {
-        if (guardf)
-          hipLaunchKernelGGL((k_nbody_mixed<false, true>), dim3(blocks), dim3(64 * kMixedWaves), 0, s, a);
-        else
-          hipLaunchKernelGGL((k_nbody_mixed<false, false>), dim3(blocks), dim3(64 * kMixedWaves), 0, s, a);
-      }
+      const bool near = a.near2 > 0.0f;
+      const dim3 g(blocks), th(64 * kMixedWaves);
+      auto* kern = mass ? (guardf ? (near ? k_nbody_mixed<true, true, true>
+                                          : k_nbody_mixed<true, true, false>)
+                                  : (near ? k_nbody_mixed<true, false, true>
+                                          : k_nbody_mixed<true, false, false>))
+                        : (guardf ? (near ? k_nbody_mixed<false, true, true>
+                                          : k_nbody_mixed<false, true, false>)
+                                  : (near ? k_nbody_mixed<false, false, true>
+                                          : k_nbody_mixed<false, false, false>));
+      hipLaunchKernelGGL(kern, g, th, 0, s, a);
       break;
     }
     default:

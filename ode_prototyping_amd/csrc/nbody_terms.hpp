@@ -118,6 +118,27 @@ __device__ inline void fast_term(double xj, double yj, double zj, double mj,
   az = __builtin_fma(dz, w, az);
 }
 
+// All-FP32 mixed-precision pair term (the ensemble kernels' form): the
+// displacement is formed in FP64 and rounded to FP32; r^-3 (hardware rsqrt)
+// and the term are FP32, summed into FP32 partials.
+template <bool kMass, bool kGuard>
+__device__ inline void mixed_term_fp32(double xj, double yj, double zj, float mj,
+                                       int j, int i, double xi, double yi,
+                                       double zi, float eps2f, float& ax, float& ay,
+                                       float& az) {
+  const float dx = static_cast<float>(xj - xi);
+  const float dy = static_cast<float>(yj - yi);
+  const float dz = static_cast<float>(zj - zi);
+  const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2f)));
+  const float rinv = __builtin_amdgcn_rsqf(r2);
+  float w = (rinv * rinv) * rinv;
+  if (kMass) w *= mj;
+  if (kGuard) w = (j == i) ? 0.0f : w;
+  ax = __builtin_fmaf(dx, w, ax);
+  ay = __builtin_fmaf(dy, w, ay);
+  az = __builtin_fmaf(dz, w, az);
+}
+
 // Mixed-precision pair term.  The displacement is formed in FP64 and rounded
 // to FP32; r^-3 (hardware rsqrt) and the term are FP32 and go into the FP32
 // partial sums (fx, fy, fz).  A near pair -- FP32 r^2 below near2 -- is left

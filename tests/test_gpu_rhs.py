@@ -113,24 +113,24 @@ def test_mixed_rhs_all_near_is_fp64(exe, n, eps):
 
 def test_mixed_rhs_close_pair_in_fp64(exe):
     """A close pair (distance 1e-3, softening 1e-4) among 500 bodies: its two
-    accelerations are dominated by the pair term, which the default near field
-    computes in FP64 (rel. error <= 1e-10); all-FP32 (near field 0) only meets
-    the FP32 bar."""
+    accelerations are dominated by the pair term, which the close-encounter
+    near field (20) computes in FP64 (rel. error <= 1e-10); the default
+    all-FP32 form (near field 0) only meets the FP32 bar."""
     import ode_prototyping_amd as oh
     n = 500
     x, m = ic(n, seed=17)
     x[3:6] = x[0:3] + np.array([6e-4, -5e-4, 6e-4])  # |d| ~ 1e-3
     want = O.rhs(O.RHS_NBODY, x, 1e-4, m, threads=8)
-    assert exe.near_field == oh.MIXED_NEAR_DEFAULT
-    got = gpu_rhs(exe, x, 1e-4, m, "mixed")
+    assert exe.near_field == oh.MIXED_NEAR_DEFAULT == 0.0
+    exe.near_field = oh.MIXED_NEAR_CLOSE
+    try:
+        got = gpu_rhs(exe, x, 1e-4, m, "mixed")
+    finally:
+        exe.near_field = oh.MIXED_NEAR_DEFAULT
     pair = want[3 * n:3 * n + 6]
     rel = np.max(np.abs(got[3 * n:3 * n + 6] - pair)) / np.max(np.abs(pair))
     assert rel <= 1e-10, rel
-    exe.near_field = 0.0
-    try:
-        got32 = gpu_rhs(exe, x, 1e-4, m, "mixed")
-    finally:
-        exe.near_field = oh.MIXED_NEAR_DEFAULT
+    got32 = gpu_rhs(exe, x, 1e-4, m, "mixed")
     rel32 = np.max(np.abs(got32[3 * n:3 * n + 6] - pair)) / np.max(np.abs(pair))
     assert rel32 <= 2e-6
     scale = np.max(np.abs(want[3 * n:]))

@@ -251,6 +251,36 @@ def test_nonexact_variants_at_fixed_time(exe, variant, bound):
                               ref.trace_acc[:k][decided])
 
 
+@pytest.mark.parametrize("n", [28, 52])
+def test_mixed_near_field_through_close_encounters(exe, n):
+    """Uniform-cube systems (seed n, masses 1/n, eps 1e-3, DOPRI5, rtol = atol
+    = 1e-10, t in [0, 0.1]) that pass through close encounters.  With the
+    close-encounter near field the mixed RHS takes exactly the FP64 oracle's
+    number of RHS evaluations and ends within 5e-9 of its final state; with
+    every pair in FP32 it needs more evaluations and ends >= 100x further
+    off."""
+    import ode_prototyping_amd as oh
+    from ode_prototyping_amd import scenarios
+    x0, m = scenarios.uniform_cube(n, seed=n)
+    ref = oracle_solve("DOPRI5", O.RHS_NBODY, x0, 0.0, 0.1, 1e-10, 1e-3, m)
+    runs = {}
+    for c in (oh.MIXED_NEAR_CLOSE, 0.0):
+        exe.near_field = c
+        try:
+            res = device_solve(exe, "DOPRI5", x0, 0.0, 0.1, 1e-10,
+                               oh.NBodyODE(1e-3, m, "mixed"))
+        finally:
+            exe.near_field = oh.MIXED_NEAR_DEFAULT
+        assert res.t == 0.1
+        rel = np.max(np.abs(res.x_final - ref.x_final)) / np.max(np.abs(ref.x_final))
+        runs[c] = (rel, res.rhs_evals)
+        print(f"n={n} near field {c}: rel err {rel:.3e}, RHS evals {res.rhs_evals} "
+              f"(oracle {ref.rhs_evals})")
+    near, fp32 = runs[oh.MIXED_NEAR_CLOSE], runs[0.0]
+    assert near[0] <= 5e-9 and near[1] == ref.rhs_evals
+    assert fp32[0] >= 100 * near[0] and fp32[1] > ref.rhs_evals
+
+
 def test_energy_drift_matches_oracle(exe):
     """Energy and momentum drift of the plain device path equal the oracle's
     (identical states), and momentum is conserved to rounding."""

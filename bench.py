@@ -373,8 +373,9 @@ def main():
                     help="only run short ensemble solves (2048 systems, t in [0, 0.005]) in "
                          "each RHS variant (the workload of the committed ensemble profiles)")
     ap.add_argument("--profile-rhs", action="store_true",
-                    help="only run each RHS variant 3x and a 64 MiB calibration copy "
-                         "(the workload of the committed counter profiles)")
+                    help="only run each RHS variant 3x, a 64 MiB calibration copy and "
+                         "the near-field mixed RHS 3x (the workload of the committed "
+                         "counter profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): this benchmark "
@@ -392,6 +393,11 @@ def main():
             for _ in range(3):
                 o(exe, xs, ks)
         a, b = oh.HipState(exe, 8 << 20), oh.HipState(exe, 8 << 20)
+        exe.near_field = oh.MIXED_NEAR_CLOSE  # the mixed kernel's near-field instance
+        o = oh.NBodyODE(1e-2, m, "mixed")
+        for _ in range(3):
+            o(exe, xs, ks)
+        exe.near_field = oh.MIXED_NEAR_DEFAULT
         for _ in range(3):
             b.assign(a)
         exe.synchronize()

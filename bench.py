@@ -354,6 +354,27 @@ def precision_block(oh, exe):
     return block
 
 
+def near_field_block(oh, exe):
+    """Mixed RHS through close encounters: all pairs FP32 vs the FP64 near field vs FP64."""
+    tf, eps, tol, tab = 0.1, 1e-3, 1e-10, "DOPRI5"
+    block = {"workload": f"single uniform-cube systems (seed = N, masses 1/N), eps={eps}, {tab}, "
+                         f"rtol=atol={tol}, t in [0, {tf}]; final state vs the exact FP64 solve",
+             "near_field": oh.MIXED_NEAR_CLOSE}
+    for n in (28, 52, 89, 126):
+        x0, m = oh.scenarios.uniform_cube(n, seed=n)
+        ref = oh.integrate(tab, x0, 0.0, tf, tol, tol, oh.NBodyODE(eps, m, "exact"), exe=exe)
+        row = {"fp64_rhs_evals": ref.rhs_evals}
+        for name, c in (("all_fp32", 0.0), ("near_field", oh.MIXED_NEAR_CLOSE)):
+            exe.near_field = c
+            r = oh.integrate(tab, x0, 0.0, tf, tol, tol, oh.NBodyODE(eps, m, "mixed"), exe=exe)
+            row[name] = {"rhs_evals": r.rhs_evals, "final_state_rel_err": float(
+                np.max(np.abs(r.x_final - ref.x_final)) / np.max(np.abs(ref.x_final)))}
+        exe.near_field = oh.MIXED_NEAR_DEFAULT
+        block[f"n{n}"] = row
+        log(f"near field n={n}: {row}")
+    return block
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -555,6 +576,7 @@ def main():
         out["controller"] = controller_block(oh, exe)
     if not args.no_precision:
         out["precision"] = precision_block(oh, exe)
+        out["near_field"] = near_field_block(oh, exe)
 
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

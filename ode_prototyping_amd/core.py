@@ -70,7 +70,11 @@ class HipExecutor:
 
     @property
     def near_field(self):
-        """Near-field coefficient c of the mixed RHS: pairs with r^2 < c / n in FP64."""
+        """Near-field coefficient c of the mixed RHS: pairs with r^2 < c / n in FP64.
+
+        0 (MIXED_NEAR_DEFAULT) keeps every pair in FP32; MIXED_NEAR_CLOSE (20) is
+        the setting for systems with close encounters.  Single-system calls only:
+        the ensemble solver's mixed RHS keeps every pair in FP32."""
         c = C.c_double()
         self.check(lib().odehip_get_near_field(self._h, C.byref(c)), "get_near_field")
         return c.value
@@ -190,7 +194,8 @@ class NBodyODE:
     ``masses``: None for unit masses (the reference), else a length-N array
     (held on the device).  ``variant``: "exact" (bit-identical to the
     reference's single-threaded evaluation), "fast" (FP64, refined rsqrt) or
-    "mixed" (FP32 pair force, FP64 accumulation).
+    "mixed" (FP32 pair force, FP64 accumulation; near pairs in FP64 when the
+    executor's ``near_field`` is set).
     """
     rhs_kind = RHS_NBODY
 

@@ -505,8 +505,9 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
 
   // Rounds run on lists of the running systems' workgroups: every system at
   // first, rebuilt (and the graph re-captured) each time the number of
-  // running systems has halved, so the long tail of a few slow systems does
-  // not launch workgroups for all the finished ones.
+  // running systems has fallen to 3/4 of the last list's, so the long tail of
+  // a few slow systems does not launch workgroups for all the finished ones
+  // (3/4 measured 1-1.5 % faster than 1/2 on 2048 systems; same results).
   auto rounds_per_replay = [&](double sum_n2) {
     int k = opt->rounds_per_launch;
     if (k <= 0) {
@@ -549,7 +550,7 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
   long launched = 0;
   int slot = 0;
   bool have_prev = false;
-  int compact_below = n_systems / 2;
+  int compact_below = n_systems * 3 / 4;
   std::vector<SolverState> hst;
   for (;;) {
     HIP_TRY(ctx, hipGraphLaunch(w->exec, s));
@@ -601,7 +602,7 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
         set_tile_cfg(&rb, run_cfg, static_cast<int>(tiles.size()));
         K = rounds_per_replay(sum_n2);
         if ((rc = capture(rb, K))) return rc;
-        compact_below = n_live / 2;
+        compact_below = n_live * 3 / 4;
         have_prev = false;
         slot = 0;
         continue;

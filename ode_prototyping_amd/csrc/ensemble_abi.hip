@@ -505,9 +505,10 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
 
   // Rounds run on lists of the running systems' workgroups: every system at
   // first, rebuilt (and the graph re-captured) each time the number of
-  // running systems has fallen to 3/4 of the last list's, so the long tail of
+  // running systems has fallen to 7/8 of the last list's, so the long tail of
   // a few slow systems does not launch workgroups for all the finished ones
-  // (3/4 measured 1-1.5 % faster than 1/2 on 2048 systems; same results).
+  // (on 2048 systems 3/4 measured 1-1.5 % faster than 1/2, and 7/8 another
+  // 0-1 %; same results).
   auto rounds_per_replay = [&](double sum_n2) {
     int k = opt->rounds_per_launch;
     if (k <= 0) {
@@ -550,7 +551,7 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
   long launched = 0;
   int slot = 0;
   bool have_prev = false;
-  int compact_below = n_systems * 3 / 4;
+  int compact_below = n_systems * 7 / 8;
   std::vector<SolverState> hst;
   for (;;) {
     HIP_TRY(ctx, hipGraphLaunch(w->exec, s));
@@ -602,7 +603,7 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
         set_tile_cfg(&rb, run_cfg, static_cast<int>(tiles.size()));
         K = rounds_per_replay(sum_n2);
         if ((rc = capture(rb, K))) return rc;
-        compact_below = n_live * 3 / 4;
+        compact_below = n_live * 7 / 8;
         have_prev = false;
         slot = 0;
         continue;

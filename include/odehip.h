@@ -113,10 +113,9 @@ int odehip_set_stream(odehip_context* ctx, void* hip_stream);
  * counterpart: the reference has only FP64 pair terms, NBodyODE.hpp:89-98).
  * Pairs with FP32 r^2 < c / n are computed and summed in FP64; c = 0 gives
  * all pairs in FP32.  c must be finite and >= 0; the default is
- * ODEHIP_MIXED_NEAR_DEFAULT.  Applies to every later single-system
- * mixed-variant call (RHS, acc, solve, leapfrog) on the context; the
- * ensemble entry points (odehip_ensemble_*) compute every mixed pair in
- * FP32 and ignore it. */
+ * ODEHIP_MIXED_NEAR_DEFAULT.  Applies to every later mixed-variant call on
+ * the context: single systems (RHS, acc, solve, leapfrog) and ensembles
+ * (odehip_ensemble_rhs / _solve, with n each system's own size). */
 int odehip_set_near_field(odehip_context* ctx, double coefficient);
 int odehip_get_near_field(const odehip_context* ctx, double* coefficient);
 int odehip_synchronize(odehip_context* ctx);

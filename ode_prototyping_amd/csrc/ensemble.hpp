@@ -54,6 +54,7 @@ struct EnsBufs {
   int wpb;         // waves per RHS workgroup
   int tpl2_min_n;  // systems with n >= this use 128-target tiles
   double eps2;
+  double near_c;  // mixed variant: pairs with FP32 r^2 < near_c / n in FP64
   const double* dense_t;  // DOPRI5 dense output: n_dense times, or null
   double* dense_out;      // n_dense x nv (packed rows)
   long n_dense;

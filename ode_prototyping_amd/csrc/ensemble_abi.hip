@@ -14,6 +14,7 @@
 
 #include "context.hpp"
 #include "ensemble.hpp"
+#include "ensemble_plan.hpp"
 #include "mlp.hpp"
 
 namespace odehip {
@@ -30,14 +31,13 @@ struct EnsembleWorkspace {
   EnsBufs graph_bufs{};
   int graph_k = 0;
   // tables currently on the device
-  std::vector<int> table_sizes;
-  int table_cfg = -2;
-  const void* table_base = nullptr;
+  TableKey tables;
   // autotuner cache
   std::vector<int> tuned_sizes;
   int tuned_variant = -1;
   bool tuned_mass = false;
   double tuned_eps2 = -1.0;
+  double tuned_near = -1.0;
   int tuned_cfg = -1;
   double tuned_ms[ODEHIP_ENS_TILE_CONFIGS] = {};
 };
@@ -61,137 +61,10 @@ using namespace odehip;
 
 namespace {
 
-// Host-side layout of an ensemble.
-struct Layout {
-  std::vector<EnsSys> sys;
-  std::vector<int2> groups;  // (system, group id) of every reduction group
-  std::vector<int> sizes;
-  size_t nv = 0;
-  int n_groups = 0;
-  double interactions = 0.0;
-  double work = 0.0;  // sum n^2
-};
-
 int make_layout(odehip_context* ctx, int n_systems, const int* n_particles,
                 Layout* L) {
-  if (n_systems <= 0) return fail(ctx, ODEHIP_ERR_INVALID, "n_systems must be positive");
-  if (!n_particles) return fail(ctx, ODEHIP_ERR_INVALID, "n_particles is null");
-  L->sys.resize(n_systems);
-  L->sizes.assign(n_particles, n_particles + n_systems);
-  long eoff = 0;
-  long groups = 0;
-  for (int s = 0; s < n_systems; ++s) {
-    const int n = n_particles[s];
-    if (n < 1 || n > (1 << 24))
-      return fail(ctx, ODEHIP_ERR_INVALID, "system sizes must be in [1, 2^24]");
-    const int ng = n_groups_for(6 * static_cast<size_t>(n));
-    L->sys[s] = EnsSys{eoff, n, static_cast<int>(groups), ng, 0};
-    eoff += 6L * n;
-    groups += ng;
-    L->interactions += static_cast<double>(n) * (n - 1);
-    L->work += static_cast<double>(n) * n;
-  }
-  if (groups > (1L << 30)) return fail(ctx, ODEHIP_ERR_INVALID, "ensemble too large");
-  L->nv = static_cast<size_t>(eoff);
-  L->n_groups = static_cast<int>(groups);
-  L->groups.resize(groups);
-  for (int s = 0; s < n_systems; ++s)
-    for (int g = 0; g < L->sys[s].n_groups; ++g) {
-      const int id = L->sys[s].gbase + g;
-      L->groups[id] = int2{s, id};
-    }
-  return ODEHIP_OK;
-}
-
-// Right-hand-side work items for one tile configuration: tiles of the
-// largest systems first.  With xcd placement the systems are dealt to eight
-// queues of about equal sum n^2 and workgroup b takes its tiles from queue
-// b mod 8, the XCD that the hardware dispatches workgroup b to, so all tiles
-// of one system share one XCD's L2.
-std::vector<EnsTile> make_tiles(const Layout& L, const EnsTileConfig& cfg,
-                                const std::vector<char>* running = nullptr) {
-  const int n_sys = static_cast<int>(L.sys.size());
-  std::vector<int> order;
-  for (int s = 0; s < n_sys; ++s)
-    if (!running || (*running)[s]) order.push_back(s);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return L.sys[a].n > L.sys[b].n; });
-  auto tiles_of = [&](int s, std::vector<EnsTile>& out) {
-    const int n = L.sys[s].n;
-    const int step = 64 * (n >= cfg.tpl2_min_n ? 2 : 1);
-    for (int i0 = 0; i0 < n; i0 += step) out.push_back(EnsTile{s, i0});
-  };
-  std::vector<EnsTile> tiles;
-  const EnsTile empty{-1, 0};
-  if (!cfg.xcd) {
-    for (int s : order) tiles_of(s, tiles);
-    while (tiles.size() % cfg.wpb) tiles.push_back(empty);
-    return tiles;
-  }
-  constexpr int kXcds = 8;
-  std::vector<std::vector<EnsTile>> q(kXcds);
-  double load[kXcds] = {};
-  for (int s : order) {
-    const int x = static_cast<int>(std::min_element(load, load + kXcds) - load);
-    load[x] += static_cast<double>(L.sys[s].n) * L.sys[s].n;
-    tiles_of(s, q[x]);
-  }
-  size_t rounds = 0;
-  for (auto& v : q) rounds = std::max(rounds, (v.size() + cfg.wpb - 1) / cfg.wpb);
-  tiles.reserve(rounds * kXcds * cfg.wpb);
-  for (size_t r = 0; r < rounds; ++r)
-    for (int x = 0; x < kXcds; ++x)
-      for (int w = 0; w < cfg.wpb; ++w) {
-        const size_t idx = r * cfg.wpb + w;
-        tiles.push_back(idx < q[x].size() ? q[x][idx] : empty);
-      }
-  return tiles;
-}
-
-size_t max_tiles(const Layout& L) {
-  size_t m = 0;
-  for (const auto& cfg : kEnsTileConfigs) m = std::max(m, make_tiles(L, cfg).size());
-  return m;
-}
-
-// Device arena: tables, per-system state, vectors.
-struct Plan {
-  size_t sys, groups, groups_act, tiles, st, n_active, xb0, xb1, ks, ytmp, kfsal,
-      sc0, partA, partB, trace, dense_t, dense_out, total;
-};
-
-Plan plan(const Layout& L, size_t tiles_cap, int S, long trace_cap,
-          bool with_state, long n_dense = 0) {
-  Plan p{};
-  size_t off = 0;
-  auto take = [&](size_t bytes) {
-    const size_t at = off;
-    off += (bytes + 255) & ~static_cast<size_t>(255);
-    return at;
-  };
-  const size_t ns = L.sys.size();
-  p.sys = take(ns * sizeof(EnsSys));
-  p.groups = take(L.groups.size() * sizeof(int2));
-  p.groups_act = with_state ? take(L.groups.size() * sizeof(int2)) : p.groups;
-  p.tiles = take(tiles_cap * sizeof(EnsTile));
-  p.st = take(ns * sizeof(SolverState));
-  p.n_active = take(sizeof(int));
-  const size_t v = L.nv * sizeof(double);
-  if (with_state) {
-    p.xb0 = take(v);
-    p.xb1 = take(v);
-    p.ks = take(static_cast<size_t>(S) * v);
-    p.ytmp = take(v);
-    p.kfsal = take(v);
-    p.sc0 = take(v);
-    p.partA = take(L.groups.size() * sizeof(double));
-    p.partB = take(L.groups.size() * sizeof(double));
-    p.trace = take(ns * static_cast<size_t>(trace_cap) * 3 * sizeof(double));
-    p.dense_t = take(static_cast<size_t>(n_dense) * sizeof(double));
-    p.dense_out = take(static_cast<size_t>(n_dense) * v);
-  }
-  p.total = off;
-  return p;
+  const char* why = build_layout(n_systems, n_particles, L);
+  return why ? fail(ctx, ODEHIP_ERR_INVALID, why) : ODEHIP_OK;
 }
 
 int workspace(odehip_context* ctx, size_t bytes, EnsembleWorkspace** out) {
@@ -214,7 +87,7 @@ int workspace(odehip_context* ctx, size_t bytes, EnsembleWorkspace** out) {
     if (w->dev) HIP_TRY(ctx, hipFree(w->dev));
     w->dev = nullptr;
     w->cap = 0;
-    w->table_cfg = -2;
+    w->tables.invalidate();
     HIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&w->dev), bytes));
     w->cap = bytes;
   }
@@ -223,13 +96,15 @@ int workspace(odehip_context* ctx, size_t bytes, EnsembleWorkspace** out) {
 }
 
 // Uploads the system/group tables and the tile list of `cfg` (skipped when
-// the device already holds them).
+// the device already holds exactly these tables at these offsets).
 int upload_tables(odehip_context* ctx, EnsembleWorkspace* w, const Plan& p,
-                  const Layout& L, int cfg, int* n_tiles, hipStream_t s) {
+                  const Layout& L, size_t tiles_cap, int cfg, int* n_tiles,
+                  hipStream_t s) {
   const std::vector<EnsTile> tiles = make_tiles(L, kEnsTileConfigs[cfg]);
+  if (const char* why = check_tiles(tiles, L, tiles_cap))
+    return fail(ctx, ODEHIP_ERR_INVALID, why);
   *n_tiles = static_cast<int>(tiles.size());
-  if (w->table_cfg == cfg && w->table_base == w->dev && w->table_sizes == L.sizes)
-    return ODEHIP_OK;
+  if (w->tables.matches(cfg, w->dev, p, L)) return ODEHIP_OK;
   HIP_TRY(ctx, hipStreamSynchronize(s));  // no kernel may still read them
   HIP_TRY(ctx, hipMemcpy(w->dev + p.sys, L.sys.data(), L.sys.size() * sizeof(EnsSys),
                          hipMemcpyHostToDevice));
@@ -237,9 +112,7 @@ int upload_tables(odehip_context* ctx, EnsembleWorkspace* w, const Plan& p,
                          L.groups.size() * sizeof(int2), hipMemcpyHostToDevice));
   HIP_TRY(ctx, hipMemcpy(w->dev + p.tiles, tiles.data(), tiles.size() * sizeof(EnsTile),
                          hipMemcpyHostToDevice));
-  w->table_cfg = cfg;
-  w->table_base = w->dev;
-  w->table_sizes = L.sizes;
+  w->tables.set(cfg, w->dev, p, L);
   return ODEHIP_OK;
 }
 
@@ -276,30 +149,10 @@ void set_tile_cfg(EnsBufs* b, int cfg, int n_tiles) {
 }
 
 bool tune_cache_hit(const EnsembleWorkspace* w, const Layout& L, int variant,
-                    bool mass, double eps2) {
+                    bool mass, double eps2, double near_c) {
   return w->tuned_cfg >= 0 && w->tuned_variant == variant && w->tuned_mass == mass &&
-         std::memcmp(&w->tuned_eps2, &eps2, sizeof eps2) == 0 && w->tuned_sizes == L.sizes;
-}
-
-// Tiles that hold targets (empty padding tiles not counted): the waves a
-// full-ensemble RHS launch puts to work under `cfg`.
-long real_tiles(const Layout& L, const EnsTileConfig& cfg) {
-  long t = 0;
-  for (const auto& sy : L.sys) {
-    const int step = 64 * (sy.n >= cfg.tpl2_min_n ? 2 : 1);
-    t += (sy.n + step - 1) / step;
-  }
-  return t;
-}
-
-// The same configuration with 64-target tiles for every system.
-int without_tpl2(int cfg) {
-  for (int c = 0; c < ODEHIP_ENS_TILE_CONFIGS; ++c)
-    if (kEnsTileConfigs[c].wpb == kEnsTileConfigs[cfg].wpb &&
-        kEnsTileConfigs[c].xcd == kEnsTileConfigs[cfg].xcd &&
-        kEnsTileConfigs[c].tpl2_min_n == kEnsNoTpl2)
-      return c;
-  return cfg;
+         std::memcmp(&w->tuned_eps2, &eps2, sizeof eps2) == 0 &&
+         std::memcmp(&w->tuned_near, &near_c, sizeof near_c) == 0 && w->tuned_sizes == L.sizes;
 }
 
 // Times one right-hand side of the state `y` into `k` for every candidate
@@ -309,13 +162,13 @@ int without_tpl2(int cfg) {
 // 64-target ones, but they halve the waves once compaction leaves a few
 // large systems running.
 int autotune(odehip_context* ctx, EnsembleWorkspace* w, const Plan& p,
-             const Layout& L, EnsBufs b, const double* y, double* k,
-             hipStream_t s) {
+             const Layout& L, size_t tiles_cap, EnsBufs b, const double* y,
+             double* k, hipStream_t s) {
   double best = 1e300;
   int best_cfg = kEnsDefaultTileConfig;
   for (int c = 0; c < ODEHIP_ENS_TILE_CONFIGS; ++c) {
     int n_tiles = 0;
-    int rc = upload_tables(ctx, w, p, L, c, &n_tiles, s);
+    int rc = upload_tables(ctx, w, p, L, tiles_cap, c, &n_tiles, s);
     if (rc) return rc;
     set_tile_cfg(&b, c, n_tiles);
     HIP_TRY(ctx, ens_launch_rhs(b, y, k, 0, s));
@@ -373,22 +226,25 @@ int odehip_ensemble_rhs(odehip_context* ctx, int n_systems,
     return fail(ctx, ODEHIP_ERR_INVALID, "bad tile_config");
   Layout L;
   if ((rc = make_layout(ctx, n_systems, n_particles, &L))) return rc;
-  const size_t tcap = max_tiles(L);
+  const size_t tcap = tiles_capacity(L);
   const Plan p = plan(L, tcap, 0, 0, false);
   EnsembleWorkspace* w = nullptr;
   if ((rc = workspace(ctx, p.total, &w))) return rc;
   const double eps2 = softening * softening;
+  const double near_c = nbody_variant == ODEHIP_NBODY_MIXED ? ctx->near_c : 0.0;
   int cfg = tile_config;
   if (cfg == ODEHIP_ENS_TILE_AUTO)
-    cfg = tune_cache_hit(w, L, nbody_variant, masses != nullptr, eps2) ? w->tuned_cfg
-                                                                      : kEnsDefaultTileConfig;
+    cfg = tune_cache_hit(w, L, nbody_variant, masses != nullptr, eps2, near_c)
+              ? w->tuned_cfg
+              : kEnsDefaultTileConfig;
   hipStream_t s = begin_work(ctx);
   int n_tiles = 0;
-  if ((rc = upload_tables(ctx, w, p, L, cfg, &n_tiles, s))) return rc;
+  if ((rc = upload_tables(ctx, w, p, L, tcap, cfg, &n_tiles, s))) return rc;
   EnsBufs b = make_bufs(w, p, L, false);
   b.mass = masses;
   b.variant = nbody_variant;
   b.eps2 = eps2;
+  b.near_c = near_c;
   set_tile_cfg(&b, cfg, n_tiles);
   HIP_TRY(ctx, hipEventRecord(w->t0, s));
   HIP_TRY(ctx, ens_launch_rhs(b, x, dxdt, 0, s));
@@ -448,11 +304,12 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
   if ((rc = make_layout(ctx, n_systems, n_particles, &L))) return rc;
   const int S = kTableaus[tableau].S;
   const long trace_cap = opt->trace ? opt->trace_cap : 0;
-  const size_t tcap = max_tiles(L);
+  const size_t tcap = tiles_capacity(L);
   const Plan p = plan(L, tcap, S, trace_cap, true, n_dense);
   EnsembleWorkspace* w = nullptr;
   if ((rc = workspace(ctx, p.total, &w))) return rc;
   const double eps2 = softening * softening;
+  const double near_c = nbody_variant == ODEHIP_NBODY_MIXED ? ctx->near_c : 0.0;
   hipStream_t s = begin_work(ctx);
 
   EnsBufs b = make_bufs(w, p, L, true);
@@ -464,6 +321,7 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
   b.tableau = tableau;
   b.variant = nbody_variant;
   b.eps2 = eps2;
+  b.near_c = near_c;
   b.controller = opt->controller;
   b.mlp = opt->controller == ODEHIP_CONTROLLER_LEARNED ? mlp_infer_ptr(opt->mlp) : nullptr;
   if (n_dense > 0) {
@@ -478,25 +336,26 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
   // tile configuration
   int cfg = opt->tile_config;
   if (cfg == ODEHIP_ENS_TILE_AUTO) {
-    if (!tune_cache_hit(w, L, nbody_variant, masses != nullptr, eps2)) {
+    if (!tune_cache_hit(w, L, nbody_variant, masses != nullptr, eps2, near_c)) {
       std::fill(w->tuned_ms, w->tuned_ms + ODEHIP_ENS_TILE_CONFIGS, 0.0);
       EnsBufs tb = b;
       tb.st = nullptr;  // plain evaluations of x0 into the k1 slot
-      if ((rc = autotune(ctx, w, p, L, tb, x0, b.ks, s))) return rc;
+      if ((rc = autotune(ctx, w, p, L, tcap, tb, x0, b.ks, s))) return rc;
       w->tuned_variant = nbody_variant;
       w->tuned_mass = masses != nullptr;
       w->tuned_eps2 = eps2;
+      w->tuned_near = near_c;
       w->tuned_sizes = L.sizes;
     }
     cfg = w->tuned_cfg;
   }
-  const bool tuned = tune_cache_hit(w, L, nbody_variant, masses != nullptr, eps2);
+  const bool tuned = tune_cache_hit(w, L, nbody_variant, masses != nullptr, eps2, near_c);
   if (opt->tune_ms) {
     for (int c = 0; c < ODEHIP_ENS_TILE_CONFIGS; ++c)
       opt->tune_ms[c] = tuned ? w->tuned_ms[c] : 0.0;
   }
   int n_tiles = 0;
-  if ((rc = upload_tables(ctx, w, p, L, cfg, &n_tiles, s))) return rc;
+  if ((rc = upload_tables(ctx, w, p, L, tcap, cfg, &n_tiles, s))) return rc;
   set_tile_cfg(&b, cfg, n_tiles);
 
   HIP_TRY(ctx, ens_enqueue_init(b, x0, opt->t0, opt->tf, opt->max_accepted,
@@ -593,11 +452,13 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
             tiles = make_tiles(L, kEnsTileConfigs[run_cfg], &live);
           }
         }
+        if (const char* why = check_tiles(tiles, L, tcap))
+          return fail(ctx, ODEHIP_ERR_INVALID, why);
         HIP_TRY(ctx, hipMemcpy(w->dev + p.groups_act, act.data(), act.size() * sizeof(int2),
                                hipMemcpyHostToDevice));
         HIP_TRY(ctx, hipMemcpy(w->dev + p.tiles, tiles.data(), tiles.size() * sizeof(EnsTile),
                                hipMemcpyHostToDevice));
-        w->table_cfg = -2;  // the tile region now holds the running systems' tiles
+        w->tables.invalidate();  // the tile region now holds the running systems' tiles
         rb.groups = reinterpret_cast<const int2*>(w->dev + p.groups_act);
         rb.n_groups = static_cast<int>(act.size());
         set_tile_cfg(&rb, run_cfg, static_cast<int>(tiles.size()));

@@ -232,12 +232,14 @@ __device__ inline void fast_tile(const double* __restrict__ y,
 }
 
 // mixed: FP64 displacement rounded to FP32, FP32 pair force, partial sums
-// in FP32 flushed into FP64 every 32 partners.
-template <bool kMass, bool kGuard, int TPL>
+// in FP32 flushed into FP64 every 32 partners.  kNear: near pairs (FP32
+// r^2 < near2) are left out of the FP32 sums and their FP64 term is added
+// into the FP64 sums; without it every pair is FP32.
+template <bool kMass, bool kGuard, bool kNear, int TPL>
 __device__ inline void mixed_tile(const double* __restrict__ y,
                                   const double* __restrict__ mass,
                                   double* __restrict__ k, int n, int i0,
-                                  double eps2) {
+                                  double eps2, float near2) {
   constexpr int kFlush = 32;
   constexpr int U = 8;
   const int lane = threadIdx.x & 63;
@@ -261,24 +263,81 @@ __device__ inline void mixed_tile(const double* __restrict__ y,
     float fx[TPL], fy[TPL], fz[TPL];
 #pragma unroll
     for (int t = 0; t < TPL; ++t) fx[t] = fy[t] = fz[t] = 0.0f;
-    for (; j + U <= jstop; j += U) {
-      const uniform_ptr pp = yu + 3 * j;
+    if constexpr (!kNear) {
+      for (; j + U <= jstop; j += U) {
+        const uniform_ptr pp = yu + 3 * j;
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const float mj = kMass ? static_cast<float>(mu[j + u]) : 1.0f;
+        for (int u = 0; u < U; ++u) {
+          const float mj = kMass ? static_cast<float>(mu[j + u]) : 1.0f;
+#pragma unroll
+          for (int t = 0; t < TPL; ++t)
+            mixed_term_fp32<kMass, kGuard>(pp[3 * u], pp[3 * u + 1], pp[3 * u + 2], mj,
+                                           j + u, ic[t], xi[t], yi[t], zi[t], eps2f, fx[t],
+                                           fy[t], fz[t]);
+        }
+      }
+      for (; j < jstop; ++j) {
+        const float mj = kMass ? static_cast<float>(mu[j]) : 1.0f;
 #pragma unroll
         for (int t = 0; t < TPL; ++t)
-          mixed_term_fp32<kMass, kGuard>(pp[3 * u], pp[3 * u + 1], pp[3 * u + 2], mj,
-                                         j + u, ic[t], xi[t], yi[t], zi[t], eps2f, fx[t],
-                                         fy[t], fz[t]);
+          mixed_term_fp32<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j,
+                                         ic[t], xi[t], yi[t], zi[t], eps2f, fx[t], fy[t],
+                                         fz[t]);
       }
-    }
-    for (; j < jstop; ++j) {
-      const float mj = kMass ? static_cast<float>(mu[j]) : 1.0f;
+    } else {
+      for (; j + U <= jstop; j += U) {
+        const uniform_ptr pp = yu + 3 * j;
+        // bit u: some lane has partner j + u in the near field of one of its targets
+        uint32_t near = 0;
 #pragma unroll
-      for (int t = 0; t < TPL; ++t)
-        mixed_term_fp32<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j,
-                                       ic[t], xi[t], yi[t], zi[t], eps2f, fx[t], fy[t], fz[t]);
+        for (int u = 0; u < U; ++u) {
+          const double mj = kMass ? mu[j + u] : 1.0;
+          uint64_t lanes = 0;
+#pragma unroll
+          for (int t = 0; t < TPL; ++t)
+            lanes |= ballot_not(mixed_term<kMass, kGuard>(pp[3 * u], pp[3 * u + 1],
+                                                          pp[3 * u + 2], mj, j + u, ic[t],
+                                                          xi[t], yi[t], zi[t], eps2f, near2,
+                                                          fx[t], fy[t], fz[t]));
+          near |= lanes ? (1u << u) : 0u;
+        }
+#pragma unroll
+        for (int t = 0; t < TPL; ++t) {
+          fx[t] = opaque(fx[t]);
+          fy[t] = opaque(fy[t]);
+          fz[t] = opaque(fz[t]);
+        }
+        if (near) {
+          double xo[TPL], yo[TPL], zo[TPL];
+#pragma unroll
+          for (int t = 0; t < TPL; ++t) {
+            xo[t] = opaque(xi[t]);
+            yo[t] = opaque(yi[t]);
+            zo[t] = opaque(zi[t]);
+          }
+          do {
+            const int u = __builtin_ctz(near);
+            near &= near - 1;
+            const double mj = kMass ? mu[j + u] : 1.0;
+#pragma unroll
+            for (int t = 0; t < TPL; ++t)
+              mixed_near_term<kMass, kGuard>(pp[3 * u], pp[3 * u + 1], pp[3 * u + 2], mj,
+                                             j + u, ic[t], xo[t], yo[t], zo[t], eps2f, eps2,
+                                             near2, ax[t], ay[t], az[t]);
+          } while (near);
+        }
+      }
+      for (; j < jstop; ++j) {
+        const double mj = kMass ? mu[j] : 1.0;
+#pragma unroll
+        for (int t = 0; t < TPL; ++t)
+          if (!mixed_term<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j,
+                                         ic[t], xi[t], yi[t], zi[t], eps2f, near2, fx[t],
+                                         fy[t], fz[t]))
+            mixed_near_term<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j,
+                                           ic[t], xi[t], yi[t], zi[t], eps2f, eps2, near2,
+                                           ax[t], ay[t], az[t]);
+      }
     }
 #pragma unroll
     for (int t = 0; t < TPL; ++t) {
@@ -292,8 +351,8 @@ __device__ inline void mixed_tile(const double* __restrict__ y,
 }
 
 // One wave per tile slot.  kSpecial: shared reciprocal (exact) or the
-// diagonal guard (fast, mixed).
-template <int kVariant, bool kMass, bool kSpecial, int WPB>
+// diagonal guard (fast, mixed); kNear: the mixed variant's near field.
+template <int kVariant, bool kMass, bool kSpecial, bool kNear, int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_ens_rhs(EnsBufs b,
                                                       const double* y_in,
                                                       double* k_out,
@@ -302,7 +361,7 @@ __global__ __launch_bounds__(64 * WPB) void k_ens_rhs(EnsBufs b,
   const int slot = blockIdx.x * WPB + wave;
   if (slot >= b.n_tiles) return;
   const EnsTile tile = b.tiles[slot];
-  if (tile.sys < 0) return;
+  if (tile.sys < 0 || tile.sys >= b.n_sys) return;
   const SolverState* st = b.st ? b.st + tile.sys : nullptr;
   // stage0 == 2: an evaluation of the dense-output pass, for the systems
   // whose pass runs this round (finished or not)
@@ -340,10 +399,11 @@ __global__ __launch_bounds__(64 * WPB) void k_ens_rhs(EnsBufs b,
     else
       fast_tile<kMass, kSpecial, 1>(y, m, k, n, tile.i0, b.eps2);
   } else {
+    const float near2 = kNear ? mixed_near2(b.near_c, n) : 0.0f;
     if (tpl == 2)
-      mixed_tile<kMass, kSpecial, 2>(y, m, k, n, tile.i0, b.eps2);
+      mixed_tile<kMass, kSpecial, kNear, 2>(y, m, k, n, tile.i0, b.eps2, near2);
     else
-      mixed_tile<kMass, kSpecial, 1>(y, m, k, n, tile.i0, b.eps2);
+      mixed_tile<kMass, kSpecial, kNear, 1>(y, m, k, n, tile.i0, b.eps2, near2);
   }
 }
 
@@ -556,16 +616,16 @@ __global__ __launch_bounds__(kBlock) void k_ens_copy_out(EnsBufs b,
   out[p.e] = (b.st[p.sys].cur ? b.xb1 : b.xb0)[p.e];
 }
 
-template <int kVariant, bool kMass, bool kSpecial>
+template <int kVariant, bool kMass, bool kSpecial, bool kNear>
 hipError_t launch_rhs_v(const EnsBufs& b, const double* y, double* k,
                         int stage0, hipStream_t s) {
   const unsigned blocks = static_cast<unsigned>(b.n_tiles / b.wpb);
   if (blocks == 0) return hipSuccess;
   if (b.wpb == 4) {
-    hipLaunchKernelGGL((k_ens_rhs<kVariant, kMass, kSpecial, 4>), dim3(blocks),
+    hipLaunchKernelGGL((k_ens_rhs<kVariant, kMass, kSpecial, kNear, 4>), dim3(blocks),
                        dim3(256), 0, s, b, y, k, stage0);
   } else if (b.wpb == 1) {
-    hipLaunchKernelGGL((k_ens_rhs<kVariant, kMass, kSpecial, 1>), dim3(blocks),
+    hipLaunchKernelGGL((k_ens_rhs<kVariant, kMass, kSpecial, kNear, 1>), dim3(blocks),
                        dim3(64), 0, s, b, y, k, stage0);
   } else {
     return hipErrorInvalidValue;
@@ -573,14 +633,14 @@ hipError_t launch_rhs_v(const EnsBufs& b, const double* y, double* k,
   return hipGetLastError();
 }
 
-template <int kVariant>
+template <int kVariant, bool kNear = false>
 hipError_t launch_rhs_m(const EnsBufs& b, const double* y, double* k,
                         int stage0, hipStream_t s, bool special) {
   if (b.mass)
-    return special ? launch_rhs_v<kVariant, true, true>(b, y, k, stage0, s)
-                   : launch_rhs_v<kVariant, true, false>(b, y, k, stage0, s);
-  return special ? launch_rhs_v<kVariant, false, true>(b, y, k, stage0, s)
-                 : launch_rhs_v<kVariant, false, false>(b, y, k, stage0, s);
+    return special ? launch_rhs_v<kVariant, true, true, kNear>(b, y, k, stage0, s)
+                   : launch_rhs_v<kVariant, true, false, kNear>(b, y, k, stage0, s);
+  return special ? launch_rhs_v<kVariant, false, true, kNear>(b, y, k, stage0, s)
+                 : launch_rhs_v<kVariant, false, false, kNear>(b, y, k, stage0, s);
 }
 
 inline unsigned groups_grid(const EnsBufs& b) {
@@ -601,9 +661,12 @@ hipError_t ens_launch_rhs(const EnsBufs& b, const double* y, double* k,
     }
     case ODEHIP_NBODY_FAST:
       return launch_rhs_m<ODEHIP_NBODY_FAST>(b, y, k, stage0, s, !(b.eps2 >= 1e-150));
-    case ODEHIP_NBODY_MIXED:
-      return launch_rhs_m<ODEHIP_NBODY_MIXED>(b, y, k, stage0, s,
-                                              !(static_cast<float>(b.eps2) >= 1e-12f));
+    case ODEHIP_NBODY_MIXED: {
+      const bool guard = !(static_cast<float>(b.eps2) >= 1e-12f);
+      return b.near_c > 0.0
+                 ? launch_rhs_m<ODEHIP_NBODY_MIXED, true>(b, y, k, stage0, s, guard)
+                 : launch_rhs_m<ODEHIP_NBODY_MIXED, false>(b, y, k, stage0, s, guard);
+    }
     default:
       return hipErrorInvalidValue;
   }

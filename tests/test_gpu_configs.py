@@ -1,0 +1,254 @@
+"""The HIP path at the size of every benchmark configuration, against the oracle.
+
+* C1 (N=64 uniform cube, DOPRI5) and C2 (N=16384 Plummer sphere, eps 1e-2, DOP853):
+  the fixed-step-count comparison with the reference's own single-threaded execution
+  (sequential reduction, the C library's pow; SingleThreadedExecutor.hpp:17-29,
+  RKEmbeddedParallel.hpp:58-79), and C2's device solve bit for bit against the oracle in
+  the device's execution (canonical reduction order, correctly rounded pow).
+* C3 (N=65536, two colliding Plummer spheres): exact RHS bit for bit, mixed RHS at the
+  FP32 bar, and a short mixed solve against the exact one at equal t.
+* C4 (2048 systems of 16-4096 bodies): the exact ensemble RHS bit for bit for every
+  system in the tile configurations the tuner picks, and a short ensemble solve bit for
+  bit per system on a sample spread over the sizes.
+* C5 (4096 held-out systems, DOP853, learned controller): the error norm still judges
+  every step, and the network's gain over the reference controller.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+THREADS = max(1, min(16, len(os.sched_getaffinity(0))))
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SEQUENTIAL = (O.REDUCE_SEQUENTIAL, 0)
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def rel_err(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def oracle(tableau, x0, tf, eps, m, reduce=O.HIP_ORDER, pow_mode=O.POW_CORRECT, **kw):
+    return O.integrate(tableau, O.RHS_NBODY, x0, 0.0, tf, 1e-10, 1e-10, softening=eps,
+                       masses=m, reduce=reduce, pow_mode=pow_mode, store_trace=True,
+                       threads=THREADS, **kw)
+
+
+def same_decisions(trace_acc, ref):
+    """Accept/reject identical wherever the reference's error norm is not within 1e-12
+    of 1 (the north_star's exemption for a norm on the threshold)."""
+    k = min(len(trace_acc), len(ref.trace_acc))
+    decided = np.abs(ref.trace_err[:k] - 1.0) > 1e-12
+    return bool(np.array_equal(np.asarray(trace_acc[:k]).astype(int)[decided],
+                               ref.trace_acc[:k][decided]))
+
+
+def test_c2_plummer_16384_dop853_three_steps(exe):
+    """C2's workload (seed 0, a = 1, M = 1, masses 1/N, eps 1e-2, DOP853, rtol = atol =
+    1e-10) for 3 accepted steps: final state, accepted times, per-attempt trace and
+    counters bit for bit against the oracle; against the reference's single-threaded
+    execution <= 1e-12 relative with the same accept/reject sequence."""
+    import ode_prototyping_amd as oh
+    n, steps = 16384, 3
+    x0, m = oh.scenarios.plummer(n, seed=0)
+    print(f"C2: N={n} DOP853 {steps} steps", flush=True)
+    res = oh.integrate("DOP853", x0, 0.0, 100.0, 1e-10, 1e-10, oh.NBodyODE(1e-2, m, "exact"),
+                       exe=exe, max_accepted=steps, trace_cap=64)
+    ref = oracle("DOP853", x0, 100.0, 1e-2, m, max_accepted=steps, store_states=True)
+    assert res.status == "max_accepted" and res.accepted == steps
+    assert (res.accepted, res.rejected, res.rhs_evals) == (ref.accepted, ref.rejected,
+                                                           ref.rhs_evals)
+    assert res.rhs_evals == 2 + 12 * (res.accepted + res.rejected)
+    assert np.array_equal(bits(res.times), bits(ref.times))
+    assert np.array_equal(bits(res.x_final), bits(ref.x_final))
+    assert np.array_equal(bits(res.trace[:, 0]), bits(ref.trace_err))
+    assert np.array_equal(bits(res.trace[:, 1]), bits(ref.trace_dt))
+    assert np.array_equal(res.trace[:, 2].astype(int), ref.trace_acc)
+    st = oracle("DOP853", x0, 100.0, 1e-2, m, reduce=SEQUENTIAL, pow_mode=O.POW_LIBM,
+                max_accepted=steps, store_states=False)
+    rel = rel_err(res.x_final, st.x_final)
+    print(f"C2 vs single-threaded reference execution after {steps} steps: {rel:.3e}")
+    assert (st.accepted, st.rejected) == (res.accepted, res.rejected)
+    assert same_decisions(res.trace[:, 2], st)
+    assert rel <= 1e-12
+
+
+@pytest.mark.parametrize("steps", [50, 200])
+def test_c1_fixed_step_count_against_reference_execution(exe, steps):
+    """C1 (committed fixture, DOPRI5, eps 1e-3, masses 1/64) after a fixed accepted-step
+    count against the reference's single-threaded execution.  The accept/reject sequence
+    is the same; the states are not within 1e-12: the embedded error norm of this
+    close-encounter system carries ~1e-6 relative precision, so a last-bit difference of
+    one norm (reduction order) moves every later step time, and the states after N steps
+    sit at slightly different times.  The reference's own thread-pool executor
+    (ThreadPoolExecutor.hpp:81-114) differs from its single-threaded one by as much, so
+    the bar here is that spread (measured in this test), not 1e-12."""
+    import ode_prototyping_amd as oh
+    f = np.load(os.path.join(GOLDEN, "c1_uniform_cube_n64_seed0.npz"))
+    x0, m = f["x0"], f["masses"]
+    res = oh.integrate("DOPRI5", x0, 0.0, 100.0, 1e-10, 1e-10, oh.NBodyODE(1e-3, m, "exact"),
+                       exe=exe, max_accepted=steps, trace_cap=4 * steps + 16)
+    dev = oracle("DOPRI5", x0, 100.0, 1e-3, m, max_accepted=steps, store_states=False)
+    assert np.array_equal(bits(res.x_final), bits(dev.x_final))  # the device's execution
+    st = oracle("DOPRI5", x0, 100.0, 1e-3, m, reduce=SEQUENTIAL, pow_mode=O.POW_LIBM,
+                max_accepted=steps, store_states=False)
+    tp = oracle("DOPRI5", x0, 100.0, 1e-3, m, reduce=(O.REDUCE_THREADPOOL, 8),
+                pow_mode=O.POW_LIBM, max_accepted=steps, store_states=False)
+    rel, spread = rel_err(res.x_final, st.x_final), rel_err(tp.x_final, st.x_final)
+    print(f"C1 after {steps} steps vs single-threaded reference execution: {rel:.3e}; "
+          f"reference thread pool (8) vs single-threaded: {spread:.3e}; "
+          f"t {res.t!r} vs {st.t!r}")
+    assert (res.accepted, res.rejected) == (st.accepted, st.rejected)
+    assert same_decisions(res.trace[:, 2], st)
+    assert rel <= max(1e-12, 2.0 * spread)
+
+
+@pytest.fixture(scope="module")
+def c3():
+    import ode_prototyping_amd as oh
+    x0, m = oh.scenarios.colliding_plummer(65536, seed=0)
+    ref = O.rhs(O.RHS_NBODY, x0, 1e-2, m, threads=THREADS)
+    return x0, m, ref
+
+
+def test_c3_rhs_65536_exact_and_mixed(exe, c3):
+    """C3's input (two Plummer spheres of 32768 bodies at x = +-2 closing at 0.3,
+    eps 1e-2, masses 1/N): exact RHS bit for bit; mixed RHS within 2e-6 of the largest
+    acceleration (FP32 pair terms)."""
+    import ode_prototyping_amd as oh
+    x0, m, ref = c3
+    n = 65536
+    print(f"C3: N={n} RHS", flush=True)
+    xs, ks = oh.HipState(exe, data=x0), oh.HipState(exe, 6 * n)
+    oh.NBodyODE(1e-2, m, "exact")(exe, xs, ks)
+    assert np.array_equal(bits(ks.copy_to_span()), bits(ref))
+    oh.NBodyODE(1e-2, m, "mixed")(exe, xs, ks)
+    got = ks.copy_to_span()
+    assert np.array_equal(got[:3 * n], ref[:3 * n])
+    scale = np.max(np.abs(ref[3 * n:]))
+    err = np.max(np.abs(got[3 * n:] - ref[3 * n:])) / scale
+    print(f"C3 mixed RHS max error / max |a|: {err:.3e}")
+    assert err <= 2e-6
+
+
+def device_energy(exe, x, n, eps, m):
+    """Kinetic + potential energy on the device (odehip_nbody_energy, whose parity with
+    the oracle's is tested in test_gpu_rhs)."""
+    import ctypes as C
+    import ode_prototyping_amd as oh
+    xd, md = oh.HipState(exe, data=x), oh.HipState(exe, data=m)
+    out = np.zeros(2)
+    exe.check(oh.lib().odehip_nbody_energy(exe.handle, xd.ptr, n, eps, md.ptr,
+                                           out.ctypes.data_as(C.POINTER(C.c_double))),
+              "nbody_energy")
+    return float(out.sum())
+
+
+def test_c3_mixed_solve_against_exact_at_equal_t(exe, c3):
+    """Two accepted DOP853 steps of the mixed path on C3's input, then the exact path to
+    the same t (the last step clipped to it): states within 1e-9 relative, energies
+    within 1e-11."""
+    import ode_prototyping_amd as oh
+    x0, m, _ = c3
+    n = 65536
+    mixed = oh.integrate("DOP853", x0, 0.0, 100.0, 1e-10, 1e-10,
+                         oh.NBodyODE(1e-2, m, "mixed"), exe=exe, max_accepted=2)
+    t = mixed.t
+    exact = oh.integrate("DOP853", x0, 0.0, t, 1e-10, 1e-10, oh.NBodyODE(1e-2, m, "exact"),
+                         exe=exe)
+    assert exact.t == t and exact.status == "reached_end"
+    rel = rel_err(mixed.x_final, exact.x_final)
+    e_m = device_energy(exe, mixed.x_final, n, 1e-2, m)
+    e_x = device_energy(exe, exact.x_final, n, 1e-2, m)
+    print(f"C3 mixed vs exact at t={t:.6g}: state {rel:.3e}, energy {abs(e_m / e_x - 1):.3e}")
+    assert rel <= 1e-9
+    assert abs(e_m / e_x - 1.0) <= 1e-11
+
+
+@pytest.fixture(scope="module")
+def c4():
+    from ode_prototyping_amd import train_controller as trc
+    from ode_prototyping_amd.ensemble import unpack
+    sizes, x0, m = trc.workload_ensemble(2048, 0)
+    sizes = [int(v) for v in sizes]
+    xs, ms = unpack(x0, sizes), unpack(m, sizes, per_particle=1)
+    return sizes, x0, m, xs, ms
+
+
+def test_c4_ensemble_rhs_exact_every_system(exe, c4):
+    """The exact ensemble RHS of C4's 2048 systems (16-4096 bodies) equals the oracle's
+    RHS of each system alone bit for bit, in the configuration the tuner picked in the
+    benchmark (7: one wave per workgroup, XCD placement), with 128-target tiles from
+    1024 bodies (9, 2) and under the tuner itself."""
+    from ode_prototyping_amd.ensemble import ensemble_rhs, unpack
+    sizes, x0, m, xs, ms = c4
+    refs = [O.rhs(O.RHS_NBODY, xs[k], 1e-3, ms[k], threads=THREADS if sizes[k] > 512 else 1)
+            for k in range(len(sizes))]
+    for cfg in (7, 9, 2, -1):
+        print(f"C4 ensemble RHS, tile config {cfg}", flush=True)
+        d, ms_ = ensemble_rhs(exe, sizes, x0, 1e-3, m, "exact", tile_config=cfg)
+        bad = [k for k, part in enumerate(unpack(d, sizes))
+               if not np.array_equal(bits(part), bits(refs[k]))]
+        assert not bad, (cfg, bad[:5], [sizes[k] for k in bad[:5]])
+        assert ms_ > 0.0
+
+
+def test_c4_short_ensemble_solve_sample(exe, c4):
+    """C4's ensemble (DOPRI5, rtol = atol = 1e-10, eps 1e-3) solved to t = 0.002 in one
+    call: 24 systems spread over the size order (largest included) equal their own oracle
+    solves bit for bit (final state, counters, per-attempt trace)."""
+    from ode_prototyping_amd.ensemble import solve_ensemble
+    sizes, x0, m, xs, ms = c4
+    tf = 0.002
+    print("C4 ensemble solve to t=0.002", flush=True)
+    res = solve_ensemble("DOPRI5", sizes, x0, 0.0, tf, 1e-10, 1e-10, 1e-3, m,
+                         variant="exact", exe=exe, trace_cap=512)
+    assert res.n_unfinished == 0 and np.all(res.t == tf)
+    order = np.argsort(sizes, kind="stable")
+    sample = sorted(set(order[np.linspace(0, len(sizes) - 1, 24).astype(int)].tolist()))
+    finals = res.states()
+    for k in sample:
+        ref = oracle("DOPRI5", xs[k], tf, 1e-3, ms[k], store_states=False)
+        assert (res.accepted[k], res.rejected[k], res.rhs_evals[k]) == (
+            ref.accepted, ref.rejected, ref.rhs_evals), k
+        assert res.dt[k] == ref.dt and res.dt_initial[k] == ref.dt_initial
+        assert np.array_equal(bits(finals[k]), bits(ref.x_final)), (k, sizes[k])
+        tr = res.trace[k]
+        assert np.array_equal(bits(tr[:, 0]), bits(ref.trace_err))
+        assert np.array_equal(bits(tr[:, 1]), bits(ref.trace_dt))
+
+
+def test_c5_learned_controller_4096_held_out(exe):
+    """C5: the shipped DOP853 network on 4096 held-out systems (16-4096 bodies, seed
+    9000, mixed RHS, t in [0, 0.05]) against the reference controller: every accepted
+    attempt has error norm <= 1 and every rejected one > 1 (the embedded estimate judges
+    every step), every system reaches t = 0.05, the PI fallback sizes some attempts, and
+    the network needs fewer RHS evaluations."""
+    from ode_prototyping_amd import train_controller as trc
+    from ode_prototyping_amd.ensemble import solve_ensemble
+    net, info = trc.pretrained(exe, "DOP853")
+    sizes, x0, m = trc.workload_ensemble(4096, 9000)
+    runs = {}
+    for ctrl in ("reference", "learned"):
+        print(f"C5 4096 systems, controller {ctrl}", flush=True)
+        runs[ctrl] = solve_ensemble("DOP853", sizes, x0, 0.0, 0.05, 1e-10, 1e-10, 1e-3, m,
+                                    variant="mixed", exe=exe, controller=ctrl,
+                                    mlp=net if ctrl == "learned" else None, trace_cap=2048)
+    for ctrl, r in runs.items():
+        assert r.n_unfinished == 0 and np.all(r.t == 0.05)
+        for tr in r.trace:
+            assert np.all(tr[tr[:, 2] == 1, 0] <= 1.0) and np.all(tr[tr[:, 2] == 0, 0] > 1.0)
+    lr = runs["learned"]
+    assert 0 < lr.total_learned_steps < lr.total_attempts
+    gain = 1.0 - lr.total_rhs_evals / runs["reference"].total_rhs_evals
+    print(f"C5 RHS evals: reference {runs['reference'].total_rhs_evals}, learned "
+          f"{lr.total_rhs_evals} (gain {gain:.3f})")
+    assert gain > 0.05, gain
+    net.close()

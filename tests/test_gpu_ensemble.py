@@ -197,10 +197,6 @@ def test_ensemble_matches_single_solves_fsal(exe, ens):
     assert np.all(res.accepted <= 25)
 
 
-@pytest.mark.skip(reason="the ensemble near-field kernel is switched off: this test (config 0) "
-                  "ended in a GPU fault once, after passing twice, and no cause was found; "
-                  "the ensemble mixed RHS now computes every pair in FP32 and ignores the "
-                  "near field")
 @pytest.mark.parametrize("cfg", [0, 5, 10])
 def test_ensemble_rhs_mixed_all_near_is_fp64(exe, ens, cfg):
     """A near field covering every pair turns the ensemble's mixed RHS into
@@ -222,6 +218,72 @@ def test_ensemble_rhs_mixed_all_near_is_fp64(exe, ens, cfg):
         scale = np.sum(np.abs(r) * w[:, :, None], axis=1).ravel() + 1e-300
         err = np.abs(part[3 * n:] - ref[3 * n:]) / scale
         assert err.max() < 2e-14, (n, err.max())
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 7])
+def test_ensemble_rhs_after_solve_and_solve_after_rhs(exe, ens, cfg):
+    """A solve that never compacts (two rounds) and then a right-hand side of the same
+    systems in the same tile configuration, and the reverse order: each call reads the
+    tile list of its own configuration (the device tables are reused only when they sit
+    at the same arena offsets), and both results equal the oracle bit for bit."""
+    from ode_prototyping_amd.ensemble import ensemble_rhs, pack, solve_ensemble, unpack
+    xs, ms = ens
+    refs = [O.rhs(O.RHS_NBODY, x, 1e-3, m) for x, m in zip(xs, ms)]
+
+    def rhs_ok():
+        d, _ = ensemble_rhs(exe, SIZES, pack(xs), 1e-3, pack(ms), "exact", tile_config=cfg)
+        for k, part in enumerate(unpack(d, SIZES)):
+            assert np.array_equal(bits(part), bits(refs[k])), (cfg, SIZES[k])
+
+    def solve_ok():
+        res = solve_ensemble("DOPRI5", SIZES, pack(xs), 0.0, 1.0, 1e-10, 1e-10, 1e-3,
+                             pack(ms), variant="exact", exe=exe, tile_config=cfg,
+                             max_attempts=2, rounds_per_launch=2, trace_cap=8)
+        finals = res.states()
+        for k in range(len(SIZES)):
+            ref = oracle_solve("DOPRI5", xs[k], 1.0, 1e-10, 1e-3, ms[k], max_attempts=2)
+            assert np.array_equal(bits(finals[k]), bits(ref.x_final)), (cfg, SIZES[k])
+
+    solve_ok()
+    rhs_ok()
+    solve_ok()
+    rhs_ok()
+
+
+def test_ensemble_mixed_near_field_through_close_encounters(exe):
+    """The close-encounter systems of the single-system near-field test (uniform cubes of
+    28 and 52 bodies, seed n, eps 1e-3, DOPRI5, t in [0, 0.1]) solved together as one
+    ensemble with the mixed RHS: with the close-encounter near field every system takes
+    the FP64 oracle's number of RHS evaluations and ends within 5e-9 of its final state;
+    with every pair in FP32 at least one system needs more evaluations and ends >= 100x
+    further off."""
+    import ode_prototyping_amd as oh
+    from ode_prototyping_amd.ensemble import pack, solve_ensemble
+    sizes = [28, 52]
+    xs, ms = [], []
+    for n in sizes:
+        x, m = oh.scenarios.uniform_cube(n, seed=n)
+        xs.append(x)
+        ms.append(m)
+    refs = [oracle_solve("DOPRI5", xs[k], 0.1, 1e-10, 1e-3, ms[k]) for k in range(2)]
+    runs = {}
+    for c in (oh.MIXED_NEAR_CLOSE, 0.0):
+        exe.near_field = c
+        try:
+            res = solve_ensemble("DOPRI5", sizes, pack(xs), 0.0, 0.1, 1e-10, 1e-10, 1e-3,
+                                 pack(ms), variant="mixed", exe=exe)
+        finally:
+            exe.near_field = oh.MIXED_NEAR_DEFAULT
+        assert np.all(res.t == 0.1)
+        finals = res.states()
+        runs[c] = [(np.max(np.abs(finals[k] - refs[k].x_final)) /
+                    np.max(np.abs(refs[k].x_final)), int(res.rhs_evals[k])) for k in range(2)]
+        print(f"near field {c}: {runs[c]} (oracle {[r.rhs_evals for r in refs]})")
+    for k in range(2):
+        assert runs[oh.MIXED_NEAR_CLOSE][k][0] <= 5e-9
+        assert runs[oh.MIXED_NEAR_CLOSE][k][1] == refs[k].rhs_evals
+        assert runs[0.0][k][0] >= 100 * runs[oh.MIXED_NEAR_CLOSE][k][0]
+    assert any(runs[0.0][k][1] > refs[k].rhs_evals for k in range(2))
 
 
 @pytest.mark.parametrize("variant", ["fast", "mixed"])

@@ -36,7 +36,9 @@ extern "C" {
 #define ODEHIP_TABLEAU_DVERK 3
 #define ODEHIP_TABLEAU_RKF78 4
 #define ODEHIP_TABLEAU_DOP853 5
-#define ODEHIP_NUM_TABLEAUS 6
+#define ODEHIP_NUM_TABLEAUS 6   /* built-in ids: 0 .. ODEHIP_NUM_TABLEAUS - 1 */
+#define ODEHIP_MAX_TABLEAUS 32  /* registered ids: ODEHIP_NUM_TABLEAUS .. 31 */
+#define ODEHIP_MAX_STAGES 13
 
 /* Right-hand sides. */
 #define ODEHIP_RHS_NBODY 0       /* src/ode/NBodyODE.hpp:31-39 */
@@ -165,6 +167,28 @@ int odehip_nbody_acc(odehip_context* ctx, const double* pos, double* acc,
  * src/ode/VanDerPolODE.hpp:16-25). */
 int odehip_ode_rhs(odehip_context* ctx, int rhs_kind, const double* x,
                    double* dxdt, size_t n_var);
+
+/* A Butcher tableau in the reference's struct format
+ * (butcher_tableaus/BTDOPRI5.hpp:5-26: a, b, bt, p, pt, n_stages), registered
+ * for every context of the process.  Replaces the integrator's tableau type
+ * argument (RKEmbeddedParallel<ButcherTableau>, RKEmbeddedParallel.hpp:11-24):
+ * the returned id is taken wherever a tableau id is (stage, update, initial
+ * step, device and ensemble solves; dense output stays limited to the
+ * built-in DOPRI5 / DOP853).
+ *   a   (n_stages - 1) x (n_stages - 1), row major: row s - 1 holds the
+ *       weights of stage s (entries j >= s are ignored, zero in the
+ *       reference's arrays)
+ *   b   n_stages propagated weights (order p); bt n_stages embedded weights
+ *       (order pt); the error weights b - bt are formed in double as
+ *       RKEmbeddedParallel.hpp:108-114 does.
+ * 2 <= n_stages <= ODEHIP_MAX_STAGES, 1 <= p, pt <= 20, finite coefficients.
+ * Registering the same tableau again returns the same id; at most
+ * ODEHIP_MAX_TABLEAUS - ODEHIP_NUM_TABLEAUS different ones per process. */
+int odehip_tableau_register(odehip_context* ctx, int n_stages, int p, int pt,
+                            const double* a, const double* b, const double* bt,
+                            int* id);
+/* Stage count and orders of a built-in or registered tableau. */
+int odehip_tableau_info(int id, int* n_stages, int* p, int* pt);
 
 /* detail::rk_stage_kernel (RKEmbeddedParallel.hpp:91-101): for stage >= 1,
  * temp = x0 + dt * sum_{j<stage} a[stage-1][j] * ks[j*n_var + .]. */

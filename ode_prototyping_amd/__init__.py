@@ -17,13 +17,14 @@ ensembles (:mod:`.ensemble`), the learned step-size controller
 """
 from ._lib import (CONTROLLERS, MIXED_NEAR_CLOSE, MIXED_NEAR_DEFAULT, NBODY_VARIANTS,
                    REDUCE_GROUP, STAGES, STATUS, TABLEAUS, lib)
-from .core import (DenseOutput, ExponentialODE, HipExecutor, HipState, MostRecentOutput,
-                   NBodyODE, OdeHipError, RawOutput, SolveResult, VanDerPolODE, integrate,
-                   integrate_host_driven, leapfrog)
+from .core import (ButcherTableau, DenseOutput, ExponentialODE, HipExecutor, HipState,
+                   MostRecentOutput, NBodyODE, OdeHipError, RawOutput, SolveResult,
+                   VanDerPolODE, integrate, integrate_host_driven, leapfrog, tableau_id)
 from . import nbody_io, scenarios
 
 __all__ = [
-    "HipExecutor", "HipState", "NBodyODE", "ExponentialODE", "VanDerPolODE",
+    "HipExecutor", "HipState", "NBodyODE", "ExponentialODE", "VanDerPolODE", "ButcherTableau",
+    "tableau_id",
     "RawOutput", "MostRecentOutput", "DenseOutput", "SolveResult", "integrate",
     "integrate_host_driven", "leapfrog", "scenarios", "nbody_io", "TABLEAUS", "STAGES", "STATUS",
     "NBODY_VARIANTS", "REDUCE_GROUP", "lib",

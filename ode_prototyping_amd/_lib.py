@@ -15,6 +15,9 @@ ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED = -1, -2, -3, -4
 
 TABLEAUS = {"HE21": 0, "RKF45": 1, "DOPRI5": 2, "DVERK": 3, "RKF78": 4, "DOP853": 5}
 STAGES = {"HE21": 2, "RKF45": 6, "DOPRI5": 7, "DVERK": 8, "RKF78": 13, "DOP853": 12}
+ORDERS = {"HE21": (2, 1), "RKF45": (4, 5), "DOPRI5": (5, 4), "DVERK": (6, 5), "RKF78": (7, 8),
+          "DOP853": (8, 5)}  # (p, pt)
+MAX_STAGES = 13  # ODEHIP_MAX_STAGES
 RHS_NBODY, RHS_EXPONENTIAL, RHS_VANDERPOL = 0, 1, 2
 NBODY_VARIANTS = {"exact": 0, "fast": 1, "mixed": 2}
 MIXED_NEAR_DEFAULT = 0.0  # ODEHIP_MIXED_NEAR_DEFAULT
@@ -122,6 +125,10 @@ SIGNATURES = {
     "odehip_nbody_acc": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_size_t, C.c_double,
                                    C.c_void_p, C.c_int]),
     "odehip_ode_rhs": (C.c_int, [_ctxp, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "odehip_tableau_register": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp,
+                                          C.POINTER(C.c_int)]),
+    "odehip_tableau_info": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]),
     "odehip_rk_stage": (C.c_int, [_ctxp, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_size_t]),
     "odehip_rk_update_state_and_error": (C.c_int, [_ctxp, C.c_int, C.c_double, C.c_void_p,

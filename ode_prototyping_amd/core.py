@@ -33,6 +33,72 @@ class OdeHipError(RuntimeError):
     pass
 
 
+class ButcherTableau:
+    """A Butcher tableau in the reference's struct format
+    (``butcher_tableaus/BTDOPRI5.hpp:5-26``: ``a``, ``b``, ``bt``, ``p``, ``pt``,
+    ``n_stages``), usable wherever a built-in tableau name is.
+
+    ``a`` is (n_stages - 1) x (n_stages - 1), row s - 1 the weights of stage s
+    (entries j >= s ignored, zero in the reference's arrays); ``b`` (order ``p``)
+    and ``bt`` (order ``pt``) have n_stages entries.  The library registers it
+    (``odehip_tableau_register``) on first use; the same coefficients as a
+    built-in method give that method's id.
+    """
+
+    def __init__(self, a, b, bt, p, pt, name=None):
+        self.b = np.ascontiguousarray(b, dtype=np.float64).ravel()
+        self.bt = np.ascontiguousarray(bt, dtype=np.float64).ravel()
+        S = self.b.size
+        if not 2 <= S <= _lib.MAX_STAGES or self.bt.size != S:
+            raise ValueError("b and bt need the same length, 2 .. 13 stages")
+        rows = [list(r) for r in a]
+        if len(rows) != S - 1:
+            raise ValueError(f"a needs {S - 1} rows for {S} stages")
+        self.a = np.zeros((S - 1, S - 1))
+        for s_, r in enumerate(rows):
+            if len(r) < s_ + 1:
+                raise ValueError(f"row {s_} of a needs {s_ + 1} weights")
+            self.a[s_, :s_ + 1] = r[:s_ + 1]
+        self.p, self.pt = int(p), int(pt)
+        self.name = name or "registered"
+        self._id = None
+
+    @property
+    def n_stages(self):
+        return self.b.size
+
+    def id(self, exe):
+        """The library's tableau id (registered on the first call)."""
+        if self._id is None:
+            out = C.c_int(-1)
+            exe.check(lib().odehip_tableau_register(exe.handle, self.n_stages, self.p, self.pt,
+                                                    dptr(self.a), dptr(self.b), dptr(self.bt),
+                                                    C.byref(out)), "tableau_register")
+            self._id = out.value
+        return self._id
+
+    def __repr__(self):
+        return f"ButcherTableau({self.name}, stages={self.n_stages}, p={self.p}, pt={self.pt})"
+
+
+def tableau_id(exe, tableau):
+    """Library id of a built-in tableau name or a :class:`ButcherTableau`."""
+    if isinstance(tableau, ButcherTableau):
+        return tableau.id(exe)
+    if tableau not in TABLEAUS:
+        raise ValueError(f"unknown tableau {tableau}")
+    return TABLEAUS[tableau]
+
+
+def tableau_shape(tableau):
+    """(n_stages, p, pt) of a built-in tableau name or a :class:`ButcherTableau`."""
+    if isinstance(tableau, ButcherTableau):
+        return tableau.n_stages, tableau.p, tableau.pt
+    if tableau not in TABLEAUS:
+        raise ValueError(f"unknown tableau {tableau}")
+    return (STAGES[tableau],) + _lib.ORDERS[tableau]
+
+
 class HipExecutor:
     """HIP execution policy: one odehip context on one device."""
 
@@ -102,12 +168,12 @@ class HipExecutor:
 
     # -- reference kernels (one call per reference kernel launch) ------------
     def rk_stage(self, tableau, stage, dt, temp, ks, x0):
-        self.check(lib().odehip_rk_stage(self._h, TABLEAUS[tableau], stage, dt, temp.ptr,
+        self.check(lib().odehip_rk_stage(self._h, tableau_id(self, tableau), stage, dt, temp.ptr,
                                          ks.ptr, x0.ptr, x0.size), "rk_stage")
 
     def rk_update_state_and_error(self, tableau, dt, x, err, ks, x0):
         self.check(lib().odehip_rk_update_state_and_error(
-            self._h, TABLEAUS[tableau], dt, x.ptr, err.ptr, ks.ptr, x0.ptr, x0.size),
+            self._h, tableau_id(self, tableau), dt, x.ptr, err.ptr, ks.ptr, x0.ptr, x0.size),
             "rk_update_state_and_error")
 
     def rk_update_error_target(self, sc, atol, rtol, x, x0):
@@ -391,8 +457,7 @@ def integrate(tableau, x0, t0, tf, atol, rtol, ode, output=None, exe=None, *,
     :class:`DenseOutput` receives the initial state and its equally spaced
     points instead of the accepted steps.
     """
-    if tableau not in TABLEAUS:
-        raise ValueError(f"unknown tableau {tableau}")
+    tableau_shape(tableau)  # a known tableau (ValueError otherwise)
     if controller not in CONTROLLERS:
         raise ValueError(f"unknown controller {controller}")
     exe = exe or HipExecutor()
@@ -440,7 +505,7 @@ def integrate(tableau, x0, t0, tf, atol, rtol, ode, output=None, exe=None, *,
                             int(dt_arr.size) if dt_arr is not None else 0,
                             dense_buf.ctypes.data if dense_buf is not None else None)
     stats = _lib.SolveStats()
-    exe.check(lib().odehip_rk_solve(exe.handle, TABLEAUS[tableau], C.byref(ode_s),
+    exe.check(lib().odehip_rk_solve(exe.handle, tableau_id(exe, tableau), C.byref(ode_s),
                                     C.byref(opt), work.ptr, C.byref(stats)), "rk_solve")
     x_final = work.copy_to_span() if fetch else None
     res = SolveResult(stats, x_final, times[:stats.n_times].copy(),
@@ -473,9 +538,7 @@ def integrate_host_driven(tableau, x0, t0, tf, atol, rtol, ode, output, exe, *,
     CUDA executor does.  ``output.save_state`` is called for every accepted
     state.  Returns (accepted, rejected, rhs_evals).
     """
-    S = STAGES[tableau]
-    p, pt = {"HE21": (2, 1), "RKF45": (4, 5), "DOPRI5": (5, 4), "DVERK": (6, 5),
-             "RKF78": (7, 8), "DOP853": (8, 5)}[tableau]
+    S, p, pt = tableau_shape(tableau)
     x0s = x0.copy() if isinstance(x0, HipState) else HipState(exe, data=x0)
     n = x0s.size
     at = _as_state(exe, atol, n, "atol")

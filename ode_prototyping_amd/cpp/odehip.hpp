@@ -36,6 +36,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <concepts>
 #include <cstddef>
 #include <span>
 #include <stdexcept>
@@ -185,6 +186,50 @@ using BTDVERK = TableauTag<ODEHIP_TABLEAU_DVERK, 6, 5, 8>;
 using BTRKF78 = TableauTag<ODEHIP_TABLEAU_RKF78, 7, 8, 13>;
 using BTDOP853 = TableauTag<ODEHIP_TABLEAU_DOP853, 8, 5, 12>;
 
+// A tableau type in the reference's own format (butcher_tableaus/BTDOPRI5.hpp:
+// 5-26): static a (n_stages - 1 rows of stage weights), b, bt, p, pt, n_stages.
+template <typename BT>
+concept ReferenceTableau = requires {
+  BT::a[0][0];
+  BT::b[0];
+  BT::bt[0];
+  { BT::p } -> std::convertible_to<int>;
+  { BT::pt } -> std::convertible_to<int>;
+  { BT::n_stages } -> std::convertible_to<int>;
+};
+
+template <typename BT>
+concept LibraryTableau = requires {
+  { BT::id } -> std::convertible_to<int>;
+};
+
+// The library's id of tableau type BT: a tag's id, or a reference-format
+// struct registered with the library on first use (odehip_tableau_register;
+// the same coefficients always give the same id).
+template <typename BT>
+int tableau_id(HipExecutor& exe) {
+  if constexpr (LibraryTableau<BT>) {
+    return BT::id;
+  } else {
+    static_assert(ReferenceTableau<BT>,
+                  "a tableau is a library tag or a struct with a, b, bt, p, pt, n_stages");
+    constexpr int S = BT::n_stages;
+    std::array<double, (S - 1) * (S - 1)> a{};
+    std::array<double, S> b{}, bt{};
+    for (int r = 0; r < S - 1; ++r)
+      for (int j = 0; j <= r; ++j) a[r * (S - 1) + j] = BT::a[r][j];
+    for (int j = 0; j < S; ++j) {
+      b[j] = BT::b[j];
+      bt[j] = BT::bt[j];
+    }
+    int id = -1;
+    exe.check(odehip_tableau_register(exe.context(), S, BT::p, BT::pt, a.data(), b.data(),
+                                      bt.data(), &id),
+              "tableau_register");
+    return id;
+  }
+}
+
 // ---------------------------------------------------------------------- ODEs
 enum class NBodyArithmetic { exact = ODEHIP_NBODY_EXACT, fast = ODEHIP_NBODY_FAST,
                              mixed = ODEHIP_NBODY_MIXED };
@@ -317,6 +362,7 @@ struct RKEmbeddedParallel {
     static const auto safety_factor = std::pow(0.38, (1.0 / (1.0 + q)));
     HipState<double, BT::n_stages * n_var> ks{};
     auto* ctx = exe.context();
+    const int tab = tableau_id<BT>(exe);
     double dt = estimate_initial_step(exe, x0, atol, rtol, ode);
     double t = t0;
     auto x = x0;
@@ -325,13 +371,13 @@ struct RKEmbeddedParallel {
     while (t < tf) {
       ode(exe, span(std::as_const(x0)), std::span<double, n_var>{ks.data(), n_var});
       for (int stage = 1; stage < BT::n_stages; ++stage) {
-        exe.check(odehip_rk_stage(ctx, BT::id, stage, dt, temp_state.data(), ks.data(),
+        exe.check(odehip_rk_stage(ctx, tab, stage, dt, temp_state.data(), ks.data(),
                                   x0.data(), n_var),
                   "rk_stage");
         ode(exe, span(std::as_const(temp_state)),
             std::span<double, n_var>{ks.data() + stage * n_var, n_var});
       }
-      exe.check(odehip_rk_update_state_and_error(ctx, BT::id, dt, x.data(),
+      exe.check(odehip_rk_update_state_and_error(ctx, tab, dt, x.data(),
                                                  error_estimate.data(), ks.data(),
                                                  x0.data(), n_var),
                 "rk_update_state_and_error");
@@ -387,7 +433,8 @@ struct RKEmbeddedParallel {
       opt.times_cap = static_cast<long>(times->size());
     }
     odehip_solve_stats st{};
-    exe.check(odehip_rk_solve(exe.context(), BT::id, &o, &opt, x.data(), &st), "rk_solve");
+    exe.check(odehip_rk_solve(exe.context(), tableau_id<BT>(exe), &o, &opt, x.data(), &st),
+              "rk_solve");
     if (times) times->resize(st.n_times);
     return {st.accepted, st.rejected, st.rhs_evals, st.status, st.t};
   }
@@ -401,6 +448,7 @@ struct RKEmbeddedParallel {
                                               const HipState<double, N>& rtol,
                                               const ODE& ode, DenseOutput<N>& output,
                                               HipExecutor& exe, bool fsal = true) {
+    static_assert(LibraryTableau<BT>, "dense output needs the library's DOP853 or DOPRI5");
     static_assert(BT::id == ODEHIP_TABLEAU_DOP853 || BT::id == ODEHIP_TABLEAU_DOPRI5,
                   "dense output needs DOP853 or DOPRI5");
     std::array<double, N> first{};
@@ -503,7 +551,8 @@ inline EnsembleResult solve_ensemble(HipExecutor& exe, const std::vector<int>& s
                                      NBodyArithmetic arithmetic = NBodyArithmetic::exact) {
   EnsembleResult r;
   r.systems.resize(sizes.size());
-  exe.check(odehip_ensemble_solve(exe.context(), BT::id, static_cast<int>(sizes.size()),
+  exe.check(odehip_ensemble_solve(exe.context(), tableau_id<BT>(exe),
+                                  static_cast<int>(sizes.size()),
                                   sizes.data(), static_cast<int>(arithmetic), softening,
                                   masses_dev, &opt, x0_dev, x_out_dev, &r.totals,
                                   r.systems.data()),

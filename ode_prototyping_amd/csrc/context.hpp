@@ -100,6 +100,12 @@ inline int use_device(odehip_context* ctx) {
   return ODEHIP_OK;
 }
 
+namespace odehip {
+// Checks a tableau id (built in or registered) and makes a registered one
+// readable by the kernels on the context's device (uploaded once per device).
+int use_tableau(odehip_context* ctx, int id);
+}  // namespace odehip
+
 #define ODEHIP_USE_DEVICE(ctx)            \
   do {                                    \
     const int _rc = use_device(ctx);      \

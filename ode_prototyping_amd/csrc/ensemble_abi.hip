@@ -269,8 +269,7 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
   ODEHIP_USE_DEVICE(ctx);
   if (!ctx) return ODEHIP_ERR_INVALID;
   if (!opt || !x0 || !x_out) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
-  if (tableau < 0 || tableau >= ODEHIP_NUM_TABLEAUS)
-    return fail(ctx, ODEHIP_ERR_INVALID, "bad tableau");
+  if (const int trc = use_tableau(ctx, tableau)) return trc;
   int rc = check_common(ctx, nbody_variant, softening);
   if (rc) return rc;
   if (opt->max_accepted < 0 || opt->max_attempts < 0 || opt->trace_cap < 0)
@@ -302,7 +301,7 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
   }
   Layout L;
   if ((rc = make_layout(ctx, n_systems, n_particles, &L))) return rc;
-  const int S = kTableaus[tableau].S;
+  const int S = host_tableau(tableau).S;
   const long trace_cap = opt->trace ? opt->trace_cap : 0;
   const size_t tcap = tiles_capacity(L);
   const Plan p = plan(L, tcap, S, trace_cap, true, n_dense);

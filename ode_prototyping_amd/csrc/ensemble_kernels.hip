@@ -25,7 +25,8 @@
 namespace odehip {
 namespace {
 
-__constant__ Tableau e_tab[ODEHIP_NUM_TABLEAUS] = {
+// built-in tableaux, then the slots of registered ones
+__constant__ Tableau e_tab[ODEHIP_MAX_TABLEAUS] = {
     kTableaus[0], kTableaus[1], kTableaus[2],
     kTableaus[3], kTableaus[4], kTableaus[5]};
 
@@ -706,7 +707,7 @@ hipError_t ens_enqueue_initial_step(const EnsBufs& b, hipStream_t s) {
 
 // One lockstep round: an attempt of every active system.
 hipError_t ens_enqueue_round(const EnsBufs& b, hipStream_t s) {
-  const Tableau& T = kTableaus[b.tableau];
+  const Tableau& T = host_tableau(b.tableau);
   hipError_t e = ens_launch_rhs(b, nullptr, b.ks, 1, s);
   if (e != hipSuccess) return e;
   for (int stage = 1; stage < T.S; ++stage) {
@@ -742,6 +743,12 @@ hipError_t ens_enqueue_copy_out(const EnsBufs& b, double* x_out,
   hipLaunchKernelGGL(k_ens_copy_out, dim3(groups_grid(b)), dim3(kBlock), 0, s, b,
                      x_out);
   return hipGetLastError();
+}
+
+hipError_t upload_ens_tableau(int id, const Tableau& T) {
+  if (id < ODEHIP_NUM_TABLEAUS || id >= ODEHIP_MAX_TABLEAUS) return hipErrorInvalidValue;
+  return hipMemcpyToSymbol(HIP_SYMBOL(e_tab), &T, sizeof(Tableau), id * sizeof(Tableau),
+                           hipMemcpyHostToDevice);
 }
 
 }  // namespace odehip

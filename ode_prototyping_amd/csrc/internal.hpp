@@ -82,6 +82,15 @@ hipError_t launch_nbody_energy(const double* x, int n, double eps2,
                                const double* mass, double* per_particle,
                                hipStream_t s);
 
+// ---- tableaux (built in, or registered at run time; odehip_abi.hip) ----
+bool valid_tableau(int id);
+// The host copy of a valid tableau id.
+const Tableau& host_tableau(int id);
+// Constant-table slot uploads of a registered tableau, on the current device
+// (rk_kernels.hip, ensemble_kernels.hip).
+hipError_t upload_rk_tableau(int id, const Tableau& T);
+hipError_t upload_ens_tableau(int id, const Tableau& T);
+
 // ---- launchers (rk_kernels.hip) ----
 hipError_t launch_simple_rhs(int kind, const double* y, const double* xb0,
                              const double* xb1, const SolverState* st,

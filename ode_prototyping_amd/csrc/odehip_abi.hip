@@ -13,8 +13,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "context.hpp"
 #include "mlp.hpp"
@@ -30,6 +32,53 @@ int odehip::ensure_scratch(odehip_context* ctx, size_t n) {
   size_t cap = std::max<size_t>(n, 4096);
   HIP_TRY(ctx, hipMalloc(&ctx->scratch, cap * sizeof(double)));
   ctx->scratch_cap = cap;
+  return ODEHIP_OK;
+}
+
+// ------------------------------------------------------------ tableaux ----
+// Registered tableaux are process-wide: host copies here, and each device's
+// constant tables receive a slot on the first call that uses it there.
+namespace {
+
+struct TableauRegistry {
+  std::mutex mu;
+  Tableau tab[ODEHIP_MAX_TABLEAUS] = {};
+  int count = ODEHIP_NUM_TABLEAUS;  // ids below this are valid
+  std::vector<int> resident;        // per device: slots below this uploaded
+};
+
+TableauRegistry& registry() {
+  static TableauRegistry r;
+  return r;
+}
+
+}  // namespace
+
+bool odehip::valid_tableau(int id) {
+  if (id >= 0 && id < ODEHIP_NUM_TABLEAUS) return true;
+  TableauRegistry& r = registry();
+  std::lock_guard<std::mutex> lock(r.mu);
+  return id >= ODEHIP_NUM_TABLEAUS && id < r.count;
+}
+
+const Tableau& odehip::host_tableau(int id) {
+  // a registered slot is written before its id becomes valid and never again
+  return id < ODEHIP_NUM_TABLEAUS ? kTableaus[id] : registry().tab[id];
+}
+
+int odehip::use_tableau(odehip_context* ctx, int id) {
+  if (id >= 0 && id < ODEHIP_NUM_TABLEAUS) return ODEHIP_OK;
+  TableauRegistry& r = registry();
+  std::lock_guard<std::mutex> lock(r.mu);
+  if (id < ODEHIP_NUM_TABLEAUS || id >= r.count)
+    return fail(ctx, ODEHIP_ERR_INVALID, "bad tableau");
+  const int d = ctx->device;
+  if (static_cast<int>(r.resident.size()) <= d)
+    r.resident.resize(d + 1, ODEHIP_NUM_TABLEAUS);
+  for (int& next = r.resident[d]; next <= id; ++next) {
+    HIP_TRY(ctx, upload_rk_tableau(next, r.tab[next]));
+    HIP_TRY(ctx, upload_ens_tableau(next, r.tab[next]));
+  }
   return ODEHIP_OK;
 }
 
@@ -54,8 +103,6 @@ int ensure_ws(odehip_context* ctx, size_t n) {
   ctx->ws_cap = n;
   return ODEHIP_OK;
 }
-
-bool valid_tableau(int t) { return t >= 0 && t < ODEHIP_NUM_TABLEAUS; }
 
 int check_ode(odehip_context* ctx, const odehip_ode* ode) {
   if (!ode) return fail(ctx, ODEHIP_ERR_INVALID, "ode is null");
@@ -111,7 +158,7 @@ int prepare_bufs(odehip_context* ctx, int tableau, const odehip_ode* ode,
                  const double* atol, const double* rtol, long times_cap,
                  long trace_cap, long states_cap, long n_dense, SolveBufs* b) {
   const size_t n = ode->n_var;
-  const int S = kTableaus[tableau].S;
+  const int S = host_tableau(tableau).S;
   size_t need = layout(b, nullptr, ctx->st, n, S, times_cap, trace_cap, states_cap, n_dense);
   int rc = ensure_ws(ctx, need);
   if (rc) return rc;
@@ -373,13 +420,72 @@ int odehip_ode_rhs(odehip_context* ctx, int kind, const double* x, double* dxdt,
   return ODEHIP_OK;
 }
 
+int odehip_tableau_register(odehip_context* ctx, int n_stages, int p, int pt,
+                            const double* a, const double* b, const double* bt,
+                            int* id) {
+  ODEHIP_USE_DEVICE(ctx);
+  if (!ctx) return ODEHIP_ERR_INVALID;
+  if (!a || !b || !bt || !id) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  const int S = n_stages;
+  if (S < 2 || S > ODEHIP_MAX_STAGES)
+    return fail(ctx, ODEHIP_ERR_INVALID, "n_stages must be in [2, ODEHIP_MAX_STAGES]");
+  if (p < 1 || p > 20 || pt < 1 || pt > 20)
+    return fail(ctx, ODEHIP_ERR_INVALID, "orders p, pt must be in [1, 20]");
+  Tableau T{};
+  T.S = S;
+  T.p = p;
+  T.pt = pt;
+  for (int s = 1; s < S; ++s)
+    for (int j = 0; j < s; ++j) {
+      const double v = a[(s - 1) * (S - 1) + j];
+      if (!std::isfinite(v)) return fail(ctx, ODEHIP_ERR_INVALID, "non-finite coefficient");
+      T.a[s - 1][j] = v;
+    }
+  for (int j = 0; j < S; ++j) {
+    if (!std::isfinite(b[j]) || !std::isfinite(bt[j]))
+      return fail(ctx, ODEHIP_ERR_INVALID, "non-finite coefficient");
+    T.b[j] = b[j];
+    T.db[j] = b[j] - bt[j];  // as the reference's constexpr db (RKEmbeddedParallel.hpp:108-114)
+  }
+  bool fsal = b[S - 1] == 0.0;
+  for (int j = 0; j < S - 1 && fsal; ++j) fsal = T.a[S - 2][j] == b[j];
+  T.fsal = fsal ? 1 : 0;
+  int found = -1;
+  for (int k = 0; k < ODEHIP_NUM_TABLEAUS && found < 0; ++k)
+    if (std::memcmp(&kTableaus[k], &T, sizeof T) == 0) found = k;
+  if (found < 0) {
+    TableauRegistry& r = registry();
+    std::lock_guard<std::mutex> lock(r.mu);
+    for (int k = ODEHIP_NUM_TABLEAUS; k < r.count && found < 0; ++k)
+      if (std::memcmp(&r.tab[k], &T, sizeof T) == 0) found = k;
+    if (found < 0) {
+      if (r.count >= ODEHIP_MAX_TABLEAUS)
+        return fail(ctx, ODEHIP_ERR_INVALID, "no free tableau slot");
+      r.tab[r.count] = T;
+      found = r.count++;
+    }
+  }
+  if (const int rc = use_tableau(ctx, found)) return rc;
+  *id = found;
+  return ODEHIP_OK;
+}
+
+int odehip_tableau_info(int id, int* n_stages, int* p, int* pt) {
+  if (!valid_tableau(id)) return ODEHIP_ERR_INVALID;
+  const Tableau& T = host_tableau(id);
+  if (n_stages) *n_stages = T.S;
+  if (p) *p = T.p;
+  if (pt) *pt = T.pt;
+  return ODEHIP_OK;
+}
+
 int odehip_rk_stage(odehip_context* ctx, int tableau, int stage, double dt,
                     double* temp, const double* ks, const double* x0,
                     size_t n_var) {
   ODEHIP_USE_DEVICE(ctx);
   if (!ctx || !temp || !ks || !x0) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
-  if (!valid_tableau(tableau)) return fail(ctx, ODEHIP_ERR_INVALID, "bad tableau");
-  if (stage < 1 || stage >= kTableaus[tableau].S)
+  if (const int trc = use_tableau(ctx, tableau)) return trc;
+  if (stage < 1 || stage >= host_tableau(tableau).S)
     return fail(ctx, ODEHIP_ERR_INVALID, "stage out of range");
   HIP_TRY(ctx, launch_rk_stage_host_dt(tableau, stage, dt, temp, ks, x0, n_var, ctx->stream));
   return ODEHIP_OK;
@@ -391,7 +497,7 @@ int odehip_rk_update_state_and_error(odehip_context* ctx, int tableau,
                                      size_t n_var) {
   ODEHIP_USE_DEVICE(ctx);
   if (!ctx || !x || !err || !ks || !x0) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
-  if (!valid_tableau(tableau)) return fail(ctx, ODEHIP_ERR_INVALID, "bad tableau");
+  if (const int trc = use_tableau(ctx, tableau)) return trc;
   HIP_TRY(ctx, launch_rk_update_host_dt(tableau, dt, x, err, ks, x0, n_var, ctx->stream));
   return ODEHIP_OK;
 }
@@ -455,7 +561,7 @@ int odehip_rk_initial_step(odehip_context* ctx, int tableau, int rhs_kind,
   ODEHIP_USE_DEVICE(ctx);
   if (!ctx || !x0 || !atol || !rtol || !host_dt)
     return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
-  if (!valid_tableau(tableau)) return fail(ctx, ODEHIP_ERR_INVALID, "bad tableau");
+  if (const int trc = use_tableau(ctx, tableau)) return trc;
   odehip_ode ode{rhs_kind, variant, n_var, softening, masses};
   int rc = check_ode(ctx, &ode);
   if (rc) return rc;
@@ -479,7 +585,7 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
   ODEHIP_USE_DEVICE(ctx);
   if (!ctx) return ODEHIP_ERR_INVALID;
   if (!opt || !x) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
-  if (!valid_tableau(tableau)) return fail(ctx, ODEHIP_ERR_INVALID, "bad tableau");
+  if (const int trc = use_tableau(ctx, tableau)) return trc;
   int rc = check_ode(ctx, ode);
   if (rc) return rc;
   if (!opt->atol || !opt->rtol) return fail(ctx, ODEHIP_ERR_INVALID, "atol/rtol null");
@@ -520,7 +626,7 @@ int odehip_rk_solve(odehip_context* ctx, int tableau, const odehip_ode* ode,
   if (rc) return rc;
   b.controller = opt->controller;
   b.mlp = opt->controller == ODEHIP_CONTROLLER_LEARNED ? mlp_infer_ptr(opt->mlp) : nullptr;
-  const int S = kTableaus[tableau].S;
+  const int S = host_tableau(tableau).S;
   const size_t n = ode->n_var;
 
   // attempts per replay: aim for several ms of work per replay

@@ -13,7 +13,8 @@
 namespace odehip {
 namespace {
 
-__constant__ Tableau c_tab[ODEHIP_NUM_TABLEAUS] = {
+// built-in tableaux, then the slots of registered ones
+__constant__ Tableau c_tab[ODEHIP_MAX_TABLEAUS] = {
     kTableaus[0], kTableaus[1], kTableaus[2],
     kTableaus[3], kTableaus[4], kTableaus[5]};
 
@@ -488,7 +489,7 @@ hipError_t enqueue_initial_step(const SolveBufs& b, hipStream_t s) {
 }
 
 hipError_t enqueue_attempt(const SolveBufs& b, hipStream_t s) {
-  const Tableau& T = kTableaus[b.tableau];
+  const Tableau& T = host_tableau(b.tableau);
   const size_t n = b.n_var;
   hipError_t e = enqueue_rhs(b, nullptr, b.ks, 1, s);
   if (e != hipSuccess) return e;
@@ -535,6 +536,12 @@ hipError_t enqueue_solve_finish(const SolveBufs& b, double* x_out,
   hipLaunchKernelGGL(k_copy_out, dim3(blocks_for(b.n_var)), dim3(kBlock), 0, s,
                      b.st, b.xb0, b.xb1, x_out, b.n_var);
   return hipGetLastError();
+}
+
+hipError_t upload_rk_tableau(int id, const Tableau& T) {
+  if (id < ODEHIP_NUM_TABLEAUS || id >= ODEHIP_MAX_TABLEAUS) return hipErrorInvalidValue;
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &T, sizeof(Tableau), id * sizeof(Tableau),
+                           hipMemcpyHostToDevice);
 }
 
 }  // namespace odehip

@@ -18,8 +18,8 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import CONTROLLERS, NBODY_VARIANTS, STATUS, TABLEAUS, lib
-from .core import HipExecutor, HipState
+from ._lib import CONTROLLERS, NBODY_VARIANTS, STATUS, lib
+from .core import HipExecutor, HipState, tableau_id, tableau_shape
 
 ENS_TILE_CONFIGS = _lib.ENS_TILE_CONFIGS
 ENS_TILE_AUTO = _lib.ENS_TILE_AUTO
@@ -119,8 +119,7 @@ def solve_ensemble(tableau, sizes, x0, t0, tf, atol, rtol, softening, masses=Non
     t0; every system's state there, from its continuous extension, is in
     ``dense_states`` (packed rows) and ``dense_rows(k)``.
     """
-    if tableau not in TABLEAUS:
-        raise ValueError(f"unknown tableau {tableau}")
+    tableau_shape(tableau)  # a known tableau (ValueError otherwise)
     if variant not in NBODY_VARIANTS:
         raise ValueError(f"unknown variant {variant}")
     if controller not in CONTROLLERS:
@@ -151,7 +150,8 @@ def solve_ensemble(tableau, sizes, x0, t0, tf, atol, rtol, softening, masses=Non
                                dense.ctypes.data if dense is not None else None)
     agg = _lib.EnsembleStats()
     per = (_lib.SystemStats * sz.size)()
-    exe.check(lib().odehip_ensemble_solve(exe.handle, TABLEAUS[tableau], int(sz.size), szp,
+    exe.check(lib().odehip_ensemble_solve(exe.handle, tableau_id(exe, tableau), int(sz.size),
+                                          szp,
                                           NBODY_VARIANTS[variant], float(softening),
                                           m.ptr if m is not None else None, C.byref(opt),
                                           xin.ptr, out.ptr, C.byref(agg), per),

@@ -332,6 +332,26 @@ void oracle_momentum(const double* x, size_t n, const double* masses,
   momentum(x, n, masses, p);
 }
 
+// Registers a tableau under `name` (a: (S-1) x (S-1) row-major, entries
+// j >= row + 1 ignored; b, bt: S), as odehip_tableau_register does for the
+// library; later integrate calls take it by that name.
+int oracle_register_tableau(const char* name, int S, int p, int pt, const double* a,
+                            const double* b, const double* bt) {
+  if (!name || S < 2 || S > 13 || !a || !b || !bt) return -1;
+  Tableau t;
+  t.name = name;
+  t.S = S;
+  t.p = p;
+  t.pt = pt;
+  t.a.assign(static_cast<size_t>(S - 1) * (S - 1), 0.0);
+  for (int r = 0; r < S - 1; ++r)
+    for (int j = 0; j <= r; ++j) t.a[r * (S - 1) + j] = a[r * (S - 1) + j];
+  t.b.assign(b, b + S);
+  t.bt.assign(bt, bt + S);
+  registered_tableaus()[t.name] = t;
+  return 0;
+}
+
 // Tableau access for tests (order conditions, row sums).
 int oracle_tableau(const char* name, int* S, int* p, int* pt, double* a,
                    double* b, double* bt) {

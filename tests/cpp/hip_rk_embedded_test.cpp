@@ -162,6 +162,59 @@ static void van_der_pol_consistency(const VdpGolden& g) {
   std::printf("ok %s\n", g.name);
 }
 
+// A user's tableau in the reference's struct format (the layout of
+// butcher_tableaus/BT*.hpp: a, b, bt, p, pt, n_stages) for a method the
+// library does not ship: Cash & Karp's embedded 5(4) pair (ACM TOMS 16, 1990).
+struct BTCashKarp {
+  static constexpr auto a = std::array{
+      std::array{1.0 / 5.0, 0.0, 0.0, 0.0, 0.0},
+      std::array{3.0 / 40.0, 9.0 / 40.0, 0.0, 0.0, 0.0},
+      std::array{3.0 / 10.0, -9.0 / 10.0, 6.0 / 5.0, 0.0, 0.0},
+      std::array{-11.0 / 54.0, 5.0 / 2.0, -70.0 / 27.0, 35.0 / 27.0, 0.0},
+      std::array{1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0,
+                 253.0 / 4096.0}};
+  static constexpr auto b =
+      std::array{37.0 / 378.0, 0.0, 250.0 / 621.0, 125.0 / 594.0, 0.0, 512.0 / 1771.0};
+  static constexpr auto bt = std::array{2825.0 / 27648.0, 0.0, 18575.0 / 48384.0,
+                                        13525.0 / 55296.0, 277.0 / 14336.0, 1.0 / 4.0};
+  static constexpr auto p = 5;
+  static constexpr auto pt = 4;
+  static constexpr auto n_stages = static_cast<int>(b.size());
+};
+static_assert(ReferenceTableau<BTCashKarp> && !LibraryTableau<BTCashKarp>);
+
+// The Cash-Karp struct crosses the C ABI (registered on first use, an id past
+// the built-in ones) and integrates Van der Pol on the host-driven and the
+// device-resident paths; both end within 1e-8 of the reference's DOPRI5
+// golden end state (SingleThreadedRKEmbeddedTest.cpp:138-160, tol 1e-10).
+static void user_tableau_struct() {
+  HipExecutor executor{};
+  const int id = tableau_id<BTCashKarp>(executor);
+  EXPECT_EQ(id >= ODEHIP_NUM_TABLEAUS, true);
+  EXPECT_EQ(tableau_id<BTCashKarp>(executor), id);
+  int S = 0, p = 0, pt = 0;
+  EXPECT_EQ(odehip_tableau_info(id, &S, &p, &pt), ODEHIP_OK);
+  EXPECT_EQ(S, 6);
+  EXPECT_EQ(p, 5);
+  VanDerPolCase test;
+  RKEmbeddedParallel<BTCashKarp>::integrate(test.x0, test.t0, test.tf, test.atol, test.rtol,
+                                            test.ode, test.output, executor);
+  const auto& states = test.output.states();
+  EXPECT_DOUBLE_EQ(2.0, test.output.times().back());
+  EXPECT_NEAR(0.32331666704309109, states.back()[0], 1e-8);
+  EXPECT_NEAR(-1.8329745679718159, states.back()[1], 1e-8);
+  VanDerPolCase dev;
+  const auto st = RKEmbeddedParallel<BTCashKarp>::integrate_on_device(
+      dev.x0, dev.t0, dev.tf, dev.atol, dev.rtol, dev.ode, executor);
+  std::array<double, 2> last{};
+  dev.x0.copy_to_span(std::span<double, 2>{last});
+  EXPECT_DOUBLE_EQ(2.0, st.t);
+  EXPECT_NEAR(0.32331666704309109, last[0], 1e-8);
+  EXPECT_NEAR(-1.8329745679718159, last[1], 1e-8);
+  std::printf("ok user tableau struct (Cash-Karp, id %d, %zu accepted host-driven, %ld "
+              "device)\n", id, states.size() - 1, st.accepted);
+}
+
 static void dopri5_exponential() {
   // SingleThreadedRKEmbeddedTest.cpp:162-192, thread-pool-style tolerance
   HipExecutor executor{};
@@ -405,6 +458,7 @@ int main() {
                                     1.7022210141241134, -0.64964031512382325,
                                     0.3233166666554449, -1.8329745686289092});
   dopri5_exponential();
+  user_tableau_struct();
   rkf78_nbody();
   rkf78_spinning_cube();
   device_resident_van_der_pol();

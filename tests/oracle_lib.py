@@ -51,6 +51,8 @@ def lib():
                                                C.c_double, _dp, _dp, C.c_long, _dp, _dp]
         L.oracle_dopri5_step_dense.argtypes = L.oracle_dop853_step_dense.argtypes
         L.oracle_nr_dopr5.argtypes = L.oracle_nr_dop853_dense.argtypes
+        L.oracle_register_tableau.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, _dp, _dp,
+                                              _dp]
         L.oracle_result_count.argtypes = [C.c_void_p]
         L.oracle_result_count.restype = C.c_long
         L.oracle_result_times.argtypes = [C.c_void_p, _dp]
@@ -248,6 +250,19 @@ def tableau(name):
     bt = np.zeros(s)
     lib().oracle_tableau(name.encode(), C.byref(S), C.byref(p), C.byref(pt), _ptr(a), _ptr(b), _ptr(bt))
     return dict(S=s, p=p.value, pt=pt.value, a=a.reshape(s - 1, s - 1), b=b, bt=bt)
+
+
+def register_tableau(name, a, b, bt, p, pt):
+    """Make a tableau (the reference's struct fields) available to integrate() by name."""
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    bt = np.ascontiguousarray(bt, dtype=np.float64)
+    s = b.size
+    full = np.zeros((s - 1, s - 1))
+    for r, row in enumerate(a):
+        full[r, :r + 1] = list(row)[:r + 1]
+    if lib().oracle_register_tableau(name.encode(), s, int(p), int(pt), _ptr(full.ravel()),
+                                     _ptr(b), _ptr(bt)):
+        raise ValueError("oracle_register_tableau failed")
 
 
 def pow_(x, y, mode=POW_LIBM):

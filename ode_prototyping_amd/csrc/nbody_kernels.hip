@@ -263,10 +263,13 @@ __global__ __launch_bounds__(64 * kExactWaves) void k_nbody_exact(NBodyArgs a) {
 // its slot's partners with vector loads (the two half-waves read different
 // addresses, one tile ahead), so no per-lane selection between wave-uniform
 // values is needed.  Two waves fold a tile's terms in partner order: wave 0
-// the x (lanes 0-31) and y (lanes 32-63) chains, wave 1 the z chains.
-constexpr int kE32Waves = 16;
-constexpr int kE32JW = 3;                       // partners per slot per tile
-constexpr int kE32TJ = 2 * kE32JW * kE32Waves;  // partners per tile (144 KB LDS)
+// the x (lanes 0-31) and y (lanes 32-63) chains, wave 1 the z chains.  W waves
+// per workgroup: a tile has 6 W partners and the two term buffers take
+// 2 x 6W x 3 x 32 doubles (144 KB at W = 16: one workgroup per CU; 72 KB at
+// W = 8: two).
+constexpr int kE32JW = 3;  // partners per slot per tile
+template <int W>
+constexpr int e32_tj = 2 * kE32JW * W;  // partners per tile
 
 // This lane's kE32JW partners (x, y, z, m), starting at index jb.
 typedef double E32Partners[kE32JW][4];
@@ -323,11 +326,13 @@ __device__ __forceinline__ void e32_terms(const E32Partners& q, double xi, doubl
 }
 
 // exact_fold for the 32-target layout (partner stride 96 doubles).
+template <int W>
 __device__ __forceinline__ double e32_fold(double acc, const double* src, int base, int cnt,
                                            int i0, int i) {
-  if (cnt == kE32TJ && (base + kE32TJ <= i0 || base >= i0 + 32)) {
+  constexpr int TJ = e32_tj<W>;
+  if (cnt == TJ && (base + TJ <= i0 || base >= i0 + 32)) {
 #pragma unroll
-    for (int m = 0; m < kE32TJ; ++m) acc += src[m * 96];
+    for (int m = 0; m < TJ; ++m) acc += src[m * 96];
   } else {
     for (int m = 0; m < cnt; ++m) {
       const double v = src[m * 96];
@@ -341,12 +346,13 @@ __device__ __forceinline__ double e32_fold(double acc, const double* src, int ba
 // tile, a two-tile steady-state loop over two register partner sets with no
 // bounds tests, then the remaining tiles).  Returns the folded sum of the
 // calling lane (meaningful on the folding lanes).
-template <bool kMass, bool kShared>
-__device__ __forceinline__ double e32_tiles(double (*terms)[kE32TJ][3][32],
+template <bool kMass, bool kShared, int W>
+__device__ __forceinline__ double e32_tiles(double (*terms)[e32_tj<W>][3][32],
                                             const double* __restrict__ y,
                                             const double* __restrict__ mass, int n,
                                             double softening_sq, double xi, double yi, double zi,
                                             int wave, int slot, int tgt, int i0, int i, int fc) {
+  constexpr int kE32TJ = e32_tj<W>;
   const int n_tiles = (n + kE32TJ - 1) / kE32TJ;
   const int n_full = n / kE32TJ;
   const int m0 = 2 * kE32JW * wave + slot * kE32JW;  // this lane's first partner in a tile
@@ -358,11 +364,12 @@ __device__ __forceinline__ double e32_tiles(double (*terms)[kE32TJ][3][32],
   __syncthreads();
   int t = 1;
   for (; t + 2 < n_full; t += 2) {
-    if (wave < 2) acc = e32_fold(acc, &terms[0][0][fc][tgt], (t - 1) * kE32TJ, kE32TJ, i0, i);
+    if (wave < 2)
+      acc = e32_fold<W>(acc, &terms[0][0][fc][tgt], (t - 1) * kE32TJ, kE32TJ, i0, i);
     e32_load<kMass>(qa, y, mass, (t + 1) * kE32TJ + m0);
     e32_terms<kMass, kShared>(qb, xi, yi, zi, softening_sq, terms[1], m0, tgt);
     __syncthreads();
-    if (wave < 2) acc = e32_fold(acc, &terms[1][0][fc][tgt], t * kE32TJ, kE32TJ, i0, i);
+    if (wave < 2) acc = e32_fold<W>(acc, &terms[1][0][fc][tgt], t * kE32TJ, kE32TJ, i0, i);
     e32_load<kMass>(qb, y, mass, (t + 2) * kE32TJ + m0);
     e32_terms<kMass, kShared>(qa, xi, yi, zi, softening_sq, terms[0], m0, tgt);
     __syncthreads();
@@ -370,7 +377,8 @@ __device__ __forceinline__ double e32_tiles(double (*terms)[kE32TJ][3][32],
   for (; t <= n_tiles; ++t) {
     if (wave < 2) {
       const int base = (t - 1) * kE32TJ;
-      acc = e32_fold(acc, &terms[(t - 1) & 1][0][fc][tgt], base, min(kE32TJ, n - base), i0, i);
+      acc = e32_fold<W>(acc, &terms[(t - 1) & 1][0][fc][tgt], base, min(kE32TJ, n - base), i0,
+                        i);
     }
     if (t < n_tiles) {
       E32Partners qc;
@@ -386,10 +394,12 @@ __device__ __forceinline__ double e32_tiles(double (*terms)[kE32TJ][3][32],
   return acc;
 }
 
-template <bool kMass, bool kSharedRcp>
-__global__ __launch_bounds__(64 * kE32Waves) void k_nbody_exact32(NBodyArgs a) {
+// At W = 8 two workgroups share a CU: 4 waves per SIMD, <= 128 VGPRs each.
+template <bool kMass, bool kSharedRcp, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4)))
+void k_nbody_exact32(NBodyArgs a) {
   if (prologue_skip(a)) return;
-  __shared__ double terms[2][kE32TJ][3][32];
+  __shared__ double terms[2][e32_tj<W>][3][32];
   const double* __restrict__ y = select_input(a);
   double* __restrict__ k = a.k;
   const double* __restrict__ mass = a.mass;
@@ -409,11 +419,11 @@ __global__ __launch_bounds__(64 * kE32Waves) void k_nbody_exact32(NBodyArgs a) {
   const bool shared_ok = kSharedRcp && all_positions_shared_ok(y, n);
   double acc;
   if (kSharedRcp && shared_ok)
-    acc = e32_tiles<kMass, true>(terms, y, mass, n, a.eps2, xi, yi, zi, wave, slot, tgt, i0, i,
-                                 fc);
+    acc = e32_tiles<kMass, true, W>(terms, y, mass, n, a.eps2, xi, yi, zi, wave, slot, tgt, i0,
+                                    i, fc);
   else
-    acc = e32_tiles<kMass, false>(terms, y, mass, n, a.eps2, xi, yi, zi, wave, slot, tgt, i0, i,
-                                  fc);
+    acc = e32_tiles<kMass, false, W>(terms, y, mass, n, a.eps2, xi, yi, zi, wave, slot, tgt, i0,
+                                     i, fc);
   if (folds && i < n) k[3L * n + 3 * i + fc] = acc;
   // dxdt[0:3n] = x[3n:6n] for this tile's 32 particles
   const int e = threadIdx.x;
@@ -666,17 +676,17 @@ hipError_t launch_nbody_rhs(const NBodyArgs& a, int variant, hipStream_t s) {
       // 64-target kernel, which needs half as many rounds.
       if (a.n <= 8192) {
         const unsigned b32 = static_cast<unsigned>((a.n + 31) / 32);
-        const dim3 th(64 * kE32Waves);
+        const dim3 th(64 * 16);
         if (mass) {
           if (shared)
-            hipLaunchKernelGGL((k_nbody_exact32<true, true>), dim3(b32), th, 0, s, a);
+            hipLaunchKernelGGL((k_nbody_exact32<true, true, 16>), dim3(b32), th, 0, s, a);
           else
-            hipLaunchKernelGGL((k_nbody_exact32<true, false>), dim3(b32), th, 0, s, a);
+            hipLaunchKernelGGL((k_nbody_exact32<true, false, 16>), dim3(b32), th, 0, s, a);
         } else {
           if (shared)
-            hipLaunchKernelGGL((k_nbody_exact32<false, true>), dim3(b32), th, 0, s, a);
+            hipLaunchKernelGGL((k_nbody_exact32<false, true, 16>), dim3(b32), th, 0, s, a);
           else
-            hipLaunchKernelGGL((k_nbody_exact32<false, false>), dim3(b32), th, 0, s, a);
+            hipLaunchKernelGGL((k_nbody_exact32<false, false, 16>), dim3(b32), th, 0, s, a);
         }
         break;
       }

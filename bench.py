@@ -17,9 +17,11 @@ headers) timed on this host on a bounded sample (2 accepted steps at full N).
 Further blocks (each a separate workload, not the headline value):
 * "ensemble": 2048 ragged systems (N log-uniform in [16, 4096], uniform-cube
   positions, N(0, 0.1^2) velocities, masses 1/N, eps 1e-3), DOPRI5, rtol =
-  atol = 1e-10, t in [0, 0.1], every system with its own step size: plain
-  FP64 (exact RHS, reference controller) against the chosen path (mixed
-  precision RHS + learned controller) and the other variants;
+  atol = 1e-10, t in [0, 0.1] (--ensemble-tf for the configured [0, 2]), every
+  system with its own step size: plain FP64 (exact RHS, reference controller)
+  against the chosen path (mixed-precision RHS with the close-encounter near
+  field, learned controller, k1 reuse) and the variants between them, with the
+  targets (particle-steps/s, RHS evals/solve, final state, energy drift) stated;
 * "controller": 4096 held-out systems of the same distribution, DOP853,
   t in [0, 0.05], mixed-precision RHS: reference vs PI vs learned controller;
 * "precision": N = 65536 colliding Plummer spheres, eps 1e-2, DOP853, 10
@@ -120,41 +122,50 @@ def nbody_energy(oh, exe, x_dev_or_host, n, eps, m):
     return out.sum()
 
 
-def ensemble_block(oh, exe):
-    """Batched ensemble of 2048 ragged systems, DOPRI5."""
+def ensemble_block(oh, exe, tf=0.1):
+    """Batched ensemble of 2048 ragged systems, DOPRI5: the plain FP64 path against
+    the chosen path and the variants between them."""
     from ode_prototyping_amd.ensemble import solve_ensemble, unpack
     from ode_prototyping_amd import train_controller as trc
-    tf, eps, tol, tab = 0.1, 1e-3, 1e-10, "DOPRI5"
+    eps, tol, tab = 1e-3, 1e-10, "DOPRI5"
     sizes, x0, m = trc.workload_ensemble(2048, 0)
     x0d, md = oh.HipState(exe, data=x0), oh.HipState(exe, data=m)
     out_state = oh.HipState(exe, x0.size)
     net, info = trc.pretrained(exe, tab)
     inter = float(np.sum(sizes.astype(float) * (sizes - 1)))
-    runs = {"plain_exact": ("exact", "reference", None),
-            "fast": ("fast", "reference", None),
-            "mixed": ("mixed", "reference", None),
-            "chosen_mixed_learned": ("mixed", "learned", net)}
+    close = oh.MIXED_NEAR_CLOSE
+    # name: (RHS variant, controller, network, near-field coefficient, k1 reuse)
+    runs = {"plain_exact": ("exact", "reference", None, 0.0, False),
+            "fast": ("fast", "reference", None, 0.0, False),
+            "fast_fsal": ("fast", "reference", None, 0.0, True),
+            "mixed": ("mixed", "reference", None, 0.0, False),
+            "mixed_near": ("mixed", "reference", None, close, False),
+            "chosen": ("mixed", "learned", net, close, True)}
     block = {"workload": f"{len(sizes)} systems, N log-uniform [16, 4096] (sum N "
                          f"{int(sizes.sum())}), uniform cube, masses 1/N, eps={eps}, {tab}, "
                          f"rtol=atol={tol}, t in [0, {tf}], per-system step size",
+             "chosen_path": "mixed-precision RHS (FP32 pair terms, FP64 sums) with the "
+                            f"close-encounter near field (pairs with r^2 < {close}/N in FP64), "
+                            "learned step-size controller (PI fallback), first-same-as-last "
+                            "k1 reuse",
              "interactions_per_rhs": inter, "learned_net": info}
     finals = {}
-    for name, (var, ctrl, mlp) in runs.items():
+    for name, (var, ctrl, mlp, near, fsal) in runs.items():
+        exe.near_field = near
+        kw = dict(variant=var, exe=exe, controller=ctrl, mlp=mlp, fsal=fsal, x_out=out_state,
+                  fetch=False)
         # untimed: the tile autotuner runs on the first solve of a variant
-        solve_ensemble(tab, sizes, x0d, 0.0, tf, tol, tol, eps, md, variant=var, exe=exe,
-                       controller=ctrl, mlp=mlp, max_attempts=1, x_out=out_state,
-                       fetch=False)
+        solve_ensemble(tab, sizes, x0d, 0.0, tf, tol, tol, eps, md, max_attempts=1, **kw)
         exe.synchronize()
         t = time.perf_counter()
-        r = solve_ensemble(tab, sizes, x0d, 0.0, tf, tol, tol, eps, md, variant=var, exe=exe,
-                           controller=ctrl, mlp=mlp, x_out=out_state, fetch=False)
+        r = solve_ensemble(tab, sizes, x0d, 0.0, tf, tol, tol, eps, md, **kw)
         exe.synchronize()
         secs = time.perf_counter() - t
         finals[name] = out_state.copy_to_span()
         peak = 2 * FP64_PEAK_TFLOPS if var == "mixed" else FP64_PEAK_TFLOPS
         ach = FLOP_PER_INTERACTION * inter / (r.rhs_ms * 1e-3) / 1e12 if r.rhs_ms else None
-        block[name] = {"variant": var, "controller": ctrl, "seconds": secs,
-                       "particle_steps_per_s": r.particle_steps / secs,
+        block[name] = {"variant": var, "controller": ctrl, "near_field": near, "fsal": fsal,
+                       "seconds": secs, "particle_steps_per_s": r.particle_steps / secs,
                        "rhs_evals_per_solve": r.total_rhs_evals / len(sizes),
                        "accepted": r.total_accepted, "rejected": r.total_rejected,
                        "rounds": r.rounds, "learned_steps": r.total_learned_steps,
@@ -162,25 +173,39 @@ def ensemble_block(oh, exe):
                        "rhs_tflops": ach, "rhs_peak_tflops": peak,
                        "rhs_frac": (ach / peak) if ach else None}
         log(f"ensemble {name}: {block[name]}")
-    # error of the chosen path vs the plain FP64 path, and energy drift
+    exe.near_field = oh.MIXED_NEAR_DEFAULT
+    # every system ends at t = tf, so final states compare at equal t; the plain path
+    # equals the FP64 oracle bit for bit (tests/test_gpu_configs.py)
     ref = unpack(finals["plain_exact"], sizes)
     ms = unpack(m, sizes, per_particle=1)
     x0s = unpack(x0, sizes)
     idx = np.linspace(0, len(sizes) - 1, 64).astype(int)  # energy on 64 systems
-    for name in ("chosen_mixed_learned", "mixed", "fast"):
+    e0 = {int(k): nbody_energy(oh, exe, x0s[k], int(sizes[k]), eps, ms[k]) for k in idx}
+    for name in runs:
         got = unpack(finals[name], sizes)
-        err = [np.max(np.abs(a - b)) / np.max(np.abs(b)) for a, b in zip(got, ref)]
-        block[name]["final_state_rel_err_vs_plain"] = {"median": float(np.median(err)),
-                                                       "max": float(np.max(err))}
-    for name in ("plain_exact", "chosen_mixed_learned", "fast"):
-        got = unpack(finals[name], sizes)
-        drift = []
-        for k in idx:
-            e0 = nbody_energy(oh, exe, x0s[k], int(sizes[k]), eps, ms[k])
-            e1 = nbody_energy(oh, exe, got[k], int(sizes[k]), eps, ms[k])
-            drift.append(abs((e1 - e0) / e0))
+        if name != "plain_exact":
+            err = np.array([np.max(np.abs(a - b)) / np.max(np.abs(b)) for a, b in zip(got, ref)])
+            block[name]["final_state_rel_err_vs_plain"] = {
+                "median": float(np.median(err)), "p90": float(np.quantile(err, 0.9)),
+                "max": float(np.max(err)), "frac_le_1e-9": float(np.mean(err <= 1e-9))}
+        drift = [abs((nbody_energy(oh, exe, got[k], int(sizes[k]), eps, ms[k]) - e0[int(k)]) /
+                     e0[int(k)]) for k in idx]
         block[name]["energy_rel_drift_64_systems"] = {"median": float(np.median(drift)),
                                                       "max": float(np.max(drift))}
+    p, c = block["plain_exact"], block["chosen"]
+    fe = c["final_state_rel_err_vs_plain"]
+    block["targets"] = {
+        "particle_steps_per_s_vs_plain": c["particle_steps_per_s"] / p["particle_steps_per_s"],
+        "rhs_evals_per_solve_vs_plain": c["rhs_evals_per_solve"] / p["rhs_evals_per_solve"],
+        "final_state_max_rel_err": fe["max"], "final_state_median_rel_err": fe["median"],
+        "energy_drift_median_chosen": c["energy_rel_drift_64_systems"]["median"],
+        "energy_drift_median_plain": p["energy_rel_drift_64_systems"]["median"],
+        "beats_plain_particle_steps": c["particle_steps_per_s"] > p["particle_steps_per_s"],
+        "beats_plain_rhs_evals": c["rhs_evals_per_solve"] < p["rhs_evals_per_solve"],
+        "final_state_le_1e-9": fe["max"] <= 1e-9,
+        "energy_drift_no_worse": (c["energy_rel_drift_64_systems"]["median"] <=
+                                  p["energy_rel_drift_64_systems"]["median"])}
+    log(f"ensemble targets: {block['targets']}")
     # DOPRI5 dense output of every system at 20 common times (fast FP64 RHS)
     times = np.linspace(0.0, tf, 21)[1:]
     exe.synchronize()
@@ -201,28 +226,110 @@ def ensemble_block(oh, exe):
     return block
 
 
-def ensemble_cpu_baseline(sizes, x0, m, eps, tol, tab, tf=0.005, count=16):
-    """The oracle restatement of the reference algorithm (per-system solves,
-    thread-parallel gather RHS) on a bounded sample of the ensemble."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as O
+def ensemble_horizon_block(oh, exe, tf=2.0, chunk=0.125):
+    """C4 over its configured horizon t in [0, tf] for the plain, fast FP64 + k1 reuse and
+    chosen paths.  Each path runs in chunks of `chunk` (every chunk a fresh solve from
+    the previous chunk's end state, logged as it ends), so no call is silent for minutes;
+    a restart re-estimates every system's initial step.  Reports time, RHS evaluations
+    and energy drift at tf over 64 systems (final states at tf are not compared: over
+    this horizon close encounters make the systems chaotic)."""
+    from ode_prototyping_amd.ensemble import solve_ensemble, unpack
+    from ode_prototyping_amd import train_controller as trc
+    eps, tol, tab = 1e-3, 1e-10, "DOPRI5"
+    sizes, x0, m = trc.workload_ensemble(2048, 0)
+    md = oh.HipState(exe, data=m)
+    a, b = oh.HipState(exe, data=x0), oh.HipState(exe, x0.size)
+    net, info = trc.pretrained(exe, tab)
+    close = oh.MIXED_NEAR_CLOSE
+    runs = {"plain_exact": ("exact", "reference", None, 0.0, False),
+            "fast_fsal": ("fast", "reference", None, 0.0, True),
+            "chosen": ("mixed", "learned", net, close, True)}
+    ms = unpack(m, sizes, per_particle=1)
+    x0s = unpack(x0, sizes)
+    idx = np.linspace(0, len(sizes) - 1, 64).astype(int)
+    e0 = {int(k): nbody_energy(oh, exe, x0s[k], int(sizes[k]), eps, ms[k]) for k in idx}
+    n_chunks = int(round(tf / chunk))
+    block = {"workload": f"{len(sizes)} systems as the ensemble block, t in [0, {tf}] in "
+                         f"{n_chunks} chunks of {chunk}", "learned_net": info}
+    for name, (var, ctrl, mlp, near, fsal) in runs.items():
+        exe.near_field = near
+        a.upload(x0)
+        secs = evals = steps = 0.0
+        for c in range(n_chunks):
+            t0, t1 = c * chunk, (c + 1) * chunk
+            exe.synchronize()
+            t = time.perf_counter()
+            r = solve_ensemble(tab, sizes, a, t0, t1, tol, tol, eps, md, variant=var, exe=exe,
+                               controller=ctrl, mlp=mlp, fsal=fsal, x_out=b, fetch=False)
+            exe.synchronize()
+            secs += time.perf_counter() - t
+            evals += r.total_rhs_evals
+            steps += r.particle_steps
+            a, b = b, a
+            log(f"horizon {name}: chunk {c + 1}/{n_chunks} t={t1:g} {secs:.1f} s")
+        got = unpack(a.copy_to_span(), sizes)
+        drift = [abs((nbody_energy(oh, exe, got[k], int(sizes[k]), eps, ms[k]) - e0[int(k)]) /
+                     e0[int(k)]) for k in idx]
+        block[name] = {"seconds": secs, "particle_steps_per_s": steps / secs,
+                       "rhs_evals_per_solve": evals / len(sizes),
+                       "energy_rel_drift_64_systems": {"median": float(np.median(drift)),
+                                                       "max": float(np.max(drift))}}
+        log(f"horizon {name}: {block[name]}")
+    exe.near_field = oh.MIXED_NEAR_DEFAULT
+    net.close()
+    return block
+
+
+def ensemble_cpu_baseline(sizes, x0, m, eps, tol, tab, tf=0.005, count=12):
+    """CPU baseline of the ensemble workload on a bounded sample: 12 systems spread over
+    the size order, each integrated alone to tf.  The reference's own ThreadPoolExecutor
+    path (oracle/_ref: RKEmbeddedParallel<BTDOPRI5> + NBodyODE, instantiated at these
+    systems' sizes) where built; otherwise the oracle restatement."""
     from ode_prototyping_amd.ensemble import unpack
     threads = max(1, min(16, len(os.sched_getaffinity(0))))
     order = np.argsort(sizes, kind="stable")
     idx = order[np.linspace(0, len(sizes) - 1, count).astype(int)]
     xs, ms = unpack(x0, sizes), unpack(m, sizes, per_particle=1)
-    t = time.perf_counter()
+    ref = os.path.join(ROOT, "oracle", "_ref", "libref.so")
     steps = 0.0
-    for k in idx:
-        r = O.integrate(tab, O.RHS_NBODY, xs[k], 0.0, tf, tol, tol, softening=eps,
-                        masses=ms[k], store_states=False, threads=threads)
-        steps += float(sizes[k]) * r.accepted
-    secs = time.perf_counter() - t
+    if os.path.exists(ref) and tab == "DOPRI5":
+        L = C.CDLL(ref)
+        dp = C.POINTER(C.c_double)
+        L.ref_solve_dopri5_n.argtypes = [C.c_int, C.c_int, dp, C.c_double, C.c_double,
+                                         C.c_double, C.c_double, dp, C.POINTER(C.c_long)]
+        L.ref_solve_dopri5_n.restype = C.c_double
+        secs = 0.0
+        for k in idx:
+            n = int(sizes[k])
+            xo = np.zeros(6 * n)
+            saved = C.c_long()
+            dt = L.ref_solve_dopri5_n(n, threads, np.ascontiguousarray(xs[k]).ctypes.data_as(dp),
+                                      tol, eps, float(ms[k][0]), tf, xo.ctypes.data_as(dp),
+                                      C.byref(saved))
+            if dt < 0:
+                raise RuntimeError(f"oracle/_ref has no DOPRI5 instance for N={n}")
+            secs += dt
+            steps += float(n) * (saved.value - 1)
+        kind = "reference"
+        what = (f"reference RKEmbeddedParallel<DOPRI5> + ThreadPoolExecutor({threads}) + "
+                f"NBodyODE (atomic pair RHS), {count} systems spread over the size order "
+                f"(N {int(sizes[idx].min())}..{int(sizes[idx].max())}), each solved alone for "
+                f"t in [0, {tf}], uniform masses applied by scaling the unit-mass RHS")
+    else:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        t = time.perf_counter()
+        for k in idx:
+            r = O.integrate(tab, O.RHS_NBODY, xs[k], 0.0, tf, tol, tol, softening=eps,
+                            masses=ms[k], store_states=False, threads=threads)
+            steps += float(sizes[k]) * r.accepted
+        secs = time.perf_counter() - t
+        kind = "port"
+        what = (f"oracle restatement ({threads} threads), {count} systems spread over "
+                f"the size order (N {int(sizes[idx].min())}..{int(sizes[idx].max())}), "
+                f"each solved alone for t in [0, {tf}]")
     return {"value": steps / secs, "unit": "particle-steps/s", "cores": threads,
-            "kind": "port", "seconds": secs,
-            "sample": f"oracle restatement ({threads} threads), {count} systems spread over "
-                      f"the size order (N {int(sizes[idx].min())}..{int(sizes[idx].max())}), "
-                      f"each solved alone for t in [0, {tf}]"}
+            "kind": kind, "seconds": secs, "sample": what}
 
 
 def controller_block(oh, exe):
@@ -238,27 +345,44 @@ def controller_block(oh, exe):
                          f"[16, 4096], eps={eps}, {tab}, rtol=atol={tol}, t in [0, {tf}], "
                          "mixed-precision RHS", "learned_net": info}
     w2 = sizes.astype(float) ** 2
-    for ctrl in ("reference", "pi", "learned"):
+    # (key, RHS variant, near-field coefficient, controller): the mixed RHS as trained on,
+    # then with the close-encounter near field and in FP64, where the error norm carries
+    # less rounding noise
+    runs = [("reference", "mixed", 0.0, "reference"), ("pi", "mixed", 0.0, "pi"),
+            ("learned", "mixed", 0.0, "learned"),
+            ("near_reference", "mixed", oh.MIXED_NEAR_CLOSE, "reference"),
+            ("near_learned", "mixed", oh.MIXED_NEAR_CLOSE, "learned"),
+            ("fast_reference", "fast", 0.0, "reference"),
+            ("fast_learned", "fast", 0.0, "learned")]
+    for key, var, near, ctrl in runs:
         mlp = net if ctrl == "learned" else None
-        solve_ensemble(tab, sizes, x0d, 0.0, tf, tol, tol, eps, md, variant="mixed", exe=exe,
+        exe.near_field = near
+        solve_ensemble(tab, sizes, x0d, 0.0, tf, tol, tol, eps, md, variant=var, exe=exe,
                        controller=ctrl, mlp=mlp, max_attempts=1, x_out=out_state, fetch=False)
         exe.synchronize()
         t = time.perf_counter()
-        r = solve_ensemble(tab, sizes, x0d, 0.0, tf, tol, tol, eps, md, variant="mixed",
+        r = solve_ensemble(tab, sizes, x0d, 0.0, tf, tol, tol, eps, md, variant=var,
                            exe=exe, controller=ctrl, mlp=mlp, x_out=out_state, fetch=False)
         exe.synchronize()
         secs = time.perf_counter() - t
-        block[ctrl] = {"seconds": secs, "rhs_evals_per_solve": r.total_rhs_evals / len(sizes),
-                       "pair_work_per_solve": float(np.sum(w2 * r.rhs_evals)) / len(sizes),
-                       "accepted": r.total_accepted, "rejected": r.total_rejected,
-                       "rounds": r.rounds, "learned_steps": r.total_learned_steps,
-                       "particle_steps_per_s": r.particle_steps / secs}
-        log(f"controller {ctrl}: {block[ctrl]}")
-    ref = block["reference"]
-    block["learned_gain"] = {
-        "rhs_evals": 1.0 - block["learned"]["rhs_evals_per_solve"] / ref["rhs_evals_per_solve"],
-        "pair_work": 1.0 - block["learned"]["pair_work_per_solve"] / ref["pair_work_per_solve"],
-        "seconds": 1.0 - block["learned"]["seconds"] / ref["seconds"]}
+        block[key] = {"variant": var, "near_field": near, "controller": ctrl, "seconds": secs,
+                      "rhs_evals_per_solve": r.total_rhs_evals / len(sizes),
+                      "pair_work_per_solve": float(np.sum(w2 * r.rhs_evals)) / len(sizes),
+                      "accepted": r.total_accepted, "rejected": r.total_rejected,
+                      "rounds": r.rounds, "learned_steps": r.total_learned_steps,
+                      "particle_steps_per_s": r.particle_steps / secs}
+        log(f"controller {key}: {block[key]}")
+    exe.near_field = oh.MIXED_NEAR_DEFAULT
+
+    def gain(learned, reference):
+        lr, rf = block[learned], block[reference]
+        return {"rhs_evals": 1.0 - lr["rhs_evals_per_solve"] / rf["rhs_evals_per_solve"],
+                "pair_work": 1.0 - lr["pair_work_per_solve"] / rf["pair_work_per_solve"],
+                "seconds": 1.0 - lr["seconds"] / rf["seconds"]}
+
+    block["learned_gain"] = gain("learned", "reference")
+    block["learned_gain_near_field"] = gain("near_learned", "near_reference")
+    block["learned_gain_fast_fp64"] = gain("fast_learned", "fast_reference")
     net.close()
     return block
 
@@ -386,6 +510,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-tf", type=float, default=0.1,
+                    help="end time of the ensemble block (the configuration's horizon is 2)")
+    ap.add_argument("--ensemble-horizon", action="store_true",
+                    help="only run the ensemble over t in [0, 2] (or --ensemble-tf if larger "
+                         "than 0.1) in chunks: plain, fast FP64 + k1 reuse and chosen paths")
     ap.add_argument("--no-controller", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-outputs", action="store_true",
@@ -393,6 +522,11 @@ def main():
     ap.add_argument("--profile-ensemble", action="store_true",
                     help="only run short ensemble solves (2048 systems, t in [0, 0.005]) in "
                          "each RHS variant (the workload of the committed ensemble profiles)")
+    ap.add_argument("--profile-ensemble-rhs", action="store_true",
+                    help="only run the ensemble right-hand side of the 2048-system input 5x "
+                         "per variant (exact, fast, mixed, mixed with the near field), tile "
+                         "configuration 7, no graphs (the workload of the committed ensemble "
+                         "RHS profile)")
     ap.add_argument("--profile-rhs", action="store_true",
                     help="only run each RHS variant 3x, a 64 MiB calibration copy and "
                          "the near-field mixed RHS 3x (the workload of the committed "
@@ -423,6 +557,29 @@ def main():
             b.assign(a)
         exe.synchronize()
         log("profile-rhs done")
+        return
+
+    if args.ensemble_horizon:
+        exe = oh.HipExecutor(0)
+        print(json.dumps({"ensemble_horizon": ensemble_horizon_block(oh, exe, args.ensemble_tf
+                                                                      if args.ensemble_tf > 0.1
+                                                                      else 2.0)}), flush=True)
+        return
+
+    if args.profile_ensemble_rhs:
+        from ode_prototyping_amd.ensemble import ensemble_rhs
+        from ode_prototyping_amd import train_controller as trc
+        sizes, x0, m = trc.workload_ensemble(2048, 0)
+        exe = oh.HipExecutor(0)
+        x0d, md = oh.HipState(exe, data=x0), oh.HipState(exe, data=m)
+        kd = oh.HipState(exe, x0.size)
+        for var, near in (("exact", 0.0), ("fast", 0.0), ("mixed", 0.0),
+                          ("mixed", oh.MIXED_NEAR_CLOSE)):
+            exe.near_field = near
+            ms = [ensemble_rhs(exe, sizes, x0d, 1e-3, md, var, tile_config=7, dxdt=kd)[1]
+                  for _ in range(5)]
+            log(f"profile-ensemble-rhs {var} near {near}: {min(ms):.3f} ms (HIP events)")
+        exe.near_field = oh.MIXED_NEAR_DEFAULT
         return
 
     if args.profile_ensemble:
@@ -572,7 +729,7 @@ def main():
                                        args.accepted, wall / steps, r.t)
 
     if not args.no_ensemble:
-        out["ensemble"] = ensemble_block(oh, exe)
+        out["ensemble"] = ensemble_block(oh, exe, args.ensemble_tf)
     if not args.no_controller:
         out["controller"] = controller_block(oh, exe)
     if not args.no_precision:

@@ -211,4 +211,46 @@ double ref_solve_64_dopri5(int threads, const double* x0, double tol,
   return std::chrono::duration<double>(t_end - t_start).count();
 }
 
+// CPU baseline of the ensemble workload: the reference's thread-pool (or,
+// with threads <= 1, single-threaded) DOPRI5 integration of one system of n
+// bodies to tf, uniform masses m by scaling the unit-mass RHS.  The reference
+// sizes states at compile time, so n must be one of the instantiated sizes
+// below: those of 12 systems spread over the size order of the 2048-system
+// ensemble (N log-uniform in [16, 4096], seed 0).  Returns wall seconds
+// (-1: size not instantiated); *n_saved = accepted steps + 1.
+double ref_solve_dopri5_n(int n, int threads, const double* x0, double tol,
+                          double softening, double m, double tf, double* x_out,
+                          long* n_saved) {
+  auto run = [&]<std::size_t N>() {
+    constexpr std::size_t NV = N * 6;
+    auto t_start = std::chrono::steady_clock::now();
+    long k;
+    if (threads <= 1) {
+      SingleThreadedExecutor exe;
+      k = run_nbody<BTDOPRI5, NV>(exe, x0, 0.0, tf, tol, softening, m, 0, x_out, nullptr, 0);
+    } else {
+      ThreadPoolExecutor exe{threads};
+      k = run_nbody<BTDOPRI5, NV>(exe, x0, 0.0, tf, tol, softening, m, 0, x_out, nullptr, 0);
+    }
+    auto t_end = std::chrono::steady_clock::now();
+    if (n_saved) *n_saved = k;
+    return std::chrono::duration<double>(t_end - t_start).count();
+  };
+  switch (n) {
+    case 16: return run.template operator()<16>();
+    case 26: return run.template operator()<26>();
+    case 43: return run.template operator()<43>();
+    case 71: return run.template operator()<71>();
+    case 116: return run.template operator()<116>();
+    case 192: return run.template operator()<192>();
+    case 332: return run.template operator()<332>();
+    case 542: return run.template operator()<542>();
+    case 912: return run.template operator()<912>();
+    case 1528: return run.template operator()<1528>();
+    case 2474: return run.template operator()<2474>();
+    case 4085: return run.template operator()<4085>();
+    default: return -1.0;
+  }
+}
+
 }  // extern "C"

@@ -201,16 +201,19 @@ def test_c4_ensemble_rhs_exact_every_system(exe, c4):
 
 
 def test_c4_short_ensemble_solve_sample(exe, c4):
-    """C4's ensemble (DOPRI5, rtol = atol = 1e-10, eps 1e-3) solved to t = 0.002 in one
+    """C4's ensemble (DOPRI5, rtol = atol = 1e-10, eps 1e-3) solved to t = 0.01 in one
     call: 24 systems spread over the size order (largest included) equal their own oracle
-    solves bit for bit (final state, counters, per-attempt trace)."""
+    solves bit for bit (final state, end time, counters, per-attempt trace).  As in the
+    reference, the first step is not clipped to tf (RKEmbeddedParallel.hpp:27-35 clips
+    only after the controller), so a system whose initial-step estimate exceeds tf ends
+    after it -- in both executions alike."""
     from ode_prototyping_amd.ensemble import solve_ensemble
     sizes, x0, m, xs, ms = c4
-    tf = 0.002
-    print("C4 ensemble solve to t=0.002", flush=True)
+    tf = 0.01
+    print("C4 ensemble solve to t=0.01", flush=True)
     res = solve_ensemble("DOPRI5", sizes, x0, 0.0, tf, 1e-10, 1e-10, 1e-3, m,
-                         variant="exact", exe=exe, trace_cap=512)
-    assert res.n_unfinished == 0 and np.all(res.t == tf)
+                         variant="exact", exe=exe, trace_cap=1024)
+    assert res.n_unfinished == 0 and np.all(res.t >= tf)
     order = np.argsort(sizes, kind="stable")
     sample = sorted(set(order[np.linspace(0, len(sizes) - 1, 24).astype(int)].tolist()))
     finals = res.states()
@@ -218,7 +221,8 @@ def test_c4_short_ensemble_solve_sample(exe, c4):
         ref = oracle("DOPRI5", xs[k], tf, 1e-3, ms[k], store_states=False)
         assert (res.accepted[k], res.rejected[k], res.rhs_evals[k]) == (
             ref.accepted, ref.rejected, ref.rhs_evals), k
-        assert res.dt[k] == ref.dt and res.dt_initial[k] == ref.dt_initial
+        assert res.t[k] == ref.t and res.dt[k] == ref.dt
+        assert res.dt_initial[k] == ref.dt_initial
         assert np.array_equal(bits(finals[k]), bits(ref.x_final)), (k, sizes[k])
         tr = res.trace[k]
         assert np.array_equal(bits(tr[:, 0]), bits(ref.trace_err))

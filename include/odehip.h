@@ -210,6 +210,17 @@ int odehip_rk_update_error_target(odehip_context* ctx, double* sc,
  * (as CudaExecutor::transform_reduce does, CudaExecutor.cuh:117-119). */
 int odehip_rk_norm(odehip_context* ctx, const double* v, const double* sc,
                    size_t n_var, double* host_result);
+/* odehip_rk_norm without any allocation: no counterpart in the reference;
+ * for callers that cannot allocate (code under graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same kernels and result bit for
+ * bit; its group partials live in a static device array the library holds on
+ * each device until the process ends (8 + ODEHIP_NOALLOC_MAX_N_VAR / 256
+ * doubles, 512 KiB), so n_var > ODEHIP_NOALLOC_MAX_N_VAR is rejected
+ * (ODEHIP_ERR_INVALID).  Every call on a device shares that array: calls that
+ * overlap on one device (other contexts, other host threads) race. */
+#define ODEHIP_NOALLOC_MAX_N_VAR (1 << 24)
+int odehip_rk_norm_noalloc(odehip_context* ctx, const double* v, const double* sc,
+                           size_t n_var, double* host_result);
 /* detail::compute_error_target_kernel (RKEmbeddedParallel.hpp:151-157):
  * sc = |x0| * rtol + atol. */
 int odehip_rk_compute_error_target(odehip_context* ctx, double* sc,

@@ -137,6 +137,7 @@ SIGNATURES = {
     "odehip_rk_update_error_target": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_void_p,
                                                 C.c_void_p, C.c_void_p, C.c_size_t]),
     "odehip_rk_norm": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_size_t, _dp]),
+    "odehip_rk_norm_noalloc": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_size_t, _dp]),
     "odehip_rk_compute_error_target": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_void_p,
                                                  C.c_void_p, C.c_size_t]),
     "odehip_rk_euler_step": (C.c_int, [_ctxp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,

@@ -58,6 +58,8 @@ struct odehip_context {
   // small scratch for reductions
   double* scratch = nullptr;
   size_t scratch_cap = 0;
+  // this device's copy of the allocation-free norm's static array (odehip_abi.hip)
+  double* noalloc_scratch = nullptr;
   // pinned host staging
   int* h_done = nullptr;           // 2 slots
   odehip::SolverState* h_st = nullptr;

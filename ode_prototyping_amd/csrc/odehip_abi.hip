@@ -82,6 +82,38 @@ int odehip::use_tableau(odehip_context* ctx, int id) {
   return ODEHIP_OK;
 }
 
+// ---------------------------------------------- allocation-free norm ----
+// Group partials (from element 8) and result (element 0) of
+// odehip_rk_norm_noalloc: a file-scope device array, so that call allocates
+// nothing.  Every device holds its own copy until the process ends; host code
+// uses the address hipGetSymbolAddress reports for a device, never the
+// array's name (which on the host is only a stand-in symbol).
+constexpr size_t kNoAllocSlots = 8 + ODEHIP_NOALLOC_MAX_N_VAR / ODEHIP_REDUCE_GROUP;
+static __device__ __align__(256) double scratch[kNoAllocSlots];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+// The array's address on ctx's device, resolved once per device (that
+// device current: odehip_create has just selected it).
+hipError_t resolve_noalloc_scratch(odehip_context* ctx) {
+  if (ctx->device < 0 || ctx->device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[ctx->device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[ctx->device] = static_cast<double*>(p);
+  }
+  ctx->noalloc_scratch = g_scratch[ctx->device];
+  return hipSuccess;
+}
+
+}  // namespace
+
 namespace {
 
 int ensure_ws(odehip_context* ctx, size_t n) {
@@ -200,6 +232,7 @@ int odehip_create(int device, odehip_context** out) {
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_out, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreate(&ctx->t_start);
   if (e == hipSuccess) e = hipEventCreate(&ctx->t_stop);
+  if (e == hipSuccess) e = resolve_noalloc_scratch(ctx);
   if (e != hipSuccess) {
     odehip_destroy(ctx);
     return ODEHIP_ERR_HIP;
@@ -524,6 +557,27 @@ int odehip_rk_norm(odehip_context* ctx, const double* v, const double* sc,
   HIP_TRY(ctx, launch_norm_partials(v, nullptr, sc, n_var, ctx->scratch + 8, ctx->stream));
   HIP_TRY(ctx, launch_fold_norm(ctx->scratch + 8, ng, n_var, ctx->scratch, ctx->stream));
   HIP_TRY(ctx, hipMemcpyAsync(ctx->h_scalar, ctx->scratch, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  *host_result = ctx->h_scalar[0];
+  return ODEHIP_OK;
+}
+
+int odehip_rk_norm_noalloc(odehip_context* ctx, const double* v, const double* sc,
+                           size_t n_var, double* host_result) {
+  ODEHIP_USE_DEVICE(ctx);
+  if (!ctx || !v || !sc || !host_result) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
+  if (n_var == 0) return fail(ctx, ODEHIP_ERR_INVALID, "n_var is 0");
+  if (n_var > ODEHIP_NOALLOC_MAX_N_VAR)
+    return fail(ctx, ODEHIP_ERR_INVALID, "n_var exceeds ODEHIP_NOALLOC_MAX_N_VAR");
+  if (!ctx->noalloc_scratch) return fail(ctx, ODEHIP_ERR_HIP, "static scratch not resolved");
+  // the kernels of odehip_rk_norm; every partial and the result are written
+  // before they are read, so nothing needs clearing between calls
+  double* part = ctx->noalloc_scratch + 8;
+  const int ng = n_groups_for(n_var);
+  HIP_TRY(ctx, launch_norm_partials(v, nullptr, sc, n_var, part, ctx->stream));
+  HIP_TRY(ctx, launch_fold_norm(part, ng, n_var, ctx->noalloc_scratch, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->h_scalar, ctx->noalloc_scratch, sizeof(double),
+                              hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   *host_result = ctx->h_scalar[0];
   return ODEHIP_OK;

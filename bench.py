@@ -523,10 +523,10 @@ def main():
                     help="only run short ensemble solves (2048 systems, t in [0, 0.005]) in "
                          "each RHS variant (the workload of the committed ensemble profiles)")
     ap.add_argument("--profile-ensemble-rhs", action="store_true",
-                    help="only run the ensemble right-hand side of the 2048-system input 5x "
-                         "per variant (exact, fast, mixed, mixed with the near field), tile "
-                         "configuration 7, no graphs (the workload of the committed ensemble "
-                         "RHS profile)")
+                    help="only run the ensemble right-hand side of the 2048-system input 8x "
+                         "per variant (exact, fast, mixed, mixed with the near field; the last "
+                         "5 timed), tile configuration 7, no graphs (the workload of the "
+                         "committed ensemble RHS profile)")
     ap.add_argument("--profile-rhs", action="store_true",
                     help="only run each RHS variant 3x, a 64 MiB calibration copy and "
                          "the near-field mixed RHS 3x (the workload of the committed "
@@ -577,7 +577,7 @@ def main():
                           ("mixed", oh.MIXED_NEAR_CLOSE)):
             exe.near_field = near
             ms = [ensemble_rhs(exe, sizes, x0d, 1e-3, md, var, tile_config=7, dxdt=kd)[1]
-                  for _ in range(5)]
+                  for _ in range(8)][3:]  # three warm-up launches, then five
             log(f"profile-ensemble-rhs {var} near {near}: {min(ms):.3f} ms (HIP events)")
         exe.near_field = oh.MIXED_NEAR_DEFAULT
         return

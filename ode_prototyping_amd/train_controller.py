@@ -39,8 +39,11 @@ def workload_ensemble(count, seed, n_min=16, n_max=4096):
     return sizes, pack(xs), pack(ms)
 
 
-def collect_traces(exe, tableau, seeds, systems, attempts, variant="mixed", log=None):
+def collect_traces(exe, tableau, seeds, systems, attempts, variant="mixed", log=None,
+                   near_field=0.0):
     traces, sizes_all = [], []
+    saved = exe.near_field
+    exe.near_field = near_field
     for seed in seeds:
         sizes, x0, m = workload_ensemble(systems, seed)
         res = solve_ensemble(tableau, sizes, x0, 0.0, 2.0, 1e-10, 1e-10, 1e-3, m,
@@ -50,17 +53,19 @@ def collect_traces(exe, tableau, seeds, systems, attempts, variant="mixed", log=
         sizes_all += [int(n) for n in sizes]
         if log:
             log(f"seed {seed}: {res.total_attempts} attempts, {res.total_rejected} rejected")
+    exe.near_field = saved
     return traces, sizes_all
 
 
 def train(exe, tableau, seeds=range(1000, 1008), systems=256, attempts=400, tau=0.65,
-          epochs=10, batch=4096, lr=2e-3, seed=1, variant="mixed", log=None):
-    """Returns (StepMLP, info dict).  `variant`: the right-hand side the
-    traces are integrated with -- the one the controller will run with
-    (its error norms carry that variant's rounding noise)."""
+          epochs=10, batch=4096, lr=2e-3, seed=1, variant="mixed", near_field=0.0, log=None):
+    """Returns (StepMLP, info dict).  `variant` (and `near_field`, its FP64 near-pair
+    coefficient for the mixed variant): the right-hand side the traces are integrated
+    with -- the one the controller will run with (its error norms carry that variant's
+    rounding noise)."""
     t0 = time.perf_counter()
     traces, sizes = collect_traces(exe, tableau, seeds, systems, attempts, variant=variant,
-                                   log=log)
+                                   log=log, near_field=near_field)
     t_traces = time.perf_counter() - t0
     k = ORDERS[tableau]
     x, t = ctl.training_pairs(traces, sizes, 2.0, k, tau=tau)
@@ -70,6 +75,7 @@ def train(exe, tableau, seeds=range(1000, 1008), systems=256, attempts=400, tau=
     losses = net.train(x, t, epochs=epochs, batch=batch, lr=lr, seed=seed + 1)
     t_train = time.perf_counter() - t1
     info = {"tableau": tableau, "k": k, "tau": tau, "variant": variant,
+            "near_field": near_field,
             "samples": int(x.shape[0]),
             "seeds": [int(s) for s in seeds], "systems_per_seed": systems,
             "attempts_per_system": attempts, "epochs": epochs, "batch": batch, "lr": lr,
@@ -108,6 +114,8 @@ def main(argv=None):
     ap.add_argument("--seeds", type=int, default=8)
     ap.add_argument("--tau", type=float, default=0.65)
     ap.add_argument("--variant", default="mixed", choices=["exact", "fast", "mixed"])
+    ap.add_argument("--near-field", type=float, default=0.0,
+                    help="FP64 near-pair coefficient of the mixed RHS for the traces")
     ap.add_argument("--epochs", type=int, default=10)
     args = ap.parse_args(argv)
     exe = HipExecutor(0)
@@ -117,7 +125,8 @@ def main(argv=None):
 
     net, info = train(exe, args.tableau, seeds=range(1000, 1000 + args.seeds),
                       systems=args.systems, attempts=args.attempts, tau=args.tau,
-                      epochs=args.epochs, variant=args.variant, log=log)
+                      epochs=args.epochs, variant=args.variant,
+                      near_field=args.near_field, log=log)
     save(net, info, args.out)
     log(json.dumps(info))
 

@@ -226,6 +226,51 @@ def ensemble_block(oh, exe, tf=0.1):
     return block
 
 
+def near_sweep_block(oh, exe, tf=0.1, coefficients=(20.0, 100.0, 500.0, 2000.0)):
+    """C4 to t = tf with the mixed RHS (reference controller, k1 reuse) at several
+    near-field coefficients c (pairs with r^2 < c / N in FP64), against the plain FP64 path
+    and fast FP64 + k1 reuse: time, RHS evaluations, final state vs plain, energy drift."""
+    from ode_prototyping_amd.ensemble import solve_ensemble, unpack
+    from ode_prototyping_amd import train_controller as trc
+    eps, tol, tab = 1e-3, 1e-10, "DOPRI5"
+    sizes, x0, m = trc.workload_ensemble(2048, 0)
+    x0d, md = oh.HipState(exe, data=x0), oh.HipState(exe, data=m)
+    out_state = oh.HipState(exe, x0.size)
+    ms, x0s = unpack(m, sizes, per_particle=1), unpack(x0, sizes)
+    idx = np.linspace(0, len(sizes) - 1, 64).astype(int)
+    e0 = {int(k): nbody_energy(oh, exe, x0s[k], int(sizes[k]), eps, ms[k]) for k in idx}
+    runs = [("plain_exact", "exact", 0.0, False), ("fast_fsal", "fast", 0.0, True)]
+    runs += [(f"mixed_near_{c:g}", "mixed", c, True) for c in coefficients]
+    block, ref = {"workload": f"2048-system ensemble, {tab}, t in [0, {tf}], reference "
+                              "controller"}, None
+    for name, var, near, fsal in runs:
+        exe.near_field = near
+        kw = dict(variant=var, exe=exe, fsal=fsal, x_out=out_state, fetch=False)
+        solve_ensemble(tab, sizes, x0d, 0.0, tf, tol, tol, eps, md, max_attempts=1, **kw)
+        exe.synchronize()
+        t = time.perf_counter()
+        r = solve_ensemble(tab, sizes, x0d, 0.0, tf, tol, tol, eps, md, **kw)
+        exe.synchronize()
+        secs = time.perf_counter() - t
+        got = unpack(out_state.copy_to_span(), sizes)
+        row = {"near_field": near, "seconds": secs,
+               "rhs_evals_per_solve": r.total_rhs_evals / len(sizes), "rhs_ms": r.rhs_ms}
+        if ref is None:
+            ref = got
+        else:
+            err = np.array([np.max(np.abs(a - b)) / np.max(np.abs(b)) for a, b in zip(got, ref)])
+            row["final_state_rel_err_vs_plain"] = {"median": float(np.median(err)),
+                                                   "max": float(np.max(err)),
+                                                   "frac_le_1e-9": float(np.mean(err <= 1e-9))}
+        drift = [abs((nbody_energy(oh, exe, got[k], int(sizes[k]), eps, ms[k]) - e0[int(k)]) /
+                     e0[int(k)]) for k in idx]
+        row["energy_rel_drift_median"] = float(np.median(drift))
+        block[name] = row
+        log(f"near sweep {name}: {row}")
+    exe.near_field = oh.MIXED_NEAR_DEFAULT
+    return block
+
+
 def ensemble_horizon_block(oh, exe, tf=2.0, chunk=0.125):
     """C4 over its configured horizon t in [0, tf] for the plain, fast FP64 + k1 reuse and
     chosen paths.  Each path runs in chunks of `chunk` (every chunk a fresh solve from
@@ -512,6 +557,9 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-tf", type=float, default=0.1,
                     help="end time of the ensemble block (the configuration's horizon is 2)")
+    ap.add_argument("--near-sweep", action="store_true",
+                    help="only run the 2048-system ensemble (t <= 0.1) with the mixed RHS at "
+                         "near-field coefficients 20 .. 2000, against plain and fast FP64")
     ap.add_argument("--ensemble-horizon", action="store_true",
                     help="only run the ensemble over t in [0, 2] (or --ensemble-tf if larger "
                          "than 0.1) in chunks: plain, fast FP64 + k1 reuse and chosen paths")
@@ -557,6 +605,11 @@ def main():
             b.assign(a)
         exe.synchronize()
         log("profile-rhs done")
+        return
+
+    if args.near_sweep:
+        exe = oh.HipExecutor(0)
+        print(json.dumps({"near_sweep": near_sweep_block(oh, exe)}), flush=True)
         return
 
     if args.ensemble_horizon:

@@ -20,8 +20,8 @@
 //            overlaps computing tile t.  Result: bit-identical to NBodyODE
 //            with SingleThreadedExecutor.  Must be compiled with
 //            -ffp-contract=off.
-//  * fast:   FP64, r^-3 from the hardware reciprocal square root refined by one
-//            third-order Newton step (~1 ulp), FMA accumulation, the j range
+//  * fast:   FP64, r^-3 from the hardware reciprocal square root with a
+//            third-order correction (a few ulp), FMA accumulation, the j range
 //            split over 16 waves per workgroup and the partial sums combined
 //            in a fixed order (deterministic, not bit-identical).
 //  * mixed:  displacement computed in FP64 and rounded to FP32, r^-3 and the
@@ -661,6 +661,23 @@ hipError_t launch_momentum_terms(const double* x, int n, const double* mass,
   return hipGetLastError();
 }
 
+namespace {
+// Compute units of the current device (cached per device id).
+int device_cus() {
+  static int cache[64] = {};
+  int d = 0;
+  if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= 64) return 256;
+  if (cache[d] <= 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, d) != hipSuccess ||
+        v <= 0)
+      v = 256;
+    cache[d] = v;
+  }
+  return cache[d];
+}
+}  // namespace
+
 hipError_t launch_nbody_rhs(const NBodyArgs& a, int variant, hipStream_t s) {
   if (a.n <= 0) return hipSuccess;
   const unsigned blocks = static_cast<unsigned>((a.n + 63) / 64);
@@ -670,11 +687,16 @@ hipError_t launch_nbody_rhs(const NBodyArgs& a, int variant, hipStream_t s) {
     case ODEHIP_NBODY_EXACT: {
       // the shared-reciprocal form needs softening^2 in [2^-200, 2^100]
       const bool shared = a.eps2 >= 0x1p-200 && a.eps2 <= 0x1p100;
-      // Both tile shapes sum every target's partners in the same order.  The
-      // 32-target kernel (96 KB LDS: one workgroup per CU) is faster while
-      // its workgroups fit the 256 CUs in one round (N <= 8192); above, the
-      // 64-target kernel, which needs half as many rounds.
-      if (a.n <= 8192) {
+      // Both tile shapes sum every target's partners in the same order and
+      // hold one workgroup per CU (144 KB LDS each), so a launch takes
+      // ceil(workgroups / CUs) rounds.  A round of the 32-target kernel costs
+      // 0.62 of a 64-target one (MI355X sweep, N = 16384 ... 65536), so it
+      // wins while it needs at most as many rounds (N <= 8192 on 256 CUs) and
+      // where the 64-target grid's last round is half empty (e.g. N = 24576).
+      const long cus = device_cus();
+      const long r64 = ((a.n + 63) / 64 + cus - 1) / cus;
+      const long r32 = ((a.n + 31) / 32 + cus - 1) / cus;
+      if (0.62 * static_cast<double>(r32) < static_cast<double>(r64)) {
         const unsigned b32 = static_cast<unsigned>((a.n + 31) / 32);
         const dim3 th(64 * 16);
         if (mass) {

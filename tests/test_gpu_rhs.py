@@ -39,14 +39,15 @@ def test_exact_rhs_bit_identical(exe, n, masses):
         f"n={n}: max abs diff {np.max(np.abs(got - want))}"
 
 
-@pytest.mark.parametrize("n", [4095, 8192, 8193, 8208, 8256, 12289, 16384, 20001])
+@pytest.mark.parametrize("n", [4095, 8192, 8193, 8208, 8256, 12289, 16384, 20001, 24576])
 def test_exact_rhs_bit_identical_large(exe, n):
     """Both tile shapes at the sizes where the dispatch switches (32-target
     tiles up to 8192, 64-target tiles of 48 partners above); above, with a
     partial last tile (8193 = 170 * 48 + 33, 12289 = 256 * 48 + 1, 16384 =
     341 * 48 + 16) and without one (8208 = 171 * 48, 8256 = 172 * 48), so
     the two-tile steady-state loop ends on either parity of full tiles; and
-    20001 bodies, 313 workgroups: more than the 256 CUs run at once."""
+    20001 and 24576 bodies, where the 64-target grid would leave its last round
+    of workgroups half empty and the 32-target kernel runs in 3 rounds."""
     x, m = ic(n, seed=n, masses=True)
     got = gpu_rhs(exe, x, 1e-3, m, "exact")
     want = O.rhs(O.RHS_NBODY, x, 1e-3, m, threads=8)

@@ -85,6 +85,23 @@ __device__ inline bool all_positions_shared_ok(const double* __restrict__ y, int
   return __syncthreads_and(ok) != 0;
 }
 
+// The common mass when all n masses are one power of two 2^k with |k| <= 300,
+// else 0 (every thread tests a strided share).  Then the per-term mass factor
+// can move out of the sum: 2^k t is exact, and scaling by 2^k commutes with
+// every rounding of the in-order sum while the values stay normal, so
+// sum_j (2^k t_j) equals 2^k (sum_j t_j) bit for bit.  On the shared-
+// reciprocal path every nonzero term is >= 2^-602 in magnitude and every
+// nonzero partial sum >= 2^-654 (a multiple of its operands' ulp), at most
+// 2^430, so |k| <= 300 keeps all of them normal and finite.
+__device__ inline double uniform_pow2_mass(const double* __restrict__ mass, int n) {
+  const uint64_t b0 = __builtin_bit_cast(uint64_t, mass[0]);
+  const uint64_t ex = b0 >> 52;  // sign 0 and the biased exponent
+  bool ok = (b0 & 0x000fffffffffffffull) == 0 && ex >= 1023 - 300 && ex <= 1023 + 300;
+  for (int e = threadIdx.x; e < n; e += blockDim.x)
+    ok = ok & (__builtin_bit_cast(uint64_t, mass[e]) == b0);
+  return __syncthreads_and(ok) != 0 ? mass[0] : 0.0;
+}
+
 // ------------------------------------------------------------------ exact --
 constexpr int kExactWaves = 16;
 constexpr int kExactJW = 3;  // partners per wave per tile (48-partner tiles, 144 KB LDS)
@@ -246,10 +263,16 @@ __global__ __launch_bounds__(64 * kExactWaves) void k_nbody_exact(NBodyArgs a) {
   const double yi = valid_i ? y[3 * i + 1] : 0.0;
   const double zi = valid_i ? y[3 * i + 2] : 0.0;
   // the shared-reciprocal form is used when every position coordinate passes
-  // coord_shared_ok (one pass over the 3N coordinates per workgroup)
+  // coord_shared_ok (one pass over the 3N coordinates per workgroup); with
+  // masses that are all one power of two, unit-mass terms and one scale of the
+  // sum (uniform_pow2_mass)
   const bool shared_ok = kSharedRcp && all_positions_shared_ok(y, n);
+  const double m_uni = kMass && kSharedRcp && shared_ok ? uniform_pow2_mass(a.mass, n) : 0.0;
   double acc;
-  if (kSharedRcp && shared_ok)
+  if (kSharedRcp && shared_ok && kMass && m_uni != 0.0)
+    acc = exact_tiles<false, true>(terms, yu, massu, n, a.eps2, xi, yi, zi, wave, lane, i0, i) *
+          m_uni;
+  else if (kSharedRcp && shared_ok)
     acc = exact_tiles<kMass, true>(terms, yu, massu, n, a.eps2, xi, yi, zi, wave, lane, i0, i);
   else
     acc = exact_tiles<kMass, false>(terms, yu, massu, n, a.eps2, xi, yi, zi, wave, lane, i0, i);

@@ -107,13 +107,16 @@ __device__ inline void fast_term(double xj, double yj, double zj, double mj,
   const double dz = zj - zi;
   const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, eps2)));
   // m r^-3 from the hardware rsq y0 with e = 1 - r2 y0^2: r2^(-3/2) =
-  // y0^3 (1 - e)^(-3/2) = y0^3 (1 + 3e/2 + 15e^2/8 + O(e^3)), the correction
-  // applied to m y0^3 directly (8 instructions from r2 to the weight)
+  // y0^3 (1 - e)^(-3/2) = y0^3 (1 + 3e/2 (1 + 5e/4) + O(e^3)), the correction
+  // applied to m y0^3 directly.  Each instruction takes at most one constant
+  // that is not an inline one (the scalar-operand limit), and the longest
+  // dependency chain from y0 is five instructions.
   const double y0 = __builtin_amdgcn_rsq(r2);
   const double y2 = y0 * y0;
   const double e = __builtin_fma(-r2, y2, 1.0);
   const double y3 = y2 * (kMass ? y0 * mj : y0);
-  double w = __builtin_fma(y3 * e, __builtin_fma(e, 1.875, 1.5), y3);
+  const double c = e * __builtin_fma(e, 1.25, 1.0);
+  double w = __builtin_fma(y3 * c, 1.5, y3);
   if (kGuard) w = (j == i) ? 0.0 : w;
   ax = __builtin_fma(dx, w, ax);
   ay = __builtin_fma(dy, w, ay);

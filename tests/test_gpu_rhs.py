@@ -280,3 +280,29 @@ def test_exact_rhs_extreme_inputs_bit_identical(exe, eps, n):
     finite = np.isfinite(want)
     assert np.array_equal(np.isfinite(got), finite)
     assert np.array_equal(got[finite].view(np.uint64), want[finite].view(np.uint64))
+
+
+@pytest.mark.parametrize("case", ["pow2", "pow2_first_only", "not_pow2", "pow2_out_of_range",
+                                  "pow2_extreme_inputs"])
+def test_exact_rhs_uniform_pow2_masses_bit_identical(exe, case):
+    """Masses that are all one power of two 2^k (|k| <= 300) take the unit-mass terms and
+    one exact scale of each sum: still bit-identical to the oracle's per-term products.
+    Arrays that only start with a power of two, a uniform mass that is not one, a power
+    of two outside the range, and a uniform power of two with coordinates outside the
+    shared-reciprocal range (IEEE path) take the per-term path, equally exact."""
+    n = 12289  # the 64-target kernel, with a partial last tile
+    rng = np.random.default_rng(99)
+    x, _ = ic(n, seed=5)
+    m = np.full(n, 2.0 ** -14)
+    if case == "pow2_first_only":
+        m[1:] = rng.uniform(0.5, 1.5, n - 1) / n
+    elif case == "not_pow2":
+        m[:] = 1.0 / 12289.0
+    elif case == "pow2_out_of_range":
+        m[:] = 2.0 ** -400
+    elif case == "pow2_extreme_inputs":
+        x[3] = 1e35  # a coordinate above 2^98: the launch takes the IEEE path
+    got = gpu_rhs(exe, x, 1e-3, m, "exact")
+    want = O.rhs(O.RHS_NBODY, x, 1e-3, m, threads=8)
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), \
+        f"{case}: max abs diff {np.max(np.abs(got - want))}"

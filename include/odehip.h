@@ -431,8 +431,17 @@ int odehip_mlp_infer(odehip_context* ctx, const odehip_mlp* mlp,
  * are chosen by an autotuner that times the candidates on the ensemble and,
  * among those within 3 % of the fastest, keeps the one with the most target
  * tiles.  When fewer than 4096 tiles of running systems remain, 128-target
- * tiles are split in two for the rest of the solve (results unchanged). */
-#define ODEHIP_ENS_TILE_CONFIGS 12
+ * tiles are split in two for the rest of the solve (results unchanged).
+ *
+ * The fast and mixed variants sum each target's partners in chunks of 128
+ * (increasing j within a chunk, chunk sums added in chunk order).
+ * Configurations ODEHIP_ENS_TILE_UNSPLIT .. ODEHIP_ENS_TILE_CONFIGS-1 (fast
+ * and mixed only; not autotuned) run the chunks of one 64-target tile on 2, 4
+ * or 8 waves of one workgroup, same bits; a solve of these variants moves to
+ * one of them whenever its running systems' tiles would leave SIMDs idle
+ * (at the start for small ensembles, and as systems finish). */
+#define ODEHIP_ENS_TILE_CONFIGS 18
+#define ODEHIP_ENS_TILE_UNSPLIT 12
 #define ODEHIP_ENS_TILE_AUTO (-1)
 
 typedef struct {

@@ -96,7 +96,8 @@ class LeapfrogStats(C.Structure):
                 ("t", C.c_double)]
 
 
-ENS_TILE_CONFIGS = 12
+ENS_TILE_CONFIGS = 18
+ENS_TILE_UNSPLIT = 12  # configurations ENS_TILE_UNSPLIT.. split tiles (fast, mixed only)
 ENS_TILE_AUTO = -1
 
 _ip = C.POINTER(C.c_int)

@@ -53,6 +53,7 @@ struct EnsBufs {
   int variant;
   int wpb;         // waves per RHS workgroup
   int tpl2_min_n;  // systems with n >= this use 128-target tiles
+  int split;       // > 1: waves sharing one tile's partner chunks (fast, mixed)
   double eps2;
   double near_c;  // mixed variant: pairs with FP32 r^2 < near_c / n in FP64
   const double* dense_t;  // DOPRI5 dense output: n_dense times, or null
@@ -60,19 +61,28 @@ struct EnsBufs {
   long n_dense;
 };
 
-// Autotuner candidates: waves per workgroup, the size from which systems use
-// 128-target tiles, and whether tiles of one system stay on one XCD.
+// Tile configurations: waves per workgroup, the size from which systems use
+// 128-target tiles, whether tiles of one system stay on one XCD, and the
+// waves that share one tile's partner chunks (1: a wave per tile).  The
+// autotuner times the unsplit ones (the first ODEHIP_ENS_TILE_UNSPLIT); the
+// split ones serve the reorderable variants when few tiles remain.
 struct EnsTileConfig {
   int wpb;
   int tpl2_min_n;
   int xcd;
+  int split;
 };
 constexpr int kEnsNoTpl2 = 1 << 30;
 constexpr EnsTileConfig kEnsTileConfigs[ODEHIP_ENS_TILE_CONFIGS] = {
-    {4, kEnsNoTpl2, 0}, {4, kEnsNoTpl2, 1}, {4, 1024, 0}, {4, 1024, 1},
-    {4, 256, 0},        {4, 256, 1},        {1, kEnsNoTpl2, 0}, {1, kEnsNoTpl2, 1},
-    {1, 1024, 0},       {1, 1024, 1},       {1, 256, 0},  {1, 256, 1}};
+    {4, kEnsNoTpl2, 0, 1}, {4, kEnsNoTpl2, 1, 1}, {4, 1024, 0, 1}, {4, 1024, 1, 1},
+    {4, 256, 0, 1},        {4, 256, 1, 1},        {1, kEnsNoTpl2, 0, 1}, {1, kEnsNoTpl2, 1, 1},
+    {1, 1024, 0, 1},       {1, 1024, 1, 1},       {1, 256, 0, 1},  {1, 256, 1, 1},
+    {1, kEnsNoTpl2, 0, 2}, {1, kEnsNoTpl2, 1, 2}, {1, kEnsNoTpl2, 0, 4},
+    {1, kEnsNoTpl2, 1, 4}, {1, kEnsNoTpl2, 0, 8}, {1, kEnsNoTpl2, 1, 8}};
 constexpr int kEnsDefaultTileConfig = 1;
+// Partners per chunk of the fast and mixed sums (a multiple of the mixed
+// variant's FP32 flush interval, 32).
+constexpr int kEnsChunk = 128;
 // Waves that give each of the 1024 SIMDs (256 CUs x 4) four, the depth that
 // hides the RHS arithmetic's latency.
 constexpr long kEnsFillWaves = 4096;

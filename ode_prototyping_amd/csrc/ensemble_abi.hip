@@ -145,6 +145,7 @@ EnsBufs make_bufs(EnsembleWorkspace* w, const Plan& p, const Layout& L,
 void set_tile_cfg(EnsBufs* b, int cfg, int n_tiles) {
   b->wpb = kEnsTileConfigs[cfg].wpb;
   b->tpl2_min_n = kEnsTileConfigs[cfg].tpl2_min_n;
+  b->split = kEnsTileConfigs[cfg].split;
   b->n_tiles = n_tiles;
 }
 
@@ -166,7 +167,7 @@ int autotune(odehip_context* ctx, EnsembleWorkspace* w, const Plan& p,
              double* k, hipStream_t s) {
   double best = 1e300;
   int best_cfg = kEnsDefaultTileConfig;
-  for (int c = 0; c < ODEHIP_ENS_TILE_CONFIGS; ++c) {
+  for (int c = 0; c < ODEHIP_ENS_TILE_UNSPLIT; ++c) {
     int n_tiles = 0;
     int rc = upload_tables(ctx, w, p, L, tiles_cap, c, &n_tiles, s);
     if (rc) return rc;
@@ -189,7 +190,7 @@ int autotune(odehip_context* ctx, EnsembleWorkspace* w, const Plan& p,
     }
   }
   long most = -1;
-  for (int c = 0; c < ODEHIP_ENS_TILE_CONFIGS; ++c) {
+  for (int c = 0; c < ODEHIP_ENS_TILE_UNSPLIT; ++c) {
     if (!(w->tuned_ms[c] <= 1.03 * best)) continue;
     const long t = real_tiles(L, kEnsTileConfigs[c]);
     if (t > most || (t == most && w->tuned_ms[c] < w->tuned_ms[best_cfg])) {
@@ -201,10 +202,16 @@ int autotune(odehip_context* ctx, EnsembleWorkspace* w, const Plan& p,
   return ODEHIP_OK;
 }
 
-int check_common(odehip_context* ctx, int nbody_variant, double softening) {
+int check_common(odehip_context* ctx, int nbody_variant, double softening,
+                 int tile_config) {
   if (nbody_variant < 0 || nbody_variant > 2)
     return fail(ctx, ODEHIP_ERR_INVALID, "bad nbody_variant");
   if (!(softening >= 0.0)) return fail(ctx, ODEHIP_ERR_INVALID, "bad softening");
+  if (tile_config < ODEHIP_ENS_TILE_AUTO || tile_config >= ODEHIP_ENS_TILE_CONFIGS)
+    return fail(ctx, ODEHIP_ERR_INVALID, "bad tile_config");
+  if (tile_config >= ODEHIP_ENS_TILE_UNSPLIT && nbody_variant == ODEHIP_NBODY_EXACT)
+    return fail(ctx, ODEHIP_ERR_INVALID,
+                "split tile configurations reorder partner sums: fast and mixed only");
   return ODEHIP_OK;
 }
 
@@ -220,10 +227,8 @@ int odehip_ensemble_rhs(odehip_context* ctx, int n_systems,
   ODEHIP_USE_DEVICE(ctx);
   if (!ctx) return ODEHIP_ERR_INVALID;
   if (!x || !dxdt) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
-  int rc = check_common(ctx, nbody_variant, softening);
+  int rc = check_common(ctx, nbody_variant, softening, tile_config);
   if (rc) return rc;
-  if (tile_config < ODEHIP_ENS_TILE_AUTO || tile_config >= ODEHIP_ENS_TILE_CONFIGS)
-    return fail(ctx, ODEHIP_ERR_INVALID, "bad tile_config");
   Layout L;
   if ((rc = make_layout(ctx, n_systems, n_particles, &L))) return rc;
   const size_t tcap = tiles_capacity(L);
@@ -270,12 +275,10 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
   if (!ctx) return ODEHIP_ERR_INVALID;
   if (!opt || !x0 || !x_out) return fail(ctx, ODEHIP_ERR_INVALID, "null argument");
   if (const int trc = use_tableau(ctx, tableau)) return trc;
-  int rc = check_common(ctx, nbody_variant, softening);
+  int rc = check_common(ctx, nbody_variant, softening, opt->tile_config);
   if (rc) return rc;
   if (opt->max_accepted < 0 || opt->max_attempts < 0 || opt->trace_cap < 0)
     return fail(ctx, ODEHIP_ERR_INVALID, "negative budget or capacity");
-  if (opt->tile_config < ODEHIP_ENS_TILE_AUTO || opt->tile_config >= ODEHIP_ENS_TILE_CONFIGS)
-    return fail(ctx, ODEHIP_ERR_INVALID, "bad tile_config");
   if (opt->controller < ODEHIP_CONTROLLER_REFERENCE ||
       opt->controller > ODEHIP_CONTROLLER_LEARNED)
     return fail(ctx, ODEHIP_ERR_INVALID, "bad controller");
@@ -353,9 +356,13 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
     for (int c = 0; c < ODEHIP_ENS_TILE_CONFIGS; ++c)
       opt->tune_ms[c] = tuned ? w->tuned_ms[c] : 0.0;
   }
+  // the reorderable variants share a tile's partner chunks among several
+  // waves from the start when the ensemble's tiles would leave SIMDs idle
+  const bool reorderable = nbody_variant != ODEHIP_NBODY_EXACT;
+  int run_cfg = reorderable ? tail_config(L, nullptr, cfg, true) : cfg;
   int n_tiles = 0;
-  if ((rc = upload_tables(ctx, w, p, L, tcap, cfg, &n_tiles, s))) return rc;
-  set_tile_cfg(&b, cfg, n_tiles);
+  if ((rc = upload_tables(ctx, w, p, L, tcap, run_cfg, &n_tiles, s))) return rc;
+  set_tile_cfg(&b, run_cfg, n_tiles);
 
   HIP_TRY(ctx, ens_enqueue_init(b, x0, opt->t0, opt->tf, opt->max_accepted,
                                 opt->max_attempts, opt->fsal ? 1 : 0, opt->controller, s));
@@ -400,8 +407,7 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
     w->graph_k = k;
     return ODEHIP_OK;
   };
-  EnsBufs rb = b;
-  int run_cfg = cfg;  // the tile shape of the running systems (cfg until compaction)
+  EnsBufs rb = b;  // run_cfg: the tile shape of the running systems
   int K = rounds_per_replay(L.work);
   if ((rc = capture(rb, K))) return rc;
 
@@ -440,17 +446,11 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
             act.push_back(L.groups[L.sys[k].gbase + g]);
         }
         if (n_live == 0) break;
-        // once the running systems' tiles no longer give every SIMD a wave,
-        // split their 128-target tiles (same results, twice the waves)
-        std::vector<EnsTile> tiles = make_tiles(L, kEnsTileConfigs[run_cfg], &live);
-        if (kEnsTileConfigs[run_cfg].tpl2_min_n != kEnsNoTpl2) {
-          long real = 0;
-          for (const EnsTile& t : tiles) real += t.sys >= 0 ? 1 : 0;
-          if (real < kEnsFillWaves) {
-            run_cfg = without_tpl2(run_cfg);
-            tiles = make_tiles(L, kEnsTileConfigs[run_cfg], &live);
-          }
-        }
+        // once the running systems' tiles no longer give every SIMD four
+        // waves: 64-target tiles, then partner chunks shared by several
+        // waves per tile (fast, mixed); same results
+        run_cfg = tail_config(L, &live, run_cfg, reorderable);
+        const std::vector<EnsTile> tiles = make_tiles(L, kEnsTileConfigs[run_cfg], &live);
         if (const char* why = check_tiles(tiles, L, tcap))
           return fail(ctx, ODEHIP_ERR_INVALID, why);
         HIP_TRY(ctx, hipMemcpy(w->dev + p.groups_act, act.data(), act.size() * sizeof(int2),

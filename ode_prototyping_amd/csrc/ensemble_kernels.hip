@@ -188,28 +188,22 @@ __device__ __forceinline__ void exact_tile(const double* __restrict__ y,
   tile_velocities<TPL>(y, k, n, i0);
 }
 
+// Partner chunks of the reorderable variants (fast, mixed): each target's
+// sum is the sum, in chunk order, of the sums over partners
+// [c kEnsChunk, (c + 1) kEnsChunk) taken in increasing j.  The order does not
+// depend on how many waves share a tile, so the split kernel below (chunks
+// of one tile on several waves, folded in order) gives the same bits as one
+// wave walking every chunk.
+
+// fast: partners j0 .. j1 - 1 summed into p* (which the caller zeroes).
 template <bool kMass, bool kGuard, int TPL>
-__device__ inline void fast_tile(const double* __restrict__ y,
-                                 const double* __restrict__ mass,
-                                 double* __restrict__ k, int n, int i0,
-                                 double eps2) {
+__device__ inline void fast_chunk(uniform_ptr yu, uniform_ptr mu, int j0, int j1,
+                                  const int* ic, const double* xi, const double* yi,
+                                  const double* zi, double eps2, double* px, double* py,
+                                  double* pz) {
   constexpr int U = TPL == 1 ? 8 : 4;
-  const int lane = threadIdx.x & 63;
-  const uniform_ptr yu = as_uniform(y);
-  const uniform_ptr mu = as_uniform(mass);
-  int ic[TPL];
-  double xi[TPL], yi[TPL], zi[TPL], ax[TPL], ay[TPL], az[TPL];
-#pragma unroll
-  for (int t = 0; t < TPL; ++t) {
-    const int i = i0 + 64 * t + lane;
-    ic[t] = i < n ? i : n - 1;
-    xi[t] = y[3 * ic[t]];
-    yi[t] = y[3 * ic[t] + 1];
-    zi[t] = y[3 * ic[t] + 2];
-    ax[t] = ay[t] = az[t] = 0.0;
-  }
-  int j = 0;
-  for (; j + U <= n; j += U) {
+  int j = j0;
+  for (; j + U <= j1; j += U) {
     const uniform_ptr pp = yu + 3 * j;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -217,16 +211,58 @@ __device__ inline void fast_tile(const double* __restrict__ y,
 #pragma unroll
       for (int t = 0; t < TPL; ++t)
         fast_term<kMass, kGuard>(pp[3 * u], pp[3 * u + 1], pp[3 * u + 2], mj, j + u,
-                                 ic[t], xi[t], yi[t], zi[t], eps2, ax[t], ay[t],
-                                 az[t]);
+                                 ic[t], xi[t], yi[t], zi[t], eps2, px[t], py[t], pz[t]);
     }
   }
-  for (; j < n; ++j) {
+  for (; j < j1; ++j) {
     const double mj = kMass ? mu[j] : 1.0;
 #pragma unroll
     for (int t = 0; t < TPL; ++t)
       fast_term<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j, ic[t],
-                               xi[t], yi[t], zi[t], eps2, ax[t], ay[t], az[t]);
+                               xi[t], yi[t], zi[t], eps2, px[t], py[t], pz[t]);
+  }
+}
+
+// The targets of a tile: coordinates of i0 + 64 t + lane (lanes past n
+// clamped to n - 1, their results not stored).
+template <int TPL>
+__device__ inline void tile_targets(const double* __restrict__ y, int n, int i0, int* ic,
+                                    double* xi, double* yi, double* zi) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int t = 0; t < TPL; ++t) {
+    const int i = i0 + 64 * t + lane;
+    ic[t] = i < n ? i : n - 1;
+    xi[t] = y[3 * ic[t]];
+    yi[t] = y[3 * ic[t] + 1];
+    zi[t] = y[3 * ic[t] + 2];
+  }
+}
+
+template <bool kMass, bool kGuard, int TPL>
+__device__ inline void fast_tile(const double* __restrict__ y,
+                                 const double* __restrict__ mass,
+                                 double* __restrict__ k, int n, int i0,
+                                 double eps2) {
+  const uniform_ptr yu = as_uniform(y);
+  const uniform_ptr mu = as_uniform(mass);
+  int ic[TPL];
+  double xi[TPL], yi[TPL], zi[TPL], ax[TPL], ay[TPL], az[TPL];
+  tile_targets<TPL>(y, n, i0, ic, xi, yi, zi);
+#pragma unroll
+  for (int t = 0; t < TPL; ++t) ax[t] = ay[t] = az[t] = 0.0;
+  for (int c0 = 0; c0 < n; c0 += kEnsChunk) {
+    double px[TPL], py[TPL], pz[TPL];
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) px[t] = py[t] = pz[t] = 0.0;
+    fast_chunk<kMass, kGuard, TPL>(yu, mu, c0, min(n, c0 + kEnsChunk), ic, xi, yi, zi, eps2,
+                                   px, py, pz);
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+      ax[t] += px[t];
+      ay[t] += py[t];
+      az[t] += pz[t];
+    }
   }
   tile_store<TPL>(k, n, i0, ax, ay, az);
   tile_velocities<TPL>(y, k, n, i0);
@@ -236,31 +272,20 @@ __device__ inline void fast_tile(const double* __restrict__ y,
 // in FP32 flushed into FP64 every 32 partners.  kNear: near pairs (FP32
 // r^2 < near2) are left out of the FP32 sums and their FP64 term is added
 // into the FP64 sums; without it every pair is FP32.
+// Partners j0 .. j1 - 1 (j0 a multiple of kFlush) summed into the FP64
+// sums a* (which the caller zeroes).
 template <bool kMass, bool kGuard, bool kNear, int TPL>
-__device__ inline void mixed_tile(const double* __restrict__ y,
-                                  const double* __restrict__ mass,
-                                  double* __restrict__ k, int n, int i0,
-                                  double eps2, float near2) {
+__device__ inline void mixed_chunk(uniform_ptr yu, uniform_ptr mu, int j0, int j1,
+                                   const int* ic, const double* xi, const double* yi,
+                                   const double* zi, double eps2, float near2, double* ax,
+                                   double* ay, double* az) {
   constexpr int kFlush = 32;
   constexpr int U = 8;
-  const int lane = threadIdx.x & 63;
-  const uniform_ptr yu = as_uniform(y);
-  const uniform_ptr mu = as_uniform(mass);
+  static_assert(kEnsChunk % kFlush == 0, "chunks start on a flush boundary");
   const float eps2f = static_cast<float>(eps2);
-  int ic[TPL];
-  double xi[TPL], yi[TPL], zi[TPL], ax[TPL], ay[TPL], az[TPL];
-#pragma unroll
-  for (int t = 0; t < TPL; ++t) {
-    const int i = i0 + 64 * t + lane;
-    ic[t] = i < n ? i : n - 1;
-    xi[t] = y[3 * ic[t]];
-    yi[t] = y[3 * ic[t] + 1];
-    zi[t] = y[3 * ic[t] + 2];
-    ax[t] = ay[t] = az[t] = 0.0;
-  }
-  int j = 0;
-  while (j < n) {
-    const int jstop = min(n, j + kFlush);
+  int j = j0;
+  while (j < j1) {
+    const int jstop = min(j1, j + kFlush);
     float fx[TPL], fy[TPL], fz[TPL];
 #pragma unroll
     for (int t = 0; t < TPL; ++t) fx[t] = fy[t] = fz[t] = 0.0f;
@@ -347,6 +372,33 @@ __device__ inline void mixed_tile(const double* __restrict__ y,
       az[t] += static_cast<double>(fz[t]);
     }
   }
+}
+
+template <bool kMass, bool kGuard, bool kNear, int TPL>
+__device__ inline void mixed_tile(const double* __restrict__ y,
+                                  const double* __restrict__ mass,
+                                  double* __restrict__ k, int n, int i0,
+                                  double eps2, float near2) {
+  const uniform_ptr yu = as_uniform(y);
+  const uniform_ptr mu = as_uniform(mass);
+  int ic[TPL];
+  double xi[TPL], yi[TPL], zi[TPL], ax[TPL], ay[TPL], az[TPL];
+  tile_targets<TPL>(y, n, i0, ic, xi, yi, zi);
+#pragma unroll
+  for (int t = 0; t < TPL; ++t) ax[t] = ay[t] = az[t] = 0.0;
+  for (int c0 = 0; c0 < n; c0 += kEnsChunk) {
+    double px[TPL], py[TPL], pz[TPL];
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) px[t] = py[t] = pz[t] = 0.0;
+    mixed_chunk<kMass, kGuard, kNear, TPL>(yu, mu, c0, min(n, c0 + kEnsChunk), ic, xi, yi,
+                                           zi, eps2, near2, px, py, pz);
+#pragma unroll
+    for (int t = 0; t < TPL; ++t) {
+      ax[t] += px[t];
+      ay[t] += py[t];
+      az[t] += pz[t];
+    }
+  }
   tile_store<TPL>(k, n, i0, ax, ay, az);
   tile_velocities<TPL>(y, k, n, i0);
 }
@@ -406,6 +458,88 @@ __global__ __launch_bounds__(64 * WPB) void k_ens_rhs(EnsBufs b,
     else
       mixed_tile<kMass, kSpecial, kNear, 1>(y, m, k, n, tile.i0, b.eps2, near2);
   }
+}
+
+// The reorderable variants with one tile per workgroup of C waves: wave w
+// sums chunks w, w + C, ... of the tile's partners, and after every C chunks
+// wave 0 adds their sums, in chunk order, from LDS -- the order fast_tile and
+// mixed_tile use, so the result is the same bits.  For the tail of a solve,
+// when few systems still run and one wave per tile would leave SIMDs idle.
+// Tiles hold 64 targets.
+template <int kVariant, bool kMass, bool kSpecial, bool kNear, int C>
+__global__ __launch_bounds__(64 * C) void k_ens_rhs_split(EnsBufs b,
+                                                          const double* y_in,
+                                                          double* k_out,
+                                                          int stage0) {
+  static_assert(kVariant != ODEHIP_NBODY_EXACT, "the exact order is sequential");
+  __shared__ double part[C][3][64];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  // every return up to the chunk loop is uniform over the workgroup
+  if (static_cast<int>(blockIdx.x) >= b.n_tiles) return;
+  const EnsTile tile = b.tiles[blockIdx.x];
+  if (tile.sys < 0 || tile.sys >= b.n_sys) return;
+  const SolverState* st = b.st ? b.st + tile.sys : nullptr;
+  if (st && (stage0 == 2 ? st->dense_go == 0 : st->done)) return;
+  const EnsSys sy = b.sys[tile.sys];
+  const int n = sy.n;
+  double* k = k_out + sy.eoff;
+  if (st && stage0 == 1 && st->k1_valid) {
+    if (st->k1_valid == 1 && wave == 0) {
+      const long lim = 3L * n;
+      const double* src = b.kfsal + sy.eoff;
+      for (int c = 0; c < 3; ++c) {
+        const long e = 3L * tile.i0 + 64 * c + lane;
+        if (e < lim) {
+          k[e] = src[e];
+          k[lim + e] = src[lim + e];
+        }
+      }
+    }
+    return;
+  }
+  const double* y = (y_in ? y_in : (st->cur ? b.xb1 : b.xb0)) + sy.eoff;
+  const double* m = kMass ? b.mass + sy.eoff / 6 : nullptr;
+  const uniform_ptr yu = as_uniform(y);
+  const uniform_ptr mu = as_uniform(m);
+  const float near2 = kNear ? mixed_near2(b.near_c, n) : 0.0f;
+  int ic[1];
+  double xi[1], yi[1], zi[1];
+  tile_targets<1>(y, n, tile.i0, ic, xi, yi, zi);
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  const int n_chunks = (n + kEnsChunk - 1) / kEnsChunk;
+  for (int r = 0; r < n_chunks; r += C) {
+    const int c = r + wave;
+    double px[1] = {0.0}, py[1] = {0.0}, pz[1] = {0.0};
+    if (c < n_chunks) {
+      const int j0 = c * kEnsChunk;
+      const int j1 = min(n, j0 + kEnsChunk);
+      if (kVariant == ODEHIP_NBODY_FAST)
+        fast_chunk<kMass, kSpecial, 1>(yu, mu, j0, j1, ic, xi, yi, zi, b.eps2, px, py, pz);
+      else
+        mixed_chunk<kMass, kSpecial, kNear, 1>(yu, mu, j0, j1, ic, xi, yi, zi, b.eps2, near2,
+                                               px, py, pz);
+    }
+    if (wave > 0) {
+      part[wave][0][lane] = px[0];
+      part[wave][1][lane] = py[0];
+      part[wave][2][lane] = pz[0];
+    }
+    __syncthreads();
+    if (wave == 0) {
+      ax += px[0];
+      ay += py[0];
+      az += pz[0];
+      for (int w = 1; w < C && r + w < n_chunks; ++w) {
+        ax += part[w][0][lane];
+        ay += part[w][1][lane];
+        az += part[w][2][lane];
+      }
+    }
+    __syncthreads();  // part is written again by the next C chunks
+  }
+  if (wave == 0) tile_store<1>(k, n, tile.i0, &ax, &ay, &az);
+  if (wave == C - 1) tile_velocities<1>(y, k, n, tile.i0);
 }
 
 // ------------------------------------------------------ element kernels --
@@ -620,6 +754,28 @@ __global__ __launch_bounds__(kBlock) void k_ens_copy_out(EnsBufs b,
 template <int kVariant, bool kMass, bool kSpecial, bool kNear>
 hipError_t launch_rhs_v(const EnsBufs& b, const double* y, double* k,
                         int stage0, hipStream_t s) {
+  if (b.split > 1) {
+    // one 64-target tile per workgroup of b.split waves
+    if constexpr (kVariant == ODEHIP_NBODY_EXACT) {
+      return hipErrorInvalidValue;
+    } else {
+      if (b.wpb != 1 || b.tpl2_min_n != kEnsNoTpl2) return hipErrorInvalidValue;
+      const unsigned blocks = static_cast<unsigned>(b.n_tiles);
+      if (blocks == 0) return hipSuccess;
+      if (b.split == 2)
+        hipLaunchKernelGGL((k_ens_rhs_split<kVariant, kMass, kSpecial, kNear, 2>),
+                           dim3(blocks), dim3(128), 0, s, b, y, k, stage0);
+      else if (b.split == 4)
+        hipLaunchKernelGGL((k_ens_rhs_split<kVariant, kMass, kSpecial, kNear, 4>),
+                           dim3(blocks), dim3(256), 0, s, b, y, k, stage0);
+      else if (b.split == 8)
+        hipLaunchKernelGGL((k_ens_rhs_split<kVariant, kMass, kSpecial, kNear, 8>),
+                           dim3(blocks), dim3(512), 0, s, b, y, k, stage0);
+      else
+        return hipErrorInvalidValue;
+      return hipGetLastError();
+    }
+  }
   const unsigned blocks = static_cast<unsigned>(b.n_tiles / b.wpb);
   if (blocks == 0) return hipSuccess;
   if (b.wpb == 4) {

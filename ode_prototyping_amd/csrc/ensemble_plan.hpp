@@ -135,24 +135,80 @@ inline const char* check_tiles(const std::vector<EnsTile>& tiles, const Layout& 
 }
 
 // Tiles that hold targets (empty padding tiles not counted): the waves a
-// full-ensemble RHS launch puts to work under `cfg`.
-inline long real_tiles(const Layout& L, const EnsTileConfig& cfg) {
+// right-hand-side launch under an unsplit `cfg` puts to work, over the
+// systems `running` marks (null: all).
+inline long real_tiles(const Layout& L, const EnsTileConfig& cfg,
+                       const std::vector<char>* running = nullptr) {
   long t = 0;
-  for (const auto& sy : L.sys) {
-    const int step = 64 * (sy.n >= cfg.tpl2_min_n ? 2 : 1);
-    t += (sy.n + step - 1) / step;
+  for (size_t s = 0; s < L.sys.size(); ++s) {
+    if (running && !(*running)[s]) continue;
+    const int n = L.sys[s].n;
+    const int step = 64 * (n >= cfg.tpl2_min_n ? 2 : 1);
+    t += (n + step - 1) / step;
   }
   return t;
 }
 
-// The same configuration with 64-target tiles for every system.
+// Waves with partner chunks to sum when `split` waves share each 64-target
+// tile of the running systems (a system of n particles has ceil(n / 128)
+// chunks, so waves past that count have nothing to do).
+inline long chunk_waves(const Layout& L, const std::vector<char>* running, int split) {
+  long w = 0;
+  for (size_t s = 0; s < L.sys.size(); ++s) {
+    if (running && !(*running)[s]) continue;
+    const long n = L.sys[s].n;
+    const long chunks = (n + kEnsChunk - 1) / kEnsChunk;
+    w += (n + 63) / 64 * std::min<long>(split, chunks);
+  }
+  return w;
+}
+
+// The configuration with the same waves per workgroup, XCD placement and
+// split, and 64-target tiles for every system.
 inline int without_tpl2(int cfg) {
-  for (int c = 0; c < ODEHIP_ENS_TILE_CONFIGS; ++c)
-    if (kEnsTileConfigs[c].wpb == kEnsTileConfigs[cfg].wpb &&
-        kEnsTileConfigs[c].xcd == kEnsTileConfigs[cfg].xcd &&
-        kEnsTileConfigs[c].tpl2_min_n == kEnsNoTpl2)
-      return c;
+  const EnsTileConfig& c = kEnsTileConfigs[cfg];
+  for (int k = 0; k < ODEHIP_ENS_TILE_CONFIGS; ++k)
+    if (kEnsTileConfigs[k].wpb == c.wpb && kEnsTileConfigs[k].xcd == c.xcd &&
+        kEnsTileConfigs[k].split == c.split && kEnsTileConfigs[k].tpl2_min_n == kEnsNoTpl2)
+      return k;
   return cfg;
+}
+
+// The one-wave-per-workgroup configuration of 64-target tiles with this XCD
+// placement and `split` waves per tile (1: unsplit).
+inline int split_config(int xcd, int split) {
+  for (int k = 0; k < ODEHIP_ENS_TILE_CONFIGS; ++k) {
+    const EnsTileConfig& c = kEnsTileConfigs[k];
+    if (c.wpb == 1 && c.tpl2_min_n == kEnsNoTpl2 && c.xcd == xcd && c.split == split)
+      return k;
+  }
+  return -1;
+}
+
+// The configuration for the right-hand sides of the running systems, from
+// `cfg`: kept while its working waves number kEnsFillWaves (four per SIMD);
+// below that, 128-target tiles become 64-target ones, and for the
+// reorderable variants a tile's partner chunks go to the fewest waves per
+// tile (2, 4 or 8) that reach kEnsFillWaves working waves, or that reach as
+// many as any split does.  Every choice gives the same results.
+inline int tail_config(const Layout& L, const std::vector<char>* running, int cfg,
+                       bool reorderable) {
+  const EnsTileConfig& c = kEnsTileConfigs[cfg];
+  const long waves = c.split > 1 ? chunk_waves(L, running, c.split)
+                                 : real_tiles(L, c, running);
+  if (waves >= kEnsFillWaves) return cfg;
+  const int unsplit = c.split > 1 ? split_config(c.xcd, 1) : without_tpl2(cfg);
+  if (!reorderable) return unsplit;
+  int best = 1;
+  long best_w = chunk_waves(L, running, 1);
+  for (int sp = 2; sp <= 8 && best_w < kEnsFillWaves; sp *= 2) {
+    const long w = chunk_waves(L, running, sp);
+    if (w > best_w) {
+      best = sp;
+      best_w = w;
+    }
+  }
+  return best == 1 ? unsplit : split_config(c.xcd, best);
 }
 
 // Device arena: tables, per-system state, vectors (byte offsets, 256-aligned).

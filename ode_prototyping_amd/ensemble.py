@@ -22,6 +22,7 @@ from ._lib import CONTROLLERS, NBODY_VARIANTS, STATUS, lib
 from .core import HipExecutor, HipState, tableau_id, tableau_shape
 
 ENS_TILE_CONFIGS = _lib.ENS_TILE_CONFIGS
+ENS_TILE_UNSPLIT = _lib.ENS_TILE_UNSPLIT
 ENS_TILE_AUTO = _lib.ENS_TILE_AUTO
 
 

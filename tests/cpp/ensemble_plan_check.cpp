@@ -63,12 +63,26 @@ int epc_check_tiles(int n, const int* sizes, const int* sys, const int* i0, long
 
 int epc_without_tpl2(int cfg) { return without_tpl2(cfg); }
 
-int epc_config(int cfg, int* wpb, int* tpl2_min_n, int* xcd) {
+int epc_config(int cfg, int* wpb, int* tpl2_min_n, int* xcd, int* split) {
   if (cfg < 0 || cfg >= ODEHIP_ENS_TILE_CONFIGS) return -1;
   *wpb = kEnsTileConfigs[cfg].wpb;
   *tpl2_min_n = kEnsTileConfigs[cfg].tpl2_min_n;
   *xcd = kEnsTileConfigs[cfg].xcd;
+  *split = kEnsTileConfigs[cfg].split;
   return 0;
+}
+
+int epc_n_unsplit() { return ODEHIP_ENS_TILE_UNSPLIT; }
+int epc_chunk() { return kEnsChunk; }
+long epc_fill_waves() { return kEnsFillWaves; }
+
+// tail_config over the systems `running` marks (null: all); -1: bad input.
+int epc_tail_config(int n, const int* sizes, const char* running, int cfg, int reorderable) {
+  Layout L;
+  if (!layout_of(n, sizes, &L) || cfg < 0 || cfg >= ODEHIP_ENS_TILE_CONFIGS) return -1;
+  std::vector<char> live;
+  if (running) live.assign(running, running + n);
+  return tail_config(L, running ? &live : nullptr, cfg, reorderable != 0);
 }
 
 // Arena plan: 18 byte offsets in Plan's field order (sys, groups, tiles,

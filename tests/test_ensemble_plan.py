@@ -43,7 +43,10 @@ def P():
                              ctypes.c_long, ctypes.POINTER(ctypes.c_ulong)]
     lib.epc_table_key_hit.argtypes = [ctypes.c_int, ip] + [ctypes.c_int] * 5
     lib.epc_table_key_hit_shifted.argtypes = [ctypes.c_int, ip, ctypes.c_int, ctypes.c_long]
-    lib.epc_config.argtypes = [ctypes.c_int, ip, ip, ip]
+    lib.epc_config.argtypes = [ctypes.c_int, ip, ip, ip, ip]
+    lib.epc_fill_waves.restype = ctypes.c_long
+    lib.epc_tail_config.argtypes = [ctypes.c_int, ip, ctypes.c_char_p, ctypes.c_int,
+                                    ctypes.c_int]
     return lib
 
 
@@ -56,9 +59,11 @@ def _sizes(a):
 
 
 def _config(P, c):
-    w, t, x = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-    assert P.epc_config(c, ctypes.byref(w), ctypes.byref(t), ctypes.byref(x)) == 0
-    return w.value, t.value, x.value
+    """(waves per workgroup, 128-target threshold, xcd placement, waves per tile)"""
+    w, t, x, sp = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    assert P.epc_config(c, ctypes.byref(w), ctypes.byref(t), ctypes.byref(x),
+                        ctypes.byref(sp)) == 0
+    return w.value, t.value, x.value, sp.value
 
 
 def _tiles(P, sizes, cfg, live=None):
@@ -73,7 +78,7 @@ def _tiles(P, sizes, cfg, live=None):
 
 def _check_coverage(P, sizes, cfg, live, sys, i0):
     """Every target of every running system in exactly one tile; nothing else."""
-    wpb, tpl2_min_n, _ = _config(P, cfg)
+    wpb, tpl2_min_n, _, _ = _config(P, cfg)
     assert len(sys) % wpb == 0
     real = sys >= 0
     run = np.ones(len(sizes), bool) if live is None else np.asarray(live, bool)
@@ -200,3 +205,68 @@ def test_table_cache_across_rhs_and_solve(P, name):
         # a key set with other table offsets never hits
         assert P.epc_table_key_hit_shifted(n, _ip(sizes), cfg, 256) == 0
         assert P.epc_table_key_hit_shifted(n, _ip(sizes), cfg, 0) == 1
+
+
+def _chunk_waves(sizes, live, split):
+    """Waves with partner chunks (128 partners each) when `split` waves share a
+    64-target tile of every running system."""
+    n = np.asarray(sizes, np.int64)[np.asarray(live, bool)]
+    return int(np.sum(-(-n // 64) * np.minimum(split, -(-n // 128))))
+
+
+@pytest.mark.parametrize("name", sorted(ENSEMBLES))
+def test_tail_config_rule(P, name):
+    """The configuration compaction (and, for small ensembles, the start of a solve)
+    moves the running systems to.  Kept while it gives kEnsFillWaves working waves;
+    below that: 64-target tiles, same XCD placement; the exact variant never gets
+    split tiles; the reorderable ones get the fewest waves per tile that reach the
+    fill, each doubling adding working waves, or no split where none would."""
+    sizes = _sizes(ENSEMBLES[name]())
+    n = len(sizes)
+    fill = P.epc_fill_waves()
+    assert fill == 4096 and P.epc_chunk() == 128
+    rng = np.random.default_rng(99)
+    for live in _subsets(rng, sizes, 10 if n > 1000 else 40):
+        mask = bytes(live.astype(np.uint8))
+        cw = {s: _chunk_waves(sizes, live, s) for s in (1, 2, 4, 8)}
+        for cfg in range(P.epc_n_configs()):
+            wpb, tpl2, xcd, split = _config(P, cfg)
+            for reorderable in (0, 1):
+                if split > 1 and not reorderable:
+                    continue  # not accepted for the exact variant
+                got = P.epc_tail_config(n, _ip(sizes), mask, cfg, reorderable)
+                gw, gt, gx, gs = _config(P, got)
+                step = np.where(sizes[live] >= tpl2, 128, 64)
+                waves = cw[split] if split > 1 else int(np.sum(-(-sizes[live] // step)))
+                if waves >= fill:
+                    assert got == cfg
+                    continue
+                assert gt == 1 << 30 and gx == xcd
+                if not reorderable:
+                    assert (gw, gs) == (wpb, 1)
+                    continue
+                if gs == 1:
+                    assert cw[2] == cw[1] or cw[1] >= fill
+                    assert gw == (wpb if split == 1 else 1)
+                    continue
+                assert gw == 1 and gs in (2, 4, 8)
+                assert cw[gs] > cw[gs // 2]      # the last doubling added working waves
+                assert cw[gs // 2] < fill        # and was needed
+                assert cw[gs] >= fill or gs == 8 or cw[2 * gs] == cw[gs]
+
+
+def test_tail_config_cases(P):
+    # the slowest systems of the 2048-system Plummer ensemble's tail: eight waves a tile
+    tail = _sizes([3393, 2817, 1908, 1279, 1010, 895, 868, 649, 538, 460, 375, 358, 318])
+    got = P.epc_tail_config(len(tail), _ip(tail), None, 1, 1)
+    assert _config(P, got) == (1, 1 << 30, 1, 8)
+    assert _config(P, P.epc_tail_config(len(tail), _ip(tail), None, 1, 0)) == (4, 1 << 30, 1, 1)
+    # one chunk per system: nothing to share
+    small = _sizes([64] * 1000)
+    assert P.epc_tail_config(len(small), _ip(small), None, 3, 1) == 1
+    # 300 bodies: three chunks, so four waves a tile (the fourth adds nothing past three)
+    one = _sizes([300])
+    assert _config(P, P.epc_tail_config(1, _ip(one), None, 6, 1)) == (1, 1 << 30, 0, 4)
+    # enough tiles: unchanged
+    big = _sizes([4096] * 64)
+    assert P.epc_tail_config(len(big), _ip(big), None, 1, 1) == 1

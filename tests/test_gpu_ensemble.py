@@ -220,6 +220,57 @@ def test_ensemble_rhs_mixed_all_near_is_fp64(exe, ens, cfg):
         assert err.max() < 2e-14, (n, err.max())
 
 
+SPLIT_SIZES = [1, 3, 64, 127, 128, 129, 255, 256, 257, 300, 1023, 1024, 1025, 2000]
+
+
+@pytest.mark.parametrize("variant", ["fast", "mixed", "mixed_near"])
+def test_ensemble_rhs_split_tiles_bitwise_unsplit(exe, variant):
+    """The fast and mixed sums run over partner chunks of 128 in chunk order, so one
+    tile's chunks shared by 2, 4 or 8 waves of a workgroup (configurations 12-17, XCD
+    placement off and on) give the same bits as one wave per tile, with 64- or
+    128-target tiles (configurations 0-11).  Systems of 1 to 16 chunks, with masses and
+    unit masses, and (mixed_near) a near field covering every pair."""
+    import ode_prototyping_amd as oh
+    from ode_prototyping_amd.ensemble import ENS_TILE_CONFIGS, ENS_TILE_UNSPLIT, ensemble_rhs, pack
+    xs, ms = make_ensemble(SPLIT_SIZES, seed0=90)
+    var = "fast" if variant == "fast" else "mixed"
+    if variant == "mixed_near":
+        exe.near_field = 1e30
+    try:
+        for masses in (pack(ms), None):
+            base, _ = ensemble_rhs(exe, SPLIT_SIZES, pack(xs), 1e-3, masses, var, tile_config=6)
+            assert np.all(np.isfinite(base))
+            for cfg in list(range(ENS_TILE_UNSPLIT)) + list(range(ENS_TILE_UNSPLIT,
+                                                                  ENS_TILE_CONFIGS)):
+                d, _ = ensemble_rhs(exe, SPLIT_SIZES, pack(xs), 1e-3, masses, var,
+                                    tile_config=cfg)
+                assert np.array_equal(bits(d), bits(base)), (variant, cfg, masses is None)
+    finally:
+        exe.near_field = oh.MIXED_NEAR_DEFAULT
+
+
+@pytest.mark.parametrize("variant", ["fast", "mixed"])
+def test_ensemble_solve_independent_of_split(exe, variant):
+    """128 systems of 2048 bodies (4096 tiles: one wave per tile at the start; as
+    systems finish, compaction moves the rest to tiles shared by several waves) give
+    every system the same bits as its solve alone (eight waves a tile from the start):
+    the split changes no result, whatever the other systems do."""
+    from ode_prototyping_amd.ensemble import pack, solve_ensemble
+    sizes = [2048] * 128
+    xs, ms = make_ensemble(sizes, seed0=400)
+    res = solve_ensemble("DOPRI5", sizes, pack(xs), 0.0, 2e-3, 1e-10, 1e-10, 1e-3, pack(ms),
+                         variant=variant, exe=exe, trace_cap=512)
+    assert len(set(res.attempts.tolist())) > 1  # systems finish in different rounds
+    finals = res.states()
+    for k in (0, 1, 37, 64, 100, 127):
+        one = solve_ensemble("DOPRI5", [2048], xs[k], 0.0, 2e-3, 1e-10, 1e-10, 1e-3, ms[k],
+                             variant=variant, exe=exe, trace_cap=512)
+        assert (one.accepted[0], one.rejected[0], one.rhs_evals[0]) == (
+            res.accepted[k], res.rejected[k], res.rhs_evals[k]), k
+        assert np.array_equal(bits(one.states()[0]), bits(finals[k])), k
+        assert np.array_equal(bits(one.trace[0]), bits(res.trace[k])), k
+
+
 @pytest.mark.parametrize("cfg", [0, 1, 7])
 def test_ensemble_rhs_after_solve_and_solve_after_rhs(exe, ens, cfg):
     """A solve that never compacts (two rounds) and then a right-hand side of the same
@@ -349,18 +400,20 @@ def test_ensemble_edge_cases(exe):
 
 
 def test_ensemble_autotune_reports_all_candidates(exe):
-    from ode_prototyping_amd.ensemble import pack, solve_ensemble
+    from ode_prototyping_amd.ensemble import ENS_TILE_UNSPLIT, pack, solve_ensemble
     sizes = [300, 20, 1000, 64, 513]
     xs, ms = make_ensemble(sizes, seed0=11)
     res = solve_ensemble("DOPRI5", sizes, pack(xs), 0.0, 1e-3, 1e-10, 1e-10, 1e-3, pack(ms),
                          variant="fast", exe=exe)
-    assert np.all(res.tune_ms > 0.0)
+    assert np.all(res.tune_ms[:ENS_TILE_UNSPLIT] > 0.0)
+    assert np.all(res.tune_ms[ENS_TILE_UNSPLIT:] == 0.0)  # split ones are not candidates
     # the choice: within 3 % of the fastest candidate, the most target tiles
     # (128-target tiles for systems of at least 1024 / 256 bodies halve them),
     # then the fastest of those
     tpl2 = [None, None, 1024, 1024, 256, 256] * 2   # the candidates' order
     tiles = [sum(-(-n // (128 if t and n >= t else 64)) for n in sizes) for t in tpl2]
-    near = [c for c in range(len(tpl2)) if res.tune_ms[c] <= 1.03 * res.tune_ms.min()]
+    tuned = res.tune_ms[:ENS_TILE_UNSPLIT]
+    near = [c for c in range(len(tpl2)) if tuned[c] <= 1.03 * tuned.min()]
     most = max(tiles[c] for c in near)
     pick = min((c for c in near if tiles[c] == most), key=lambda c: (res.tune_ms[c], c))
     assert res.tile_config == pick
@@ -373,7 +426,9 @@ def test_ensemble_rejects_bad_arguments(exe):
     with pytest.raises(OdeHipError):
         ensemble_rhs(exe, [4, 0], np.zeros(24), 1e-3)
     with pytest.raises(OdeHipError):
-        ensemble_rhs(exe, [4], np.zeros(24), 1e-3, tile_config=12)
+        ensemble_rhs(exe, [4], np.zeros(24), 1e-3, tile_config=18)
+    with pytest.raises(OdeHipError):  # split tiles reorder the sums: not for exact
+        ensemble_rhs(exe, [4], np.zeros(24), 1e-3, "exact", tile_config=12)
     with pytest.raises(ValueError):
         ensemble_rhs(exe, [4], np.zeros(23), 1e-3)
     with pytest.raises(OdeHipError):

@@ -435,11 +435,13 @@ int odehip_mlp_infer(odehip_context* ctx, const odehip_mlp* mlp,
  *
  * The fast and mixed variants sum each target's partners in chunks of 128
  * (increasing j within a chunk, chunk sums added in chunk order).
- * Configurations ODEHIP_ENS_TILE_UNSPLIT .. ODEHIP_ENS_TILE_CONFIGS-1 (fast
- * and mixed only; not autotuned) run the chunks of one 64-target tile on 2, 4
- * or 8 waves of one workgroup, same bits; a solve of these variants moves to
- * one of them whenever its running systems' tiles would leave SIMDs idle
- * (at the start for small ensembles, and as systems finish). */
+ * Configurations ODEHIP_ENS_TILE_UNSPLIT .. ODEHIP_ENS_TILE_CONFIGS-1 (not
+ * autotuned) share one 64-target tile among 2, 4 or 8 waves of a workgroup,
+ * with the same bits: fast and mixed fold the waves' chunk sums in chunk
+ * order; exact computes the terms of blocks of partners on all waves and adds
+ * them in increasing j on three.  A solve moves to one of them whenever its
+ * running systems' tiles would leave SIMDs idle (at the start for small
+ * ensembles, and as systems finish). */
 #define ODEHIP_ENS_TILE_CONFIGS 18
 #define ODEHIP_ENS_TILE_UNSPLIT 12
 #define ODEHIP_ENS_TILE_AUTO (-1)

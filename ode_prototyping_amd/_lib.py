@@ -97,7 +97,7 @@ class LeapfrogStats(C.Structure):
 
 
 ENS_TILE_CONFIGS = 18
-ENS_TILE_UNSPLIT = 12  # configurations ENS_TILE_UNSPLIT.. split tiles (fast, mixed only)
+ENS_TILE_UNSPLIT = 12  # configurations ENS_TILE_UNSPLIT.. share a tile among 2, 4, 8 waves
 ENS_TILE_AUTO = -1
 
 _ip = C.POINTER(C.c_int)

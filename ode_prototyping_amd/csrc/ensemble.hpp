@@ -53,7 +53,7 @@ struct EnsBufs {
   int variant;
   int wpb;         // waves per RHS workgroup
   int tpl2_min_n;  // systems with n >= this use 128-target tiles
-  int split;       // > 1: waves sharing one tile's partner chunks (fast, mixed)
+  int split;       // > 1: waves sharing one tile's partners
   double eps2;
   double near_c;  // mixed variant: pairs with FP32 r^2 < near_c / n in FP64
   const double* dense_t;  // DOPRI5 dense output: n_dense times, or null
@@ -63,9 +63,9 @@ struct EnsBufs {
 
 // Tile configurations: waves per workgroup, the size from which systems use
 // 128-target tiles, whether tiles of one system stay on one XCD, and the
-// waves that share one tile's partner chunks (1: a wave per tile).  The
-// autotuner times the unsplit ones (the first ODEHIP_ENS_TILE_UNSPLIT); the
-// split ones serve the reorderable variants when few tiles remain.
+// waves that share one tile's partners (1: a wave per tile).  The autotuner
+// times the unsplit ones (the first ODEHIP_ENS_TILE_UNSPLIT); the split ones
+// serve solves whose running systems leave few tiles.
 struct EnsTileConfig {
   int wpb;
   int tpl2_min_n;

@@ -209,9 +209,6 @@ int check_common(odehip_context* ctx, int nbody_variant, double softening,
   if (!(softening >= 0.0)) return fail(ctx, ODEHIP_ERR_INVALID, "bad softening");
   if (tile_config < ODEHIP_ENS_TILE_AUTO || tile_config >= ODEHIP_ENS_TILE_CONFIGS)
     return fail(ctx, ODEHIP_ERR_INVALID, "bad tile_config");
-  if (tile_config >= ODEHIP_ENS_TILE_UNSPLIT && nbody_variant == ODEHIP_NBODY_EXACT)
-    return fail(ctx, ODEHIP_ERR_INVALID,
-                "split tile configurations reorder partner sums: fast and mixed only");
   return ODEHIP_OK;
 }
 
@@ -356,10 +353,9 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
     for (int c = 0; c < ODEHIP_ENS_TILE_CONFIGS; ++c)
       opt->tune_ms[c] = tuned ? w->tuned_ms[c] : 0.0;
   }
-  // the reorderable variants share a tile's partner chunks among several
-  // waves from the start when the ensemble's tiles would leave SIMDs idle
-  const bool reorderable = nbody_variant != ODEHIP_NBODY_EXACT;
-  int run_cfg = reorderable ? tail_config(L, nullptr, cfg, true) : cfg;
+  // a tile's partners are shared by several waves from the start when the
+  // ensemble's tiles would leave SIMDs idle
+  int run_cfg = tail_config(L, nullptr, cfg);
   int n_tiles = 0;
   if ((rc = upload_tables(ctx, w, p, L, tcap, run_cfg, &n_tiles, s))) return rc;
   set_tile_cfg(&b, run_cfg, n_tiles);
@@ -447,9 +443,9 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
         }
         if (n_live == 0) break;
         // once the running systems' tiles no longer give every SIMD four
-        // waves: 64-target tiles, then partner chunks shared by several
-        // waves per tile (fast, mixed); same results
-        run_cfg = tail_config(L, &live, run_cfg, reorderable);
+        // waves: 64-target tiles, then partners shared by several waves per
+        // tile; same results
+        run_cfg = tail_config(L, &live, run_cfg);
         const std::vector<EnsTile> tiles = make_tiles(L, kEnsTileConfigs[run_cfg], &live);
         if (const char* why = check_tiles(tiles, L, tcap))
           return fail(ctx, ODEHIP_ERR_INVALID, why);

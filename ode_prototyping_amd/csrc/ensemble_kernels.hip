@@ -460,6 +460,114 @@ __global__ __launch_bounds__(64 * WPB) void k_ens_rhs(EnsBufs b,
   }
 }
 
+// The exact variant with one 64-target tile per workgroup of C waves, for the
+// tail of a solve: the C waves compute the terms of a block of C kExactP
+// partners (kExactP consecutive ones each) into LDS, then waves 0, 1, 2 add
+// the block's x, y, z terms to their component's sum in increasing j,
+// skipping j == i -- the order of exact_tile, so the same bits.
+constexpr int kExactP = 4;
+
+template <bool kMass, bool kSharedRcp, int C>
+__global__ __launch_bounds__(64 * C) void k_ens_rhs_exact_split(EnsBufs b,
+                                                                const double* y_in,
+                                                                double* k_out,
+                                                                int stage0) {
+  constexpr int B = C * kExactP;
+  __shared__ double term[3][B][64];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  // every return up to the block loop is uniform over the workgroup
+  if (static_cast<int>(blockIdx.x) >= b.n_tiles) return;
+  const EnsTile tile = b.tiles[blockIdx.x];
+  if (tile.sys < 0 || tile.sys >= b.n_sys) return;
+  const SolverState* st = b.st ? b.st + tile.sys : nullptr;
+  if (st && (stage0 == 2 ? st->dense_go == 0 : st->done)) return;
+  const EnsSys sy = b.sys[tile.sys];
+  const int n = sy.n;
+  double* k = k_out + sy.eoff;
+  if (st && stage0 == 1 && st->k1_valid) {
+    if (st->k1_valid == 1 && wave == 0) {
+      const long lim = 3L * n;
+      const double* src = b.kfsal + sy.eoff;
+      for (int c = 0; c < 3; ++c) {
+        const long e = 3L * tile.i0 + 64 * c + lane;
+        if (e < lim) {
+          k[e] = src[e];
+          k[lim + e] = src[lim + e];
+        }
+      }
+    }
+    return;
+  }
+  const double* y = (y_in ? y_in : (st->cur ? b.xb1 : b.xb0)) + sy.eoff;
+  const double* m = kMass ? b.mass + sy.eoff / 6 : nullptr;
+  const uniform_ptr yu = as_uniform(y);
+  const uniform_ptr mu = as_uniform(m);
+  int ic[1];
+  double xi[1], yi[1], zi[1];
+  tile_targets<1>(y, n, tile.i0, ic, xi, yi, zi);
+  // the shared-reciprocal form when every position coordinate of the system
+  // passes coord_shared_ok (the same test, on the same data, in every wave)
+  bool shared_ok = kSharedRcp;
+  if (kSharedRcp) {
+    for (int e = lane; e < 3 * n; e += 64) shared_ok = shared_ok & coord_shared_ok(y[e]);
+    shared_ok = __all(shared_ok);
+  }
+  const int i0 = tile.i0;
+  const int s1 = min(n, i0 + 64);
+  // acc0: component `wave` (waves 0..2); acc1: component 2 on wave 0 when C == 2
+  double acc0 = 0.0, acc1 = 0.0;
+  auto fold = [&](int comp, int jb, int cnt, bool self, double& acc) {
+    if (self) {
+      for (int q = 0; q < cnt; ++q) {
+        const double t = term[comp][q][lane];
+        acc = (jb + q == ic[0]) ? acc : acc + t;
+      }
+    } else if (cnt == B) {
+#pragma unroll
+      for (int q = 0; q < B; ++q) acc += term[comp][q][lane];
+    } else {
+      for (int q = 0; q < cnt; ++q) acc += term[comp][q][lane];
+    }
+  };
+  for (int jb = 0; jb < n; jb += B) {
+#pragma unroll
+    for (int p = 0; p < kExactP; ++p) {
+      const int j = jb + wave * kExactP + p;
+      double tx = 0.0, ty = 0.0, tz = 0.0;
+      if (j < n) {
+        const double dx = yu[3 * j] - xi[0];
+        const double dy = yu[3 * j + 1] - yi[0];
+        const double dz = yu[3 * j + 2] - zi[0];
+        const double mj = kMass ? mu[j] : 1.0;
+        if (kSharedRcp && shared_ok)
+          exact_term_shared_rcp<kMass>(dx, dy, dz, mj, b.eps2, tx, ty, tz);
+        else
+          exact_term_ieee<kMass>(dx, dy, dz, mj, b.eps2, tx, ty, tz);
+      }
+      term[0][wave * kExactP + p][lane] = tx;
+      term[1][wave * kExactP + p][lane] = ty;
+      term[2][wave * kExactP + p][lane] = tz;
+    }
+    __syncthreads();
+    const int cnt = min(B, n - jb);
+    const bool self = jb < s1 && jb + B > i0;  // the block holds targets of this tile
+    if (wave < 3) fold(wave, jb, cnt, self, acc0);
+    if constexpr (C == 2) {
+      if (wave == 0) fold(2, jb, cnt, self, acc1);
+    }
+    __syncthreads();  // term is written again by the next block
+  }
+  const int i = i0 + lane;
+  if (i < n) {
+    if (wave < 3) k[3L * n + 3 * i + wave] = acc0;
+    if constexpr (C == 2) {
+      if (wave == 0) k[3L * n + 3 * i + 2] = acc1;
+    }
+  }
+  if (wave == C - 1) tile_velocities<1>(y, k, n, i0);
+}
+
 // The reorderable variants with one tile per workgroup of C waves: wave w
 // sums chunks w, w + C, ... of the tile's partners, and after every C chunks
 // wave 0 adds their sums, in chunk order, from LDS -- the order fast_tile and
@@ -756,12 +864,23 @@ hipError_t launch_rhs_v(const EnsBufs& b, const double* y, double* k,
                         int stage0, hipStream_t s) {
   if (b.split > 1) {
     // one 64-target tile per workgroup of b.split waves
+    if (b.wpb != 1 || b.tpl2_min_n != kEnsNoTpl2) return hipErrorInvalidValue;
+    const unsigned blocks = static_cast<unsigned>(b.n_tiles);
+    if (blocks == 0) return hipSuccess;
     if constexpr (kVariant == ODEHIP_NBODY_EXACT) {
-      return hipErrorInvalidValue;
+      if (b.split == 2)
+        hipLaunchKernelGGL((k_ens_rhs_exact_split<kMass, kSpecial, 2>), dim3(blocks),
+                           dim3(128), 0, s, b, y, k, stage0);
+      else if (b.split == 4)
+        hipLaunchKernelGGL((k_ens_rhs_exact_split<kMass, kSpecial, 4>), dim3(blocks),
+                           dim3(256), 0, s, b, y, k, stage0);
+      else if (b.split == 8)
+        hipLaunchKernelGGL((k_ens_rhs_exact_split<kMass, kSpecial, 8>), dim3(blocks),
+                           dim3(512), 0, s, b, y, k, stage0);
+      else
+        return hipErrorInvalidValue;
+      return hipGetLastError();
     } else {
-      if (b.wpb != 1 || b.tpl2_min_n != kEnsNoTpl2) return hipErrorInvalidValue;
-      const unsigned blocks = static_cast<unsigned>(b.n_tiles);
-      if (blocks == 0) return hipSuccess;
       if (b.split == 2)
         hipLaunchKernelGGL((k_ens_rhs_split<kVariant, kMass, kSpecial, kNear, 2>),
                            dim3(blocks), dim3(128), 0, s, b, y, k, stage0);

@@ -187,18 +187,17 @@ inline int split_config(int xcd, int split) {
 
 // The configuration for the right-hand sides of the running systems, from
 // `cfg`: kept while its working waves number kEnsFillWaves (four per SIMD);
-// below that, 128-target tiles become 64-target ones, and for the
-// reorderable variants a tile's partner chunks go to the fewest waves per
-// tile (2, 4 or 8) that reach kEnsFillWaves working waves, or that reach as
-// many as any split does.  Every choice gives the same results.
-inline int tail_config(const Layout& L, const std::vector<char>* running, int cfg,
-                       bool reorderable) {
+// below that, 128-target tiles become 64-target ones, and a tile's partners
+// go to the fewest waves per tile (2, 4 or 8) that reach kEnsFillWaves
+// working waves, or that reach as many as any split does (working waves
+// counted in chunks of kEnsChunk partners).  Every choice gives the same
+// results.
+inline int tail_config(const Layout& L, const std::vector<char>* running, int cfg) {
   const EnsTileConfig& c = kEnsTileConfigs[cfg];
   const long waves = c.split > 1 ? chunk_waves(L, running, c.split)
                                  : real_tiles(L, c, running);
   if (waves >= kEnsFillWaves) return cfg;
   const int unsplit = c.split > 1 ? split_config(c.xcd, 1) : without_tpl2(cfg);
-  if (!reorderable) return unsplit;
   int best = 1;
   long best_w = chunk_waves(L, running, 1);
   for (int sp = 2; sp <= 8 && best_w < kEnsFillWaves; sp *= 2) {

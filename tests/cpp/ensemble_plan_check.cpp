@@ -77,12 +77,12 @@ int epc_chunk() { return kEnsChunk; }
 long epc_fill_waves() { return kEnsFillWaves; }
 
 // tail_config over the systems `running` marks (null: all); -1: bad input.
-int epc_tail_config(int n, const int* sizes, const char* running, int cfg, int reorderable) {
+int epc_tail_config(int n, const int* sizes, const char* running, int cfg) {
   Layout L;
   if (!layout_of(n, sizes, &L) || cfg < 0 || cfg >= ODEHIP_ENS_TILE_CONFIGS) return -1;
   std::vector<char> live;
   if (running) live.assign(running, running + n);
-  return tail_config(L, running ? &live : nullptr, cfg, reorderable != 0);
+  return tail_config(L, running ? &live : nullptr, cfg);
 }
 
 // Arena plan: 18 byte offsets in Plan's field order (sys, groups, tiles,

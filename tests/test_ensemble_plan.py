@@ -45,8 +45,7 @@ def P():
     lib.epc_table_key_hit_shifted.argtypes = [ctypes.c_int, ip, ctypes.c_int, ctypes.c_long]
     lib.epc_config.argtypes = [ctypes.c_int, ip, ip, ip, ip]
     lib.epc_fill_waves.restype = ctypes.c_long
-    lib.epc_tail_config.argtypes = [ctypes.c_int, ip, ctypes.c_char_p, ctypes.c_int,
-                                    ctypes.c_int]
+    lib.epc_tail_config.argtypes = [ctypes.c_int, ip, ctypes.c_char_p, ctypes.c_int]
     return lib
 
 
@@ -218,9 +217,9 @@ def _chunk_waves(sizes, live, split):
 def test_tail_config_rule(P, name):
     """The configuration compaction (and, for small ensembles, the start of a solve)
     moves the running systems to.  Kept while it gives kEnsFillWaves working waves;
-    below that: 64-target tiles, same XCD placement; the exact variant never gets
-    split tiles; the reorderable ones get the fewest waves per tile that reach the
-    fill, each doubling adding working waves, or no split where none would."""
+    below that: 64-target tiles, same XCD placement, and the fewest waves per tile
+    that reach the fill, each doubling adding working waves, or no split where none
+    would."""
     sizes = _sizes(ENSEMBLES[name]())
     n = len(sizes)
     fill = P.epc_fill_waves()
@@ -231,42 +230,35 @@ def test_tail_config_rule(P, name):
         cw = {s: _chunk_waves(sizes, live, s) for s in (1, 2, 4, 8)}
         for cfg in range(P.epc_n_configs()):
             wpb, tpl2, xcd, split = _config(P, cfg)
-            for reorderable in (0, 1):
-                if split > 1 and not reorderable:
-                    continue  # not accepted for the exact variant
-                got = P.epc_tail_config(n, _ip(sizes), mask, cfg, reorderable)
-                gw, gt, gx, gs = _config(P, got)
-                step = np.where(sizes[live] >= tpl2, 128, 64)
-                waves = cw[split] if split > 1 else int(np.sum(-(-sizes[live] // step)))
-                if waves >= fill:
-                    assert got == cfg
-                    continue
-                assert gt == 1 << 30 and gx == xcd
-                if not reorderable:
-                    assert (gw, gs) == (wpb, 1)
-                    continue
-                if gs == 1:
-                    assert cw[2] == cw[1] or cw[1] >= fill
-                    assert gw == (wpb if split == 1 else 1)
-                    continue
-                assert gw == 1 and gs in (2, 4, 8)
-                assert cw[gs] > cw[gs // 2]      # the last doubling added working waves
-                assert cw[gs // 2] < fill        # and was needed
-                assert cw[gs] >= fill or gs == 8 or cw[2 * gs] == cw[gs]
+            got = P.epc_tail_config(n, _ip(sizes), mask, cfg)
+            gw, gt, gx, gs = _config(P, got)
+            step = np.where(sizes[live] >= tpl2, 128, 64)
+            waves = cw[split] if split > 1 else int(np.sum(-(-sizes[live] // step)))
+            if waves >= fill:
+                assert got == cfg
+                continue
+            assert gt == 1 << 30 and gx == xcd
+            if gs == 1:
+                assert cw[2] == cw[1] or cw[1] >= fill
+                assert gw == (wpb if split == 1 else 1)
+                continue
+            assert gw == 1 and gs in (2, 4, 8)
+            assert cw[gs] > cw[gs // 2]      # the last doubling added working waves
+            assert cw[gs // 2] < fill        # and was needed
+            assert cw[gs] >= fill or gs == 8 or cw[2 * gs] == cw[gs]
 
 
 def test_tail_config_cases(P):
     # the slowest systems of the 2048-system Plummer ensemble's tail: eight waves a tile
     tail = _sizes([3393, 2817, 1908, 1279, 1010, 895, 868, 649, 538, 460, 375, 358, 318])
-    got = P.epc_tail_config(len(tail), _ip(tail), None, 1, 1)
+    got = P.epc_tail_config(len(tail), _ip(tail), None, 1)
     assert _config(P, got) == (1, 1 << 30, 1, 8)
-    assert _config(P, P.epc_tail_config(len(tail), _ip(tail), None, 1, 0)) == (4, 1 << 30, 1, 1)
     # one chunk per system: nothing to share
     small = _sizes([64] * 1000)
-    assert P.epc_tail_config(len(small), _ip(small), None, 3, 1) == 1
+    assert P.epc_tail_config(len(small), _ip(small), None, 3) == 1
     # 300 bodies: three chunks, so four waves a tile (the fourth adds nothing past three)
     one = _sizes([300])
-    assert _config(P, P.epc_tail_config(1, _ip(one), None, 6, 1)) == (1, 1 << 30, 0, 4)
+    assert _config(P, P.epc_tail_config(1, _ip(one), None, 6)) == (1, 1 << 30, 0, 4)
     # enough tiles: unchanged
     big = _sizes([4096] * 64)
-    assert P.epc_tail_config(len(big), _ip(big), None, 1, 1) == 1
+    assert P.epc_tail_config(len(big), _ip(big), None, 1) == 1

@@ -38,7 +38,7 @@ def ens():
     return make_ensemble(SIZES)
 
 
-@pytest.mark.parametrize("cfg", list(range(12)))
+@pytest.mark.parametrize("cfg", list(range(18)))
 @pytest.mark.parametrize("with_mass", [True, False])
 def test_ensemble_rhs_exact_bitwise(exe, ens, cfg, with_mass):
     from ode_prototyping_amd.ensemble import ensemble_rhs, pack, unpack
@@ -51,11 +51,12 @@ def test_ensemble_rhs_exact_bitwise(exe, ens, cfg, with_mass):
     assert ms_ > 0.0
 
 
-@pytest.mark.parametrize("cfg", [-1, 1, 6, 11])
+@pytest.mark.parametrize("cfg", [-1, 1, 6, 11, 12, 15, 17])
 def test_ensemble_rhs_exact_bitwise_many_systems(exe, cfg):
     """600 systems of 16 to 300 bodies (about 1700 tiles): more workgroups
     than the 256 CUs hold at once, XCD placement on and off, 128-target tiles
-    for the systems of at least 256 bodies in configuration 11."""
+    for the systems of at least 256 bodies in configuration 11, a tile shared
+    by 2, 4 and 8 waves in configurations 12, 15 and 17."""
     from ode_prototyping_amd.ensemble import ensemble_rhs, pack, unpack
     rng = np.random.default_rng(600)
     sizes = [int(v) for v in rng.integers(16, 301, size=600)]
@@ -66,16 +67,17 @@ def test_ensemble_rhs_exact_bitwise_many_systems(exe, cfg):
         assert np.array_equal(bits(part), bits(ref)), (cfg, k, sizes[k])
 
 
-def test_ensemble_rhs_exact_zero_softening(exe):
+@pytest.mark.parametrize("cfg", [2, 12, 14, 16])
+def test_ensemble_rhs_exact_zero_softening(exe, cfg):
     """eps = 0: the IEEE term path (no shared reciprocal) and the skipped
-    diagonal (0/0 would be NaN)."""
+    diagonal (0/0 would be NaN), one wave per tile and shared tiles."""
     from ode_prototyping_amd.ensemble import ensemble_rhs, pack, unpack
     sizes = [7, 70, 130]
     xs, ms = make_ensemble(sizes, seed0=50)
-    d, _ = ensemble_rhs(exe, sizes, pack(xs), 0.0, pack(ms), "exact", tile_config=2)
+    d, _ = ensemble_rhs(exe, sizes, pack(xs), 0.0, pack(ms), "exact", tile_config=cfg)
     for k, part in enumerate(unpack(d, sizes)):
         ref = O.rhs(O.RHS_NBODY, xs[k], 0.0, ms[k])
-        assert np.array_equal(bits(part), bits(ref))
+        assert np.array_equal(bits(part), bits(ref)), (cfg, sizes[k])
 
 
 @pytest.mark.parametrize("variant,tol", [("fast", 2e-14), ("mixed", 5e-6)])
@@ -249,12 +251,13 @@ def test_ensemble_rhs_split_tiles_bitwise_unsplit(exe, variant):
         exe.near_field = oh.MIXED_NEAR_DEFAULT
 
 
-@pytest.mark.parametrize("variant", ["fast", "mixed"])
+@pytest.mark.parametrize("variant", ["exact", "fast", "mixed"])
 def test_ensemble_solve_independent_of_split(exe, variant):
     """128 systems of 2048 bodies (4096 tiles: one wave per tile at the start; as
     systems finish, compaction moves the rest to tiles shared by several waves) give
     every system the same bits as its solve alone (eight waves a tile from the start):
-    the split changes no result, whatever the other systems do."""
+    the split changes no result, whatever the other systems do.  (Exact: both also
+    equal the oracle, which the smaller ensemble tests check.)"""
     from ode_prototyping_amd.ensemble import pack, solve_ensemble
     sizes = [2048] * 128
     xs, ms = make_ensemble(sizes, seed0=400)
@@ -427,8 +430,6 @@ def test_ensemble_rejects_bad_arguments(exe):
         ensemble_rhs(exe, [4, 0], np.zeros(24), 1e-3)
     with pytest.raises(OdeHipError):
         ensemble_rhs(exe, [4], np.zeros(24), 1e-3, tile_config=18)
-    with pytest.raises(OdeHipError):  # split tiles reorder the sums: not for exact
-        ensemble_rhs(exe, [4], np.zeros(24), 1e-3, "exact", tile_config=12)
     with pytest.raises(ValueError):
         ensemble_rhs(exe, [4], np.zeros(23), 1e-3)
     with pytest.raises(OdeHipError):

@@ -577,6 +577,13 @@ def main():
                          "per variant (exact, fast, mixed, mixed with the near field; the last "
                          "5 timed), tile configuration 7, no graphs (the workload of the "
                          "committed ensemble RHS profile)")
+    ap.add_argument("--profile-ensemble-tail", action="store_true",
+                    help="only run the ensemble right-hand side of the 32 and of the 8 "
+                         "largest systems of the 2048-system input (a solve's tail: few "
+                         "running systems), exact and fast, one wave per tile "
+                         "(configuration 7) against the shared tiles the tail rule picks "
+                         "(4 and 8 waves per tile: configurations 15 and 17); 8 launches "
+                         "each, the last 5 timed, no graphs")
     ap.add_argument("--profile-rhs", action="store_true",
                     help="only run each RHS variant 3x, a 64 MiB calibration copy and "
                          "the near-field mixed RHS 3x (the workload of the committed "
@@ -635,6 +642,28 @@ def main():
                   for _ in range(8)][3:]  # three warm-up launches, then five
             log(f"profile-ensemble-rhs {var} near {near}: {min(ms):.3f} ms (HIP events)")
         exe.near_field = oh.MIXED_NEAR_DEFAULT
+        return
+
+    if args.profile_ensemble_tail:
+        from ode_prototyping_amd.ensemble import ensemble_rhs, pack, unpack
+        from ode_prototyping_amd import train_controller as trc
+        sizes, x0, m = trc.workload_ensemble(2048, 0)
+        order = np.argsort(-sizes, kind="stable")
+        xs, ms = unpack(x0, sizes), unpack(m, sizes, per_particle=1)
+        exe = oh.HipExecutor(0)
+        for count, split_cfg in ((32, 15), (8, 17)):
+            pick = order[:count]
+            sub = sizes[pick]
+            x0d = oh.HipState(exe, data=pack([xs[k] for k in pick]))
+            md = oh.HipState(exe, data=pack([ms[k] for k in pick]))
+            kd = oh.HipState(exe, x0d.size)
+            for var in ("exact", "fast"):
+                for cfg in (7, split_cfg):
+                    t = [ensemble_rhs(exe, sub, x0d, 1e-3, md, var, tile_config=cfg,
+                                      dxdt=kd)[1] for _ in range(8)][3:]
+                    log(f"profile-ensemble-tail {count} largest (sum n {int(sub.sum())}) "
+                        f"{var} config {cfg}: {min(t):.4f} ms min, "
+                        f"{sum(t) / len(t):.4f} ms mean (HIP events)")
         return
 
     if args.profile_ensemble:

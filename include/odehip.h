@@ -435,13 +435,14 @@ int odehip_mlp_infer(odehip_context* ctx, const odehip_mlp* mlp,
  *
  * The fast and mixed variants sum each target's partners in chunks of 128
  * (increasing j within a chunk, chunk sums added in chunk order).
- * Configurations ODEHIP_ENS_TILE_UNSPLIT .. ODEHIP_ENS_TILE_CONFIGS-1 (not
- * autotuned) share one 64-target tile among 2, 4 or 8 waves of a workgroup,
- * with the same bits: fast and mixed fold the waves' chunk sums in chunk
- * order; exact computes the terms of blocks of partners on all waves and adds
- * them in increasing j on three.  A solve moves to one of them whenever its
- * running systems' tiles would leave SIMDs idle (at the start for small
- * ensembles, and as systems finish). */
+ * Configurations ODEHIP_ENS_TILE_UNSPLIT .. ODEHIP_ENS_TILE_CONFIGS-1 share
+ * one 64-target tile among 2, 4 or 8 waves of a workgroup, with the same
+ * bits: fast and mixed fold the waves' chunk sums in chunk order; exact
+ * computes the terms of blocks of partners on all waves and adds them in
+ * increasing j on three.  The autotuner times all configurations; a solve
+ * also moves to a shared-tile one, at the start and as systems finish, when
+ * its running systems are few or their longest partner walks dominate (a
+ * decision table measured on this device). */
 #define ODEHIP_ENS_TILE_CONFIGS 18
 #define ODEHIP_ENS_TILE_UNSPLIT 12
 #define ODEHIP_ENS_TILE_AUTO (-1)

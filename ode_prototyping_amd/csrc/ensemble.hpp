@@ -63,9 +63,10 @@ struct EnsBufs {
 
 // Tile configurations: waves per workgroup, the size from which systems use
 // 128-target tiles, whether tiles of one system stay on one XCD, and the
-// waves that share one tile's partners (1: a wave per tile).  The autotuner
-// times the unsplit ones (the first ODEHIP_ENS_TILE_UNSPLIT); the split ones
-// serve solves whose running systems leave few tiles.
+// waves that share one tile's partners (1: a wave per tile; the last
+// ODEHIP_ENS_TILE_CONFIGS - ODEHIP_ENS_TILE_UNSPLIT configurations).  The
+// autotuner times every one; tail_config moves a solve's running systems to
+// a shared-tile one when they are few.
 struct EnsTileConfig {
   int wpb;
   int tpl2_min_n;

@@ -167,7 +167,7 @@ int autotune(odehip_context* ctx, EnsembleWorkspace* w, const Plan& p,
              double* k, hipStream_t s) {
   double best = 1e300;
   int best_cfg = kEnsDefaultTileConfig;
-  for (int c = 0; c < ODEHIP_ENS_TILE_UNSPLIT; ++c) {
+  for (int c = 0; c < ODEHIP_ENS_TILE_CONFIGS; ++c) {
     int n_tiles = 0;
     int rc = upload_tables(ctx, w, p, L, tiles_cap, c, &n_tiles, s);
     if (rc) return rc;
@@ -190,7 +190,7 @@ int autotune(odehip_context* ctx, EnsembleWorkspace* w, const Plan& p,
     }
   }
   long most = -1;
-  for (int c = 0; c < ODEHIP_ENS_TILE_UNSPLIT; ++c) {
+  for (int c = 0; c < ODEHIP_ENS_TILE_CONFIGS; ++c) {
     if (!(w->tuned_ms[c] <= 1.03 * best)) continue;
     const long t = real_tiles(L, kEnsTileConfigs[c]);
     if (t > most || (t == most && w->tuned_ms[c] < w->tuned_ms[best_cfg])) {
@@ -355,7 +355,8 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
   }
   // a tile's partners are shared by several waves from the start when the
   // ensemble's tiles would leave SIMDs idle
-  int run_cfg = tail_config(L, nullptr, cfg);
+  const bool exact = nbody_variant == ODEHIP_NBODY_EXACT;
+  int run_cfg = tail_config(L, nullptr, cfg, exact);
   int n_tiles = 0;
   if ((rc = upload_tables(ctx, w, p, L, tcap, run_cfg, &n_tiles, s))) return rc;
   set_tile_cfg(&b, run_cfg, n_tiles);
@@ -445,7 +446,7 @@ int odehip_ensemble_solve(odehip_context* ctx, int tableau, int n_systems,
         // once the running systems' tiles no longer give every SIMD four
         // waves: 64-target tiles, then partners shared by several waves per
         // tile; same results
-        run_cfg = tail_config(L, &live, run_cfg);
+        run_cfg = tail_config(L, &live, run_cfg, exact);
         const std::vector<EnsTile> tiles = make_tiles(L, kEnsTileConfigs[run_cfg], &live);
         if (const char* why = check_tiles(tiles, L, tcap))
           return fail(ctx, ODEHIP_ERR_INVALID, why);

@@ -185,29 +185,71 @@ inline int split_config(int xcd, int split) {
   return -1;
 }
 
-// The configuration for the right-hand sides of the running systems, from
-// `cfg`: kept while its working waves number kEnsFillWaves (four per SIMD);
-// below that, 128-target tiles become 64-target ones, and a tile's partners
-// go to the fewest waves per tile (2, 4 or 8) that reach kEnsFillWaves
-// working waves, or that reach as many as any split does (working waves
-// counted in chunks of kEnsChunk partners).  Every choice gives the same
-// results.
-inline int tail_config(const Layout& L, const std::vector<char>* running, int cfg) {
-  const EnsTileConfig& c = kEnsTileConfigs[cfg];
-  const long waves = c.split > 1 ? chunk_waves(L, running, c.split)
-                                 : real_tiles(L, c, running);
-  if (waves >= kEnsFillWaves) return cfg;
-  const int unsplit = c.split > 1 ? split_config(c.xcd, 1) : without_tpl2(cfg);
-  int best = 1;
-  long best_w = chunk_waves(L, running, 1);
-  for (int sp = 2; sp <= 8 && best_w < kEnsFillWaves; sp *= 2) {
-    const long w = chunk_waves(L, running, sp);
-    if (w > best_w) {
-      best = sp;
-      best_w = w;
-    }
+// SIMDs of the device (256 CUs x 4): the waves one launch needs to give
+// each SIMD one.
+constexpr long kEnsSimds = 1024;
+// Tiles from which the reorderable variants keep the tuned configuration.
+constexpr long kEnsSplitAmple = 8192;
+
+// Shape of the running systems' right-hand side: 64-target tiles, the
+// longest partner walk of a tile and the ratio of that walk to the partner
+// steps per SIMD of a perfectly balanced launch (above 1 the longest tiles,
+// not the total work, set the time of a launch with one wave per tile).
+struct TailShape {
+  long tiles = 0;
+  long longest = 0;
+  double ratio = 0.0;
+};
+
+inline TailShape tail_shape(const Layout& L, const std::vector<char>* running) {
+  TailShape t;
+  double work = 0.0;
+  for (size_t s = 0; s < L.sys.size(); ++s) {
+    if (running && !(*running)[s]) continue;
+    const long n = L.sys[s].n;
+    t.tiles += (n + 63) / 64;
+    t.longest = std::max(t.longest, n);
+    work += static_cast<double>((n + 63) / 64) * static_cast<double>(n);
   }
-  return best == 1 ? unsplit : split_config(c.xcd, best);
+  t.ratio = work > 0.0 ? static_cast<double>(t.longest) * kEnsSimds / work : 0.0;
+  return t;
+}
+
+// The configuration for the right-hand sides of the running systems, from
+// `cfg` (a decision table measured on MI355X over subsets of the 2048-system
+// ensemble; every choice gives the same results):
+//  * exact: a tile shared by 8 waves when the longest walks dominate (ratio
+//    above 1.2: the terms of a block of partners on all waves, the ordered
+//    fold on three, 1.3-1.7x faster there and 4-21 % slower on balanced
+//    launches); otherwise one wave per tile, with 64-target tiles once fewer
+//    than kEnsFillWaves remain;
+//  * fast, mixed: the configuration kept while kEnsSplitAmple tiles remain;
+//    below, a tile's chunks shared by 8 waves (ratio at least 1) or 4 (3-11 %
+//    faster even on balanced launches of 2000-7000 tiles, up to 2.3x on
+//    dominated ones);
+// in both cases never more waves per tile than add working waves
+// (chunk_waves).
+inline int tail_config(const Layout& L, const std::vector<char>* running, int cfg,
+                       bool exact) {
+  const EnsTileConfig& c = kEnsTileConfigs[cfg];
+  const TailShape t = tail_shape(L, running);
+  const int unsplit = c.split > 1 ? split_config(c.xcd, 1) : cfg;
+  auto useful = [&](int split) {
+    while (split > 1 && chunk_waves(L, running, split) <= chunk_waves(L, running, split / 2))
+      split /= 2;
+    return split;
+  };
+  if (exact) {
+    if (t.ratio > 1.2) {
+      if (useful(8) == 8) return split_config(c.xcd, 8);
+    }
+    return real_tiles(L, kEnsTileConfigs[unsplit], running) < kEnsFillWaves
+               ? without_tpl2(unsplit)
+               : unsplit;
+  }
+  if (t.tiles >= kEnsSplitAmple) return cfg;
+  const int split = useful(t.ratio >= 1.0 ? 8 : 4);
+  return split > 1 ? split_config(c.xcd, split) : without_tpl2(unsplit);
 }
 
 // Device arena: tables, per-system state, vectors (byte offsets, 256-aligned).

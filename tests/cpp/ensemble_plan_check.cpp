@@ -73,16 +73,17 @@ int epc_config(int cfg, int* wpb, int* tpl2_min_n, int* xcd, int* split) {
 }
 
 int epc_n_unsplit() { return ODEHIP_ENS_TILE_UNSPLIT; }
+int epc_split_config(int xcd, int split) { return split_config(xcd, split); }
 int epc_chunk() { return kEnsChunk; }
 long epc_fill_waves() { return kEnsFillWaves; }
 
 // tail_config over the systems `running` marks (null: all); -1: bad input.
-int epc_tail_config(int n, const int* sizes, const char* running, int cfg) {
+int epc_tail_config(int n, const int* sizes, const char* running, int cfg, int exact) {
   Layout L;
   if (!layout_of(n, sizes, &L) || cfg < 0 || cfg >= ODEHIP_ENS_TILE_CONFIGS) return -1;
   std::vector<char> live;
   if (running) live.assign(running, running + n);
-  return tail_config(L, running ? &live : nullptr, cfg);
+  return tail_config(L, running ? &live : nullptr, cfg, exact != 0);
 }
 
 // Arena plan: 18 byte offsets in Plan's field order (sys, groups, tiles,

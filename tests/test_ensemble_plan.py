@@ -45,7 +45,8 @@ def P():
     lib.epc_table_key_hit_shifted.argtypes = [ctypes.c_int, ip, ctypes.c_int, ctypes.c_long]
     lib.epc_config.argtypes = [ctypes.c_int, ip, ip, ip, ip]
     lib.epc_fill_waves.restype = ctypes.c_long
-    lib.epc_tail_config.argtypes = [ctypes.c_int, ip, ctypes.c_char_p, ctypes.c_int]
+    lib.epc_tail_config.argtypes = [ctypes.c_int, ip, ctypes.c_char_p, ctypes.c_int,
+                                    ctypes.c_int]
     return lib
 
 
@@ -213,52 +214,78 @@ def _chunk_waves(sizes, live, split):
     return int(np.sum(-(-n // 64) * np.minimum(split, -(-n // 128))))
 
 
+def _tail_model(P, sizes, live, cfg, exact):
+    """The decision table the library documents for tail_config, restated."""
+    n = np.asarray(sizes, np.int64)[np.asarray(live, bool)]
+    tiles = int(np.sum(-(-n // 64)))
+    ratio = float(n.max()) * 1024 / float(np.sum(-(-n // 64) * n))
+    wpb, tpl2, xcd, split = _config(P, cfg)
+    unsplit = cfg if split == 1 else P.epc_split_config(xcd, 1)
+
+    def useful(c):
+        while c > 1 and _chunk_waves(sizes, live, c) <= _chunk_waves(sizes, live, c // 2):
+            c //= 2
+        return c
+
+    if exact:
+        if ratio > 1.2 and useful(8) == 8:
+            return P.epc_split_config(xcd, 8)
+        _, t2, _, _ = _config(P, unsplit)
+        real = int(np.sum(-(-n // np.where(n >= t2, 128, 64))))
+        return P.epc_without_tpl2(unsplit) if real < 4096 else unsplit
+    if tiles >= 8192:
+        return cfg
+    c = useful(8 if ratio >= 1.0 else 4)
+    return P.epc_split_config(xcd, c) if c > 1 else P.epc_without_tpl2(unsplit)
+
+
 @pytest.mark.parametrize("name", sorted(ENSEMBLES))
 def test_tail_config_rule(P, name):
-    """The configuration compaction (and, for small ensembles, the start of a solve)
-    moves the running systems to.  Kept while it gives kEnsFillWaves working waves;
-    below that: 64-target tiles, same XCD placement, and the fewest waves per tile
-    that reach the fill, each doubling adding working waves, or no split where none
-    would."""
+    """The configuration compaction (and the start of a solve) moves the running
+    systems to follows the documented decision table for every running subset and
+    starting configuration, keeps the XCD placement, and never shares a tile among
+    more waves than add working waves."""
     sizes = _sizes(ENSEMBLES[name]())
     n = len(sizes)
-    fill = P.epc_fill_waves()
-    assert fill == 4096 and P.epc_chunk() == 128
+    assert P.epc_fill_waves() == 4096 and P.epc_chunk() == 128
     rng = np.random.default_rng(99)
     for live in _subsets(rng, sizes, 10 if n > 1000 else 40):
         mask = bytes(live.astype(np.uint8))
-        cw = {s: _chunk_waves(sizes, live, s) for s in (1, 2, 4, 8)}
         for cfg in range(P.epc_n_configs()):
-            wpb, tpl2, xcd, split = _config(P, cfg)
-            got = P.epc_tail_config(n, _ip(sizes), mask, cfg)
-            gw, gt, gx, gs = _config(P, got)
-            step = np.where(sizes[live] >= tpl2, 128, 64)
-            waves = cw[split] if split > 1 else int(np.sum(-(-sizes[live] // step)))
-            if waves >= fill:
-                assert got == cfg
-                continue
-            assert gt == 1 << 30 and gx == xcd
-            if gs == 1:
-                assert cw[2] == cw[1] or cw[1] >= fill
-                assert gw == (wpb if split == 1 else 1)
-                continue
-            assert gw == 1 and gs in (2, 4, 8)
-            assert cw[gs] > cw[gs // 2]      # the last doubling added working waves
-            assert cw[gs // 2] < fill        # and was needed
-            assert cw[gs] >= fill or gs == 8 or cw[2 * gs] == cw[gs]
+            xcd = _config(P, cfg)[2]
+            for exact in (0, 1):
+                got = P.epc_tail_config(n, _ip(sizes), mask, cfg, exact)
+                assert got == _tail_model(P, sizes, live, cfg, exact), (name, cfg, exact)
+                gw, gt, gx, gs = _config(P, got)
+                assert gx == xcd
+                if gs > 1:
+                    assert gw == 1 and gt == 1 << 30
+                    assert _chunk_waves(sizes, live, gs) > _chunk_waves(sizes, live, gs // 2)
+                if exact:
+                    assert gs in (1, 8)
 
 
 def test_tail_config_cases(P):
-    # the slowest systems of the 2048-system Plummer ensemble's tail: eight waves a tile
-    tail = _sizes([3393, 2817, 1908, 1279, 1010, 895, 868, 649, 538, 460, 375, 358, 318])
-    got = P.epc_tail_config(len(tail), _ip(tail), None, 1)
-    assert _config(P, got) == (1, 1 << 30, 1, 8)
+    """Subsets of the 2048-system ensemble whose right-hand sides were timed for the
+    table, starting from configuration 7 (one wave per tile, XCD placement)."""
+    sizes = _sizes(ENSEMBLES["c4_2048"]())
+    order = np.argsort(-sizes, kind="stable")
+    cases = {  # subset: (exact, fast) waves per tile
+        "top8": (order[:8], 8, 8), "top16": (order[:16], 1, 8),
+        "top32": (order[:32], 1, 4), "top128": (order[:128], 1, 4),
+        "every32": (order[::32], 8, 8), "every16": (order[::16], 8, 8),
+        "mid64": (order[256:320], 1, 4), "mid16": (order[512:528], 8, 8)}
+    for name, (pick, want_exact, want_fast) in cases.items():
+        live = np.zeros(len(sizes), bool)
+        live[pick] = True
+        mask = bytes(live.astype(np.uint8))
+        for exact, want in ((1, want_exact), (0, want_fast)):
+            got = P.epc_tail_config(len(sizes), _ip(sizes), mask, 7, exact)
+            assert _config(P, got)[3] == want, (name, exact)
+    # the full ensemble keeps its tuned configuration
+    for cfg in (1, 3, 7):
+        for exact in (0, 1):
+            assert P.epc_tail_config(len(sizes), _ip(sizes), None, cfg, exact) == cfg
     # one chunk per system: nothing to share
     small = _sizes([64] * 1000)
-    assert P.epc_tail_config(len(small), _ip(small), None, 3) == 1
-    # 300 bodies: three chunks, so four waves a tile (the fourth adds nothing past three)
-    one = _sizes([300])
-    assert _config(P, P.epc_tail_config(1, _ip(one), None, 6)) == (1, 1 << 30, 0, 4)
-    # enough tiles: unchanged
-    big = _sizes([4096] * 64)
-    assert P.epc_tail_config(len(big), _ip(big), None, 1) == 1
+    assert P.epc_tail_config(len(small), _ip(small), None, 3, 0) == 1

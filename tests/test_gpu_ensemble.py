@@ -233,7 +233,8 @@ def test_ensemble_rhs_split_tiles_bitwise_unsplit(exe, variant):
     128-target tiles (configurations 0-11).  Systems of 1 to 16 chunks, with masses and
     unit masses, and (mixed_near) a near field covering every pair."""
     import ode_prototyping_amd as oh
-    from ode_prototyping_amd.ensemble import ENS_TILE_CONFIGS, ENS_TILE_UNSPLIT, ensemble_rhs, pack
+    from ode_prototyping_amd.ensemble import (ENS_TILE_CONFIGS, ENS_TILE_UNSPLIT, ensemble_rhs,
+                                              pack)
     xs, ms = make_ensemble(SPLIT_SIZES, seed0=90)
     var = "fast" if variant == "fast" else "mixed"
     if variant == "mixed_near":
@@ -403,20 +404,20 @@ def test_ensemble_edge_cases(exe):
 
 
 def test_ensemble_autotune_reports_all_candidates(exe):
-    from ode_prototyping_amd.ensemble import ENS_TILE_UNSPLIT, pack, solve_ensemble
+    from ode_prototyping_amd.ensemble import (ENS_TILE_CONFIGS, ENS_TILE_UNSPLIT, pack,
+                                              solve_ensemble)
     sizes = [300, 20, 1000, 64, 513]
     xs, ms = make_ensemble(sizes, seed0=11)
     res = solve_ensemble("DOPRI5", sizes, pack(xs), 0.0, 1e-3, 1e-10, 1e-10, 1e-3, pack(ms),
                          variant="fast", exe=exe)
-    assert np.all(res.tune_ms[:ENS_TILE_UNSPLIT] > 0.0)
-    assert np.all(res.tune_ms[ENS_TILE_UNSPLIT:] == 0.0)  # split ones are not candidates
+    assert res.tune_ms.size == ENS_TILE_CONFIGS and np.all(res.tune_ms > 0.0)
     # the choice: within 3 % of the fastest candidate, the most target tiles
-    # (128-target tiles for systems of at least 1024 / 256 bodies halve them),
-    # then the fastest of those
-    tpl2 = [None, None, 1024, 1024, 256, 256] * 2   # the candidates' order
+    # (128-target tiles for systems of at least 1024 / 256 bodies halve them;
+    # the shared-tile configurations have 64-target tiles), then the fastest
+    tpl2 = ([None, None, 1024, 1024, 256, 256] * 2 +
+            [None] * (ENS_TILE_CONFIGS - ENS_TILE_UNSPLIT))
     tiles = [sum(-(-n // (128 if t and n >= t else 64)) for n in sizes) for t in tpl2]
-    tuned = res.tune_ms[:ENS_TILE_UNSPLIT]
-    near = [c for c in range(len(tpl2)) if tuned[c] <= 1.03 * tuned.min()]
+    near = [c for c in range(len(tpl2)) if res.tune_ms[c] <= 1.03 * res.tune_ms.min()]
     most = max(tiles[c] for c in near)
     pick = min((c for c in near if tiles[c] == most), key=lambda c: (res.tune_ms[c], c))
     assert res.tile_config == pick

@@ -14,9 +14,12 @@
 // partner coordinates arriving through scalar loads (wave-uniform), and
 // accumulates every target's acceleration in registers -- for the exact
 // variant exactly the increasing-j order of the reference's single-threaded
-// pair loop, so no cross-wave fold is needed.  Tiles are listed largest
-// system first, so the long tiles start early and the short ones fill the
-// tail; the list is built on the host per tile configuration.
+// pair loop, so no cross-wave fold is needed (the fast and mixed variants add
+// partner chunks of 128 in chunk order).  Tiles are listed largest system
+// first, so the long tiles start early and the short ones fill the tail; the
+// list is built on the host per tile configuration.  When few systems still
+// run, a tile can instead be shared by the 2, 4 or 8 waves of a workgroup
+// (k_ens_rhs_split, k_ens_rhs_exact_split), with the same bits.
 #include "ensemble.hpp"
 #include "mlp.hpp"
 #include "nbody_terms.hpp"

@@ -388,21 +388,24 @@ def controller_block(oh, exe):
     x0d, md = oh.HipState(exe, data=x0), oh.HipState(exe, data=m)
     out_state = oh.HipState(exe, x0.size)
     net, info = trc.pretrained(exe, tab)
+    net_fast, info_fast = trc.pretrained(exe, tab, "fast")
     block = {"workload": f"{len(sizes)} held-out systems (seed 9000), N log-uniform "
                          f"[16, 4096], eps={eps}, {tab}, rtol=atol={tol}, t in [0, {tf}], "
-                         "mixed-precision RHS", "learned_net": info}
+                         "mixed-precision and fast FP64 RHS", "learned_net": info,
+             "learned_net_fast": info_fast}
     w2 = sizes.astype(float) ** 2
-    # (key, RHS variant, near-field coefficient, controller): the mixed RHS as trained on,
-    # then with the close-encounter near field and in FP64, where the error norm carries
-    # less rounding noise
-    runs = [("reference", "mixed", 0.0, "reference"), ("pi", "mixed", 0.0, "pi"),
-            ("learned", "mixed", 0.0, "learned"),
-            ("near_reference", "mixed", oh.MIXED_NEAR_CLOSE, "reference"),
-            ("near_learned", "mixed", oh.MIXED_NEAR_CLOSE, "learned"),
-            ("fast_reference", "fast", 0.0, "reference"),
-            ("fast_learned", "fast", 0.0, "learned")]
-    for key, var, near, ctrl in runs:
-        mlp = net if ctrl == "learned" else None
+    # (key, RHS variant, near-field coefficient, controller, network): the mixed RHS with
+    # the network trained on its traces, then with the close-encounter near field; in
+    # FP64 (fast RHS) the mixed-trained network, and the one trained on fast-RHS traces
+    # (its target error chosen on a separate validation ensemble)
+    runs = [("reference", "mixed", 0.0, "reference", None), ("pi", "mixed", 0.0, "pi", None),
+            ("learned", "mixed", 0.0, "learned", net),
+            ("near_reference", "mixed", oh.MIXED_NEAR_CLOSE, "reference", None),
+            ("near_learned", "mixed", oh.MIXED_NEAR_CLOSE, "learned", net),
+            ("fast_reference", "fast", 0.0, "reference", None),
+            ("fast_learned_mixed_net", "fast", 0.0, "learned", net),
+            ("fast_learned", "fast", 0.0, "learned", net_fast)]
+    for key, var, near, ctrl, mlp in runs:
         exe.near_field = near
         solve_ensemble(tab, sizes, x0d, 0.0, tf, tol, tol, eps, md, variant=var, exe=exe,
                        controller=ctrl, mlp=mlp, max_attempts=1, x_out=out_state, fetch=False)
@@ -430,7 +433,9 @@ def controller_block(oh, exe):
     block["learned_gain"] = gain("learned", "reference")
     block["learned_gain_near_field"] = gain("near_learned", "near_reference")
     block["learned_gain_fast_fp64"] = gain("fast_learned", "fast_reference")
+    block["learned_gain_fast_fp64_mixed_net"] = gain("fast_learned_mixed_net", "fast_reference")
     net.close()
+    net_fast.close()
     return block
 
 

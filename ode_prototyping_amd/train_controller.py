@@ -97,11 +97,15 @@ def load(exe, path):
     return net, info
 
 
-def pretrained(exe, tableau):
-    """The network shipped with the package for this tableau."""
-    path = os.path.join(DATA_DIR, f"step_mlp_{tableau.lower()}.npz")
+def pretrained(exe, tableau, variant="mixed"):
+    """The network shipped with the package for this tableau, trained on traces of the
+    given right-hand-side variant (step_mlp_<tableau>.npz: mixed precision;
+    step_mlp_<tableau>_fast.npz: fast FP64, DOP853 only).  The error norms of the two
+    carry different rounding noise, so each network is for its own variant."""
+    suffix = "" if variant == "mixed" else f"_{variant}"
+    path = os.path.join(DATA_DIR, f"step_mlp_{tableau.lower()}{suffix}.npz")
     if not os.path.exists(path):
-        raise FileNotFoundError(f"no trained controller for {tableau} ({path})")
+        raise FileNotFoundError(f"no trained controller for {tableau}, {variant} RHS ({path})")
     return load(exe, path)
 
 

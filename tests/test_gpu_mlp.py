@@ -143,3 +143,36 @@ def test_trained_controller_gain(exe):
     assert gain > 0.05, gain
     assert runs["learned"].total_rhs_evals < runs["pi"].total_rhs_evals
     net.close()
+
+
+def test_trained_controller_gain_fp64(exe):
+    """The shipped DOP853 network for the fast FP64 RHS (trained on its PI-controller
+    traces, target error chosen on a validation ensemble) against the reference
+    controller on held-out systems: fewer RHS evaluations, every accepted step at error
+    norm <= 1, and final states within 1e-8 of the reference controller's."""
+    from ode_prototyping_amd import train_controller as trc
+    from ode_prototyping_amd.ensemble import solve_ensemble
+    net, info = trc.pretrained(exe, "DOP853", "fast")
+    assert info["variant"] == "fast" and info["samples"] > 100000
+    assert info["tau_selection"]["validation_seed"] != 9100
+    sizes, x0, m = trc.workload_ensemble(256, 9100)
+    runs = {}
+    for ctrl in ("reference", "learned"):
+        runs[ctrl] = solve_ensemble("DOP853", sizes, x0, 0.0, 0.05, 1e-10, 1e-10, 1e-3, m,
+                                    variant="fast", exe=exe, controller=ctrl,
+                                    mlp=net if ctrl == "learned" else None, trace_cap=1 << 14)
+    for ctrl, r in runs.items():
+        print(ctrl, r.total_rhs_evals, r.total_accepted, r.total_rejected, r.rounds)
+        # every system reaches t = 0.05 (a first step longer than the interval is not
+        # clipped, as in the reference: such a system ends past it)
+        assert r.n_unfinished == 0 and np.all(r.t >= 0.05)
+        for tr in r.trace:
+            assert np.all(tr[tr[:, 2] == 1, 0] <= 1.0)
+    gain = 1.0 - runs["learned"].total_rhs_evals / runs["reference"].total_rhs_evals
+    assert gain > 0.05, gain
+    same_t = np.flatnonzero(runs["learned"].t == runs["reference"].t)
+    assert same_t.size >= 250
+    a, b = runs["learned"].states(), runs["reference"].states()
+    for k in same_t:
+        assert np.max(np.abs(a[k] - b[k])) <= 1e-8 * np.max(np.abs(b[k])), k
+    net.close()

@@ -43,7 +43,7 @@ def oracle(tableau, x0, tf, eps, m, reduce=O.HIP_ORDER, pow_mode=O.POW_CORRECT, 
 
 def same_decisions(trace_acc, ref):
     """Accept/reject identical wherever the reference's error norm is not within 1e-12
-    of 1 (the north_star's exemption for a norm on the threshold)."""
+    of 1 (a norm on the acceptance threshold may go either way under rounding)."""
     k = min(len(trace_acc), len(ref.trace_acc))
     decided = np.abs(ref.trace_err[:k] - 1.0) > 1e-12
     return bool(np.array_equal(np.asarray(trace_acc[:k]).astype(int)[decided],

@@ -501,6 +501,52 @@ def outputs_block(oh, exe, x0_dev, atol, rtol, ode, n, eps, m, accepted, plain_s
     return block
 
 
+def c1_block(oh, exe, reps=5):
+    """The reference's CPU-runnable configuration: N = 64 uniform cube (seed 0, masses 1/64),
+    eps 1e-3, DOPRI5, rtol = atol = 1e-10, t in [0, 1].  The device solve against the
+    reference's own single-thread and thread-pool paths (oracle/_ref) on this host, and
+    the largest final-state difference between them (the device reduces the error norm in
+    its canonical order and uses a correctly rounded pow, the reference's executors their
+    own orders and glibc's pow)."""
+    n, eps, tol, tf = 64, 1e-3, 1e-10, 1.0
+    x0, m = oh.scenarios.uniform_cube(n, seed=0)
+    ode = oh.NBodyODE(eps, m, "exact")
+    x0_dev = oh.HipState(exe, data=x0)
+    res = oh.integrate("DOPRI5", x0_dev, 0.0, tf, tol, tol, ode, exe=exe)  # untimed
+    exe.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        res = oh.integrate("DOPRI5", x0_dev, 0.0, tf, tol, tol, ode, exe=exe)
+    exe.synchronize()
+    secs = (time.perf_counter() - t) / reps
+    block = {"workload": f"N={n} uniform cube (seed 0, masses 1/{n}), eps={eps}, DOPRI5, "
+                         f"rtol=atol={tol}, t in [0, {tf}]",
+             "gpu": {"seconds": secs, "accepted": res.accepted, "rejected": res.rejected,
+                     "rhs_evals": res.rhs_evals, "particle_steps_per_s": n * res.accepted / secs}}
+    log(f"c1 gpu: {block['gpu']}")
+    ref = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+    if os.path.exists(ref):
+        L = C.CDLL(ref)
+        dp = C.POINTER(C.c_double)
+        L.ref_solve_64_dopri5.argtypes = [C.c_int, dp, C.c_double, C.c_double, C.c_double,
+                                          C.c_double, dp, C.POINTER(C.c_long)]
+        L.ref_solve_64_dopri5.restype = C.c_double
+        threads = max(2, min(16, len(os.sched_getaffinity(0))))
+        for key, th in (("cpu_reference_single_thread", 1), ("cpu_reference_thread_pool", threads)):
+            xo = np.zeros(6 * n)
+            saved = C.c_long()
+            sec = L.ref_solve_64_dopri5(th, np.ascontiguousarray(x0).ctypes.data_as(dp), tol, eps,
+                                        float(m[0]), tf, xo.ctypes.data_as(dp), C.byref(saved))
+            acc = saved.value - 1
+            block[key] = {"threads": th, "seconds": sec, "accepted": acc,
+                          "particle_steps_per_s": n * acc / sec,
+                          "final_state_rel_diff_vs_gpu": float(
+                              np.max(np.abs(xo - res.x_final)) / np.max(np.abs(xo)))}
+            log(f"c1 {key}: {block[key]}")
+    x0_dev.free()
+    return block
+
+
 def precision_block(oh, exe):
     """Mixed precision on 65536 colliding Plummer bodies, DOP853."""
     n, eps, tol, tab, acc = 65536, 1e-2, 1e-10, "DOP853", 10
@@ -838,6 +884,8 @@ def main():
     if not args.no_precision:
         out["precision"] = precision_block(oh, exe)
         out["near_field"] = near_field_block(oh, exe)
+    if not args.no_extras:
+        out["c1"] = c1_block(oh, exe)
 
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

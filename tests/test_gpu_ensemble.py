@@ -67,6 +67,21 @@ def test_ensemble_rhs_exact_bitwise_many_systems(exe, cfg):
         assert np.array_equal(bits(part), bits(ref)), (cfg, k, sizes[k])
 
 
+@pytest.mark.parametrize("cfg", [7, 13, 15, 17])
+def test_ensemble_rhs_exact_bitwise_large_shared_tiles(exe, cfg):
+    """Systems of 513 to 4085 bodies -- the sizes whose tiles the tail rule shares
+    among 8 waves in a solve -- bit for bit as the oracle, one wave per tile and 2, 4 or
+    8 waves per tile (blocks of 8 to 32 partners, several blocks per tile, the own
+    targets' block in the middle of the range)."""
+    from ode_prototyping_amd.ensemble import ensemble_rhs, pack, unpack
+    sizes = [513, 1024, 2047, 4085]
+    xs, ms = make_ensemble(sizes, seed0=700)
+    d, _ = ensemble_rhs(exe, sizes, pack(xs), 1e-3, pack(ms), "exact", tile_config=cfg)
+    for k, part in enumerate(unpack(d, sizes)):
+        ref = O.rhs(O.RHS_NBODY, xs[k], 1e-3, ms[k], threads=8)
+        assert np.array_equal(bits(part), bits(ref)), (cfg, sizes[k])
+
+
 @pytest.mark.parametrize("cfg", [2, 12, 14, 16])
 def test_ensemble_rhs_exact_zero_softening(exe, cfg):
     """eps = 0: the IEEE term path (no shared reciprocal) and the skipped

@@ -375,6 +375,39 @@ def test_ensemble_matches_single_solves_variants(exe, variant):
         assert err < tol, (variant, n, err)
 
 
+def test_ensemble_shared_tiles_dense_output_and_k1_reuse(exe):
+    """Systems of up to 1500 bodies, where a solve shares each tile among 8 waves from
+    the start: the shared tiles' first-same-as-last k1 copy and DOP853 dense-output
+    evaluations.  Exact: every system's dense rows, final state and counters bit for bit
+    as its single-system solve.  Fast: with and without k1 reuse the same bits (the chunk
+    order makes the copied k1 equal to a recomputed one), with fewer RHS evaluations."""
+    import ode_prototyping_amd as oh
+    from ode_prototyping_amd.ensemble import pack, solve_ensemble
+    sizes = [20, 90, 300, 1000, 1500]
+    xs, ms = make_ensemble(sizes, seed0=80)
+    tf = 0.01
+    times = np.linspace(0.0, tf, 6)[1:-1]
+    res = solve_ensemble("DOP853", sizes, pack(xs), 0.0, tf, 1e-10, 1e-10, 1e-3, pack(ms),
+                         exe=exe, dense_times=times, fsal=True)
+    for k in range(len(sizes)):
+        one = oh.integrate("DOP853", xs[k], 0.0, tf, 1e-10, 1e-10,
+                           oh.NBodyODE(1e-3, ms[k], "exact"), exe=exe, dense_times=times,
+                           fsal=True)
+        assert (res.accepted[k], res.rejected[k], res.rhs_evals[k]) == (
+            one.accepted, one.rejected, one.rhs_evals), k
+        assert np.array_equal(bits(res.dense_rows(k)), bits(one.dense_states)), k
+        assert np.array_equal(bits(res.states()[k]), bits(one.x_final)), k
+    runs = [solve_ensemble("DOP853", sizes, pack(xs), 0.0, tf, 1e-10, 1e-10, 1e-3, pack(ms),
+                           variant="fast", exe=exe, dense_times=times, fsal=f)
+            for f in (False, True)]
+    assert np.array_equal(runs[0].accepted, runs[1].accepted)
+    assert np.array_equal(runs[0].rejected, runs[1].rejected)
+    assert np.all(runs[1].rhs_evals <= runs[0].rhs_evals)  # a one-step system reuses none
+    assert runs[1].total_rhs_evals < runs[0].total_rhs_evals
+    assert np.array_equal(bits(runs[0].x_final), bits(runs[1].x_final))
+    assert np.array_equal(bits(runs[0].dense_states), bits(runs[1].dense_states))
+
+
 def test_ensemble_pi_controller_matches_oracle(exe):
     from ode_prototyping_amd.ensemble import pack, solve_ensemble
     sizes = [16, 65, 150]

@@ -489,7 +489,8 @@ typedef struct {
   long accepted, rejected, attempts, rhs_evals, learned_steps; /* sums */
   double particle_steps; /* sum over systems of n_k * accepted_k */
   double interactions;   /* sum n_k (n_k - 1): pair terms per ensemble RHS */
-  int tile_config;       /* the configuration used */
+  int tile_config;       /* the tuned (or requested) configuration; the running
+                            systems may move to a shared-tile one (same results) */
   int n_unfinished;      /* systems still running at return (0 on success) */
   double rhs_ms;         /* autotuner's time of one RHS in that configuration
                             (0 when not tuned) */

@@ -242,11 +242,12 @@ __device__ inline void tile_targets(const double* __restrict__ y, int n, int i0,
   }
 }
 
+// scale: 1, or the common mass when the terms were summed with unit masses.
 template <bool kMass, bool kGuard, int TPL>
 __device__ inline void fast_tile(const double* __restrict__ y,
                                  const double* __restrict__ mass,
                                  double* __restrict__ k, int n, int i0,
-                                 double eps2) {
+                                 double eps2, double scale) {
   const uniform_ptr yu = as_uniform(y);
   const uniform_ptr mu = as_uniform(mass);
   int ic[TPL];
@@ -266,6 +267,12 @@ __device__ inline void fast_tile(const double* __restrict__ y,
       ay[t] += py[t];
       az[t] += pz[t];
     }
+  }
+#pragma unroll
+  for (int t = 0; t < TPL; ++t) {
+    ax[t] *= scale;
+    ay[t] *= scale;
+    az[t] *= scale;
   }
   tile_store<TPL>(k, n, i0, ax, ay, az);
   tile_velocities<TPL>(y, k, n, i0);
@@ -381,7 +388,7 @@ template <bool kMass, bool kGuard, bool kNear, int TPL>
 __device__ inline void mixed_tile(const double* __restrict__ y,
                                   const double* __restrict__ mass,
                                   double* __restrict__ k, int n, int i0,
-                                  double eps2, float near2) {
+                                  double eps2, float near2, double scale) {
   const uniform_ptr yu = as_uniform(y);
   const uniform_ptr mu = as_uniform(mass);
   int ic[TPL];
@@ -402,8 +409,22 @@ __device__ inline void mixed_tile(const double* __restrict__ y,
       az[t] += pz[t];
     }
   }
+#pragma unroll
+  for (int t = 0; t < TPL; ++t) {
+    ax[t] *= scale;
+    ay[t] *= scale;
+    az[t] *= scale;
+  }
   tile_store<TPL>(k, n, i0, ax, ay, az);
   tile_velocities<TPL>(y, k, n, i0);
+}
+
+// The system's common mass when all its masses are equal (finite, nonzero),
+// else 0: from the solver state a solve fills at its start, or, for a plain
+// evaluation, a vote of the calling wave (every lane must call).
+__device__ inline double system_uniform_mass(const SolverState* st, const double* m, int n) {
+  if (st) return st->mass_uniform;
+  return __all(lane_masses_uniform(m, n, threadIdx.x & 63, 64)) ? m[0] : 0.0;
 }
 
 // One wave per tile slot.  kSpecial: shared reciprocal (exact) or the
@@ -449,17 +470,37 @@ __global__ __launch_bounds__(64 * WPB) void k_ens_rhs(EnsBufs b,
       exact_tile<kMass, kSpecial, 2>(y, m, k, n, tile.i0, b.eps2);
     else
       exact_tile<kMass, kSpecial, 1>(y, m, k, n, tile.i0, b.eps2);
-  } else if (kVariant == ODEHIP_NBODY_FAST) {
-    if (tpl == 2)
-      fast_tile<kMass, kSpecial, 2>(y, m, k, n, tile.i0, b.eps2);
-    else
-      fast_tile<kMass, kSpecial, 1>(y, m, k, n, tile.i0, b.eps2);
   } else {
-    const float near2 = kNear ? mixed_near2(b.near_c, n) : 0.0f;
-    if (tpl == 2)
-      mixed_tile<kMass, kSpecial, kNear, 2>(y, m, k, n, tile.i0, b.eps2, near2);
-    else
-      mixed_tile<kMass, kSpecial, kNear, 1>(y, m, k, n, tile.i0, b.eps2, near2);
+    // equal masses: unit-mass terms, each finished sum scaled once
+    const double m_uni = kMass ? system_uniform_mass(st, m, n) : 0.0;
+    const bool per_term = kMass && m_uni == 0.0;
+    const double scale = per_term || !kMass ? 1.0 : m_uni;
+    if (kVariant == ODEHIP_NBODY_FAST) {
+      if (per_term) {
+        if (tpl == 2)
+          fast_tile<true, kSpecial, 2>(y, m, k, n, tile.i0, b.eps2, 1.0);
+        else
+          fast_tile<true, kSpecial, 1>(y, m, k, n, tile.i0, b.eps2, 1.0);
+      } else {
+        if (tpl == 2)
+          fast_tile<false, kSpecial, 2>(y, m, k, n, tile.i0, b.eps2, scale);
+        else
+          fast_tile<false, kSpecial, 1>(y, m, k, n, tile.i0, b.eps2, scale);
+      }
+    } else {
+      const float near2 = kNear ? mixed_near2(b.near_c, n) : 0.0f;
+      if (per_term) {
+        if (tpl == 2)
+          mixed_tile<true, kSpecial, kNear, 2>(y, m, k, n, tile.i0, b.eps2, near2, 1.0);
+        else
+          mixed_tile<true, kSpecial, kNear, 1>(y, m, k, n, tile.i0, b.eps2, near2, 1.0);
+      } else {
+        if (tpl == 2)
+          mixed_tile<false, kSpecial, kNear, 2>(y, m, k, n, tile.i0, b.eps2, near2, scale);
+        else
+          mixed_tile<false, kSpecial, kNear, 1>(y, m, k, n, tile.i0, b.eps2, near2, scale);
+      }
+    }
   }
 }
 
@@ -614,6 +655,10 @@ __global__ __launch_bounds__(64 * C) void k_ens_rhs_split(EnsBufs b,
   const uniform_ptr yu = as_uniform(y);
   const uniform_ptr mu = as_uniform(m);
   const float near2 = kNear ? mixed_near2(b.near_c, n) : 0.0f;
+  // equal masses: unit-mass terms, each finished sum scaled once (as k_ens_rhs)
+  const double m_uni = kMass ? system_uniform_mass(st, m, n) : 0.0;
+  const bool per_term = kMass && m_uni == 0.0;
+  const double scale = per_term || !kMass ? 1.0 : m_uni;
   int ic[1];
   double xi[1], yi[1], zi[1];
   tile_targets<1>(y, n, tile.i0, ic, xi, yi, zi);
@@ -625,11 +670,19 @@ __global__ __launch_bounds__(64 * C) void k_ens_rhs_split(EnsBufs b,
     if (c < n_chunks) {
       const int j0 = c * kEnsChunk;
       const int j1 = min(n, j0 + kEnsChunk);
-      if (kVariant == ODEHIP_NBODY_FAST)
-        fast_chunk<kMass, kSpecial, 1>(yu, mu, j0, j1, ic, xi, yi, zi, b.eps2, px, py, pz);
-      else
-        mixed_chunk<kMass, kSpecial, kNear, 1>(yu, mu, j0, j1, ic, xi, yi, zi, b.eps2, near2,
-                                               px, py, pz);
+      if (kVariant == ODEHIP_NBODY_FAST) {
+        if (per_term)
+          fast_chunk<true, kSpecial, 1>(yu, mu, j0, j1, ic, xi, yi, zi, b.eps2, px, py, pz);
+        else
+          fast_chunk<false, kSpecial, 1>(yu, mu, j0, j1, ic, xi, yi, zi, b.eps2, px, py, pz);
+      } else {
+        if (per_term)
+          mixed_chunk<true, kSpecial, kNear, 1>(yu, mu, j0, j1, ic, xi, yi, zi, b.eps2, near2,
+                                                px, py, pz);
+        else
+          mixed_chunk<false, kSpecial, kNear, 1>(yu, mu, j0, j1, ic, xi, yi, zi, b.eps2,
+                                                 near2, px, py, pz);
+      }
     }
     if (wave > 0) {
       part[wave][0][lane] = px[0];
@@ -649,7 +702,12 @@ __global__ __launch_bounds__(64 * C) void k_ens_rhs_split(EnsBufs b,
     }
     __syncthreads();  // part is written again by the next C chunks
   }
-  if (wave == 0) tile_store<1>(k, n, tile.i0, &ax, &ay, &az);
+  if (wave == 0) {
+    ax *= scale;
+    ay *= scale;
+    az *= scale;
+    tile_store<1>(k, n, tile.i0, &ax, &ay, &az);
+  }
   if (wave == C - 1) tile_velocities<1>(y, k, n, tile.i0);
 }
 
@@ -802,6 +860,19 @@ __global__ void k_ens_init(EnsBufs b, double t0, double tf, long max_acc,
   if (s >= b.n_sys) return;
   ctrl_init(b.st + s, e_tab[b.tableau], t0, tf, max_acc, max_att, fsal, b.tableau,
             b.sys[s].n_groups, 0, b.trace ? b.trace_cap : 0, controller);
+}
+
+// Each system's common mass (SolverState::mass_uniform), one wave per system.
+__global__ __launch_bounds__(64) void k_ens_mass_uniform(EnsBufs b) {
+  const int s = blockIdx.x;
+  if (s >= b.n_sys) return;
+  double m_uni = 0.0;
+  if (b.mass) {
+    const EnsSys sy = b.sys[s];
+    const double* m = b.mass + sy.eoff / 6;
+    m_uni = __all(lane_masses_uniform(m, sy.n, threadIdx.x, 64)) ? m[0] : 0.0;
+  }
+  if (threadIdx.x == 0) b.st[s].mass_uniform = m_uni;
 }
 
 // compute_error_target_kernel (RKEmbeddedParallel.hpp:151-157).
@@ -959,6 +1030,7 @@ hipError_t ens_enqueue_init(const EnsBufs& b, const double* x_in, double t0,
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_ens_init, dim3(sys_grid(b)), dim3(64), 0, s, b, t0, tf,
                      max_acc, max_att, fsal, controller);
+  hipLaunchKernelGGL(k_ens_mass_uniform, dim3(b.n_sys), dim3(64), 0, s, b);
   return hipGetLastError();
 }
 

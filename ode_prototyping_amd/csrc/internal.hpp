@@ -32,6 +32,7 @@ struct SolverState {
   long dense_lo, dense_hi;  // rows the current dense pass writes
   long dense_seen;     // accepted count the dense pass last examined
   long dense_steps, dense_rhs_evals;
+  double mass_uniform;  // ensembles: the system's common mass (all equal), else 0
   int dense_go;        // the dense pass of this attempt runs
   int pad2;
   int cur;       // which ping-pong buffer holds x0

@@ -457,6 +457,38 @@ void k_nbody_exact32(NBodyArgs a) {
 // ------------------------------------------------------------------- fast --
 constexpr int kFastWaves = 16;
 
+// Partners jbeg .. jend - 1 of target i, 8 independent chains per step
+// (wave-uniform scalar loads); kMass: per-term masses.
+template <bool kMass, bool kGuard>
+__device__ __forceinline__ void fast_partners(uniform_ptr yu, uniform_ptr mass, int jbeg,
+                                              int jend, int i, double xi, double yi,
+                                              double zi, double eps2, double& ax, double& ay,
+                                              double& az) {
+  constexpr int U = 8;
+  int j = jbeg;
+  for (; j + U <= jend; j += U) {
+    const uniform_ptr pp = yu + 3 * j;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double mj = kMass ? mass[j + u] : 1.0;
+      fast_term<kMass, kGuard>(pp[3 * u], pp[3 * u + 1], pp[3 * u + 2], mj, j + u, i, xi,
+                               yi, zi, eps2, ax, ay, az);
+    }
+  }
+  for (; j < jend; ++j) {
+    const double mj = kMass ? mass[j] : 1.0;
+    fast_term<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j, i, xi,
+                             yi, zi, eps2, ax, ay, az);
+  }
+}
+
+// The common mass when all n masses are equal (finite, nonzero), else 0
+// (a workgroup vote over strided shares).
+__device__ inline double workgroup_uniform_mass(const double* __restrict__ mass, int n) {
+  const bool ok = lane_masses_uniform(mass, n, threadIdx.x, blockDim.x);
+  return __syncthreads_and(ok) != 0 ? mass[0] : 0.0;
+}
+
 template <bool kMass, bool kGuard>
 __global__ __launch_bounds__(64 * kFastWaves) void k_nbody_fast(NBodyArgs a) {
   if (prologue_skip(a)) return;
@@ -477,24 +509,13 @@ __global__ __launch_bounds__(64 * kFastWaves) void k_nbody_fast(NBodyArgs a) {
   const int seg = (n + kFastWaves - 1) / kFastWaves;
   const int jbeg = min(n, wave * seg);
   const int jend = min(n, jbeg + seg);
+  // equal masses: unit-mass terms and one scale of each finished sum
+  const double m_uni = kMass ? workgroup_uniform_mass(a.mass, n) : 0.0;
   double ax = 0.0, ay = 0.0, az = 0.0;
-  // 8 independent partner chains per step (wave-uniform scalar loads)
-  constexpr int U = 8;
-  int j = jbeg;
-  for (; j + U <= jend; j += U) {
-    const uniform_ptr pp = yu + 3 * j;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const double mj = kMass ? mass[j + u] : 1.0;
-      fast_term<kMass, kGuard>(pp[3 * u], pp[3 * u + 1], pp[3 * u + 2], mj, j + u, i, xi,
-                               yi, zi, eps2, ax, ay, az);
-    }
-  }
-  for (; j < jend; ++j) {
-    const double mj = kMass ? mass[j] : 1.0;
-    fast_term<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j, i, xi,
-                             yi, zi, eps2, ax, ay, az);
-  }
+  if (kMass && m_uni == 0.0)
+    fast_partners<true, kGuard>(yu, mass, jbeg, jend, i, xi, yi, zi, eps2, ax, ay, az);
+  else
+    fast_partners<false, kGuard>(yu, mass, jbeg, jend, i, xi, yi, zi, eps2, ax, ay, az);
   red[wave][0][lane] = ax;
   red[wave][1][lane] = ay;
   red[wave][2][lane] = az;
@@ -503,7 +524,7 @@ __global__ __launch_bounds__(64 * kFastWaves) void k_nbody_fast(NBodyArgs a) {
     double s = 0.0;
 #pragma unroll
     for (int w = 0; w < kFastWaves; ++w) s += red[w][wave][lane];
-    k[3L * n + 3 * i + wave] = s;
+    k[3L * n + 3 * i + wave] = kMass && m_uni != 0.0 ? s * m_uni : s;
   }
   copy_velocities(y, k, n, i0);
 }
@@ -512,29 +533,15 @@ __global__ __launch_bounds__(64 * kFastWaves) void k_nbody_fast(NBodyArgs a) {
 constexpr int kMixedWaves = 16;
 constexpr int kMixedFlush = 32;
 
-// kNear: near pairs (a.near2 > 0) in FP64; without it every pair is FP32.
+// Partners jbeg .. jend - 1 of target i into the FP64 sums a*: FP32 terms
+// flushed every kMixedFlush partners; kNear: near pairs (FP32 r^2 < near2) in
+// FP64; kMass: per-term masses.
 template <bool kMass, bool kGuard, bool kNear>
-__global__ __launch_bounds__(64 * kMixedWaves) void k_nbody_mixed(NBodyArgs a) {
-  if (prologue_skip(a)) return;
-  __shared__ double red[kMixedWaves][3][64];
-  const double* __restrict__ y = select_input(a);
-  const uniform_ptr yu = as_uniform(y);
-  double* __restrict__ k = a.k;
-  const uniform_ptr mass = as_uniform(a.mass);
-  const int n = a.n;
-  const float eps2f = static_cast<float>(a.eps2);
-  const float near2 = a.near2;
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int i0 = blockIdx.x * 64;
-  const int i = i0 + lane;
-  const bool valid_i = i < n;
-  const int ic = valid_i ? i : n - 1;
-  const double xi = y[3 * ic], yi = y[3 * ic + 1], zi = y[3 * ic + 2];
-  const int seg = (n + kMixedWaves - 1) / kMixedWaves;
-  const int jbeg = min(n, wave * seg);
-  const int jend = min(n, jbeg + seg);
-  double ax = 0.0, ay = 0.0, az = 0.0;
+__device__ __forceinline__ void mixed_partners(uniform_ptr yu, uniform_ptr mass, int jbeg,
+                                               int jend, int i, double xi, double yi,
+                                               double zi, float eps2f, double eps2,
+                                               float near2, double& ax, double& ay,
+                                               double& az) {
   int j = jbeg;
   while (j < jend) {
     const int jstop = min(jend, j + kMixedFlush);
@@ -577,7 +584,7 @@ __global__ __launch_bounds__(64 * kMixedWaves) void k_nbody_mixed(NBodyArgs a) {
           near &= near - 1;
           const double mj = kMass ? mass[j + u] : 1.0;
           mixed_near_term<kMass, kGuard>(pp[3 * u], pp[3 * u + 1], pp[3 * u + 2], mj, j + u,
-                                         i, xo, yo, zo, eps2f, a.eps2, near2, ax, ay, az);
+                                         i, xo, yo, zo, eps2f, eps2, near2, ax, ay, az);
         } while (near);
       }
     }
@@ -586,12 +593,45 @@ __global__ __launch_bounds__(64 * kMixedWaves) void k_nbody_mixed(NBodyArgs a) {
       if (!mixed_term<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j, i, xi,
                                      yi, zi, eps2f, near2, fx, fy, fz))
         mixed_near_term<kMass, kGuard>(yu[3 * j], yu[3 * j + 1], yu[3 * j + 2], mj, j, i,
-                                       xi, yi, zi, eps2f, a.eps2, near2, ax, ay, az);
+                                       xi, yi, zi, eps2f, eps2, near2, ax, ay, az);
     }
     ax += static_cast<double>(fx);
     ay += static_cast<double>(fy);
     az += static_cast<double>(fz);
   }
+}
+
+// kNear: near pairs (a.near2 > 0) in FP64; without it every pair is FP32.
+template <bool kMass, bool kGuard, bool kNear>
+__global__ __launch_bounds__(64 * kMixedWaves) void k_nbody_mixed(NBodyArgs a) {
+  if (prologue_skip(a)) return;
+  __shared__ double red[kMixedWaves][3][64];
+  const double* __restrict__ y = select_input(a);
+  const uniform_ptr yu = as_uniform(y);
+  double* __restrict__ k = a.k;
+  const uniform_ptr mass = as_uniform(a.mass);
+  const int n = a.n;
+  const float eps2f = static_cast<float>(a.eps2);
+  const float near2 = a.near2;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i0 = blockIdx.x * 64;
+  const int i = i0 + lane;
+  const bool valid_i = i < n;
+  const int ic = valid_i ? i : n - 1;
+  const double xi = y[3 * ic], yi = y[3 * ic + 1], zi = y[3 * ic + 2];
+  const int seg = (n + kMixedWaves - 1) / kMixedWaves;
+  const int jbeg = min(n, wave * seg);
+  const int jend = min(n, jbeg + seg);
+  // equal masses: unit-mass terms and one FP64 scale of each finished sum
+  const double m_uni = kMass ? workgroup_uniform_mass(a.mass, n) : 0.0;
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  if (kMass && m_uni == 0.0)
+    mixed_partners<true, kGuard, kNear>(yu, mass, jbeg, jend, i, xi, yi, zi, eps2f, a.eps2,
+                                        near2, ax, ay, az);
+  else
+    mixed_partners<false, kGuard, kNear>(yu, mass, jbeg, jend, i, xi, yi, zi, eps2f, a.eps2,
+                                         near2, ax, ay, az);
   red[wave][0][lane] = ax;
   red[wave][1][lane] = ay;
   red[wave][2][lane] = az;
@@ -600,7 +640,7 @@ __global__ __launch_bounds__(64 * kMixedWaves) void k_nbody_mixed(NBodyArgs a) {
     double s = 0.0;
 #pragma unroll
     for (int w = 0; w < kMixedWaves; ++w) s += red[w][wave][lane];
-    k[3L * n + 3 * i + wave] = s;
+    k[3L * n + 3 * i + wave] = kMass && m_uni != 0.0 ? s * m_uni : s;
   }
   copy_velocities(y, k, n, i0);
 }

@@ -97,6 +97,21 @@ __device__ inline bool point_shared_ok(double x, double y, double z) {
   return coord_shared_ok(x) & coord_shared_ok(y) & coord_shared_ok(z);
 }
 
+// The reorderable variants (fast, mixed) with all n masses equal: unit-mass
+// terms, each finished sum scaled by the common mass once (one multiply per
+// pair less; not the per-term rounding, which these variants do not promise).
+// Whether this lane's strided share of the masses all have mass[0]'s bits
+// and mass[0] is finite and nonzero; callers combine the lanes (__all or a
+// workgroup vote).
+__device__ inline bool lane_masses_uniform(const double* __restrict__ mass, int n, int first,
+                                           int stride) {
+  const double m0 = mass[0];
+  const uint64_t b0 = __builtin_bit_cast(uint64_t, m0);
+  bool ok = __builtin_isfinite(m0) && m0 != 0.0;
+  for (int e = first; e < n; e += stride) ok = ok & (__builtin_bit_cast(uint64_t, mass[e]) == b0);
+  return ok;
+}
+
 template <bool kMass, bool kGuard>
 __device__ inline void fast_term(double xj, double yj, double zj, double mj,
                                  int j, int i, double xi, double yi, double zi,

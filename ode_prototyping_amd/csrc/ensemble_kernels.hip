@@ -172,10 +172,7 @@ __device__ __forceinline__ void exact_tile(const double* __restrict__ y,
   // the shared-reciprocal form when every position coordinate of the system
   // passes coord_shared_ok (one pass of the wave over its 3n coordinates)
   bool shared_ok = kSharedRcp;
-  if (kSharedRcp) {
-    for (int e = lane; e < 3 * n; e += 64) shared_ok = shared_ok & coord_shared_ok(y[e]);
-    shared_ok = __all(shared_ok);
-  }
+  if (kSharedRcp) shared_ok = __all(strided_all_shared_ok(y, 3L * n, lane, 64));
   // targets of this wave: i0 .. s1 - 1 (lanes past n are clamped to n - 1)
   const int s1 = min(n, i0 + 64 * TPL);
   if (kSharedRcp && shared_ok) {
@@ -551,12 +548,10 @@ __global__ __launch_bounds__(64 * C) void k_ens_rhs_exact_split(EnsBufs b,
   double xi[1], yi[1], zi[1];
   tile_targets<1>(y, n, tile.i0, ic, xi, yi, zi);
   // the shared-reciprocal form when every position coordinate of the system
-  // passes coord_shared_ok (the same test, on the same data, in every wave)
+  // passes coord_shared_ok (the waves test strided shares, then vote)
   bool shared_ok = kSharedRcp;
-  if (kSharedRcp) {
-    for (int e = lane; e < 3 * n; e += 64) shared_ok = shared_ok & coord_shared_ok(y[e]);
-    shared_ok = __all(shared_ok);
-  }
+  if (kSharedRcp)
+    shared_ok = __syncthreads_and(strided_all_shared_ok(y, 3L * n, threadIdx.x, 64 * C)) != 0;
   const int i0 = tile.i0;
   const int s1 = min(n, i0 + 64);
   // acc0: component `wave` (waves 0..2); acc1: component 2 on wave 0 when C == 2

@@ -79,9 +79,7 @@ __device__ inline void copy_velocities(const double* __restrict__ y,
 // Whether every position coordinate of the state passes coord_shared_ok:
 // all threads of the workgroup test a strided share of the 3N coordinates.
 __device__ inline bool all_positions_shared_ok(const double* __restrict__ y, int n) {
-  bool ok = true;
-  const long lim = 3L * n;
-  for (long e = threadIdx.x; e < lim; e += blockDim.x) ok = ok & coord_shared_ok(y[e]);
+  const bool ok = strided_all_shared_ok(y, 3L * n, threadIdx.x, blockDim.x);
   return __syncthreads_and(ok) != 0;
 }
 
@@ -97,8 +95,7 @@ __device__ inline double uniform_pow2_mass(const double* __restrict__ mass, int 
   const uint64_t b0 = __builtin_bit_cast(uint64_t, mass[0]);
   const uint64_t ex = b0 >> 52;  // sign 0 and the biased exponent
   bool ok = (b0 & 0x000fffffffffffffull) == 0 && ex >= 1023 - 300 && ex <= 1023 + 300;
-  for (int e = threadIdx.x; e < n; e += blockDim.x)
-    ok = ok & (__builtin_bit_cast(uint64_t, mass[e]) == b0);
+  ok = ok & strided_all_bits(mass, n, threadIdx.x, blockDim.x, b0);
   return __syncthreads_and(ok) != 0 ? mass[0] : 0.0;
 }
 

@@ -97,6 +97,49 @@ __device__ inline bool point_shared_ok(double x, double y, double z) {
   return coord_shared_ok(x) & coord_shared_ok(y) & coord_shared_ok(z);
 }
 
+// Whole-array tests over a thread's strided share (elements first, first +
+// stride, ... < lim), with kScanLoads loads issued per round trip: a loop
+// that waits for every load would take one memory latency per element.
+constexpr int kScanLoads = 16;
+
+// Indices past lim load element e instead (in range: the loop runs while
+// e < lim) and are left out of the test, so every load is unconditional.
+
+// Every element passes coord_shared_ok.
+__device__ inline bool strided_all_shared_ok(const double* __restrict__ v, long lim,
+                                             long first, long stride) {
+  bool ok = true;
+  for (long e = first; e < lim; e += kScanLoads * stride) {
+    double c[kScanLoads];
+#pragma unroll
+    for (int u = 0; u < kScanLoads; ++u) {
+      const long x = e + u * stride;
+      c[u] = v[x < lim ? x : e];
+    }
+#pragma unroll
+    for (int u = 0; u < kScanLoads; ++u)
+      ok = ok & ((e + u * stride >= lim) | coord_shared_ok(c[u]));
+  }
+  return ok;
+}
+
+// Every element has the bit pattern b.
+__device__ inline bool strided_all_bits(const double* __restrict__ v, long lim, long first,
+                                        long stride, uint64_t b) {
+  bool ok = true;
+  for (long e = first; e < lim; e += kScanLoads * stride) {
+    uint64_t c[kScanLoads];
+#pragma unroll
+    for (int u = 0; u < kScanLoads; ++u) {
+      const long x = e + u * stride;
+      c[u] = __builtin_bit_cast(uint64_t, v[x < lim ? x : e]);
+    }
+#pragma unroll
+    for (int u = 0; u < kScanLoads; ++u) ok = ok & ((e + u * stride >= lim) | (c[u] == b));
+  }
+  return ok;
+}
+
 // The reorderable variants (fast, mixed) with all n masses equal: unit-mass
 // terms, each finished sum scaled by the common mass once (one multiply per
 // pair less; not the per-term rounding, which these variants do not promise).
@@ -106,10 +149,8 @@ __device__ inline bool point_shared_ok(double x, double y, double z) {
 __device__ inline bool lane_masses_uniform(const double* __restrict__ mass, int n, int first,
                                            int stride) {
   const double m0 = mass[0];
-  const uint64_t b0 = __builtin_bit_cast(uint64_t, m0);
-  bool ok = __builtin_isfinite(m0) && m0 != 0.0;
-  for (int e = first; e < n; e += stride) ok = ok & (__builtin_bit_cast(uint64_t, mass[e]) == b0);
-  return ok;
+  const bool ok = __builtin_isfinite(m0) && m0 != 0.0;
+  return ok & strided_all_bits(mass, n, first, stride, __builtin_bit_cast(uint64_t, m0));
 }
 
 template <bool kMass, bool kGuard>

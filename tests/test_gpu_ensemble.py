@@ -115,6 +115,72 @@ def test_ensemble_rhs_fast_mixed(exe, ens, variant, tol, cfg):
         assert err.max() < tol, (variant, n, err.max())
 
 
+def mass_forms(sizes, seed):
+    """Per system, in turn: equal masses 1 / n, equal 0.37 / n (not a power
+    of two), masses spread over [0.5, 2] / n, and 1 / n but for the last
+    body (1.5 / n)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k, n in enumerate(sizes):
+        if k % 4 == 0:
+            m = np.full(n, 1.0 / n)
+        elif k % 4 == 1:
+            m = np.full(n, 0.37 / n)
+        elif k % 4 == 2:
+            m = rng.uniform(0.5, 2.0, n) / n
+        else:
+            m = np.full(n, 1.0 / n)
+            m[-1] = 1.5 / n
+        out.append(m)
+    return out
+
+
+@pytest.mark.parametrize("variant,tol", [("fast", 2e-14), ("mixed", 5e-6)])
+@pytest.mark.parametrize("cfg", [0, 7, 11, 12, 17])
+def test_ensemble_rhs_fast_mixed_mass_forms(exe, variant, tol, cfg):
+    """Systems with equal masses (unit-mass terms, one scale of each finished
+    sum) beside systems with spread masses and with one differing mass
+    (per-term products), in one-wave, 128-target and shared tiles: every
+    acceleration within the variant's bar of the oracle, scaled as in
+    test_ensemble_rhs_fast_mixed."""
+    from ode_prototyping_amd.ensemble import ensemble_rhs, pack, unpack
+    sizes = SPLIT_SIZES
+    xs, _ = make_ensemble(sizes, seed0=300)
+    ms = mass_forms(sizes, 300)
+    d, _ = ensemble_rhs(exe, sizes, pack(xs), 1e-3, pack(ms), variant, tile_config=cfg)
+    for k, part in enumerate(unpack(d, sizes)):
+        n = sizes[k]
+        ref = O.rhs(O.RHS_NBODY, xs[k], 1e-3, ms[k])
+        assert np.array_equal(part[:3 * n], ref[:3 * n])
+        pos = xs[k][:3 * n].reshape(n, 3)
+        r = pos[None, :, :] - pos[:, None, :]
+        w = ms[k][None, :] / (np.sum(r * r, axis=2) + 1e-6) ** 1.5
+        scale = np.sum(np.abs(r) * w[:, :, None], axis=1).ravel() + 1e-300
+        err = np.abs(part[3 * n:] - ref[3 * n:]) / scale
+        assert err.max() < tol, (variant, cfg, n, k % 4, err.max())
+
+
+@pytest.mark.parametrize("variant,bar", [("fast", 1e-9), ("mixed", 1e-6)])
+def test_ensemble_solve_fast_mixed_mass_forms(exe, variant, bar):
+    """An ensemble solve records at its start whether each system's masses are
+    all equal, and its right-hand sides take the equal-mass form from that
+    record: systems of every mass form follow their exact oracle solve to
+    the variant's accuracy (a misweighed body would move the state by about
+    1e-4 of its size by t = 0.01)."""
+    from ode_prototyping_amd.ensemble import pack, solve_ensemble
+    sizes = [40, 41, 42, 43, 300, 301, 302, 303]
+    xs, _ = make_ensemble(sizes, seed0=400)
+    ms = mass_forms(sizes, 400)
+    res = solve_ensemble("DOPRI5", sizes, pack(xs), 0.0, 0.01, 1e-10, 1e-10, 1e-3, pack(ms),
+                         variant=variant, exe=exe)
+    assert res.n_unfinished == 0
+    finals = res.states()
+    for k in range(len(sizes)):
+        ref = oracle_solve("DOPRI5", xs[k], 0.01, 1e-10, 1e-3, ms[k])
+        err = np.max(np.abs(finals[k] - ref.x_final)) / np.max(np.abs(ref.x_final))
+        assert err < bar, (variant, sizes[k], k % 4, err)
+
+
 def oracle_solve(tableau, x, tf, tol, eps, m, **kw):
     return O.integrate(tableau, O.RHS_NBODY, x, 0.0, tf, tol, tol, softening=eps, masses=m,
                        reduce=O.HIP_ORDER, pow_mode=O.POW_CORRECT, store_states=False,
@@ -245,8 +311,9 @@ def test_ensemble_rhs_split_tiles_bitwise_unsplit(exe, variant):
     """The fast and mixed sums run over partner chunks of 128 in chunk order, so one
     tile's chunks shared by 2, 4 or 8 waves of a workgroup (configurations 12-17, XCD
     placement off and on) give the same bits as one wave per tile, with 64- or
-    128-target tiles (configurations 0-11).  Systems of 1 to 16 chunks, with masses and
-    unit masses, and (mixed_near) a near field covering every pair."""
+    128-target tiles (configurations 0-11).  Systems of 1 to 16 chunks, with equal masses,
+    unit masses and the mixed mass forms of mass_forms, and (mixed_near) a near field
+    covering every pair."""
     import ode_prototyping_amd as oh
     from ode_prototyping_amd.ensemble import (ENS_TILE_CONFIGS, ENS_TILE_UNSPLIT, ensemble_rhs,
                                               pack)
@@ -255,7 +322,7 @@ def test_ensemble_rhs_split_tiles_bitwise_unsplit(exe, variant):
     if variant == "mixed_near":
         exe.near_field = 1e30
     try:
-        for masses in (pack(ms), None):
+        for masses in (pack(ms), None, pack(mass_forms(SPLIT_SIZES, 91))):
             base, _ = ensemble_rhs(exe, SPLIT_SIZES, pack(xs), 1e-3, masses, var, tile_config=6)
             assert np.all(np.isfinite(base))
             for cfg in list(range(ENS_TILE_UNSPLIT)) + list(range(ENS_TILE_UNSPLIT,

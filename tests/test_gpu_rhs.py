@@ -95,6 +95,45 @@ def test_mixed_rhs_close(exe, n):
     assert np.max(np.abs(acc_g - acc_w)) <= 2e-6 * scale
 
 
+def mass_case(n, case, seed):
+    """Masses of one mass form: all equal (0.37 / n, not a power of two),
+    spread over [0.5, 2] / n, or equal 1 / n but for the first or the last
+    body (1.5 / n)."""
+    if case == "equal":
+        return np.full(n, 0.37 / n)
+    if case == "spread":
+        return np.random.default_rng(seed).uniform(0.5, 2.0, n) / n
+    m = np.full(n, 1.0 / n)
+    m[0 if case == "first_differs" else -1] = 1.5 / n
+    return m
+
+
+@pytest.mark.parametrize("variant,bar", [("fast", 5e-14), ("mixed", 2e-6), ("mixed_near", 2e-6)])
+@pytest.mark.parametrize("case", ["equal", "spread", "first_differs", "last_differs"])
+@pytest.mark.parametrize("n", [1000, 4096])
+def test_fast_mixed_rhs_mass_forms(exe, variant, bar, case, n):
+    """Equal masses take unit-mass terms and one scale of each finished sum,
+    any other masses the per-term products.  Every form meets the variant's
+    bar against the oracle's per-term products; a body whose mass differs
+    from the rest, first or last, must keep the workgroups off the equal-mass
+    form (which would misweigh its terms by a third: about 1e-3 of the
+    largest acceleration here, far above the mixed bar)."""
+    import ode_prototyping_amd as oh
+    x, _ = ic(n, seed=19)
+    m = mass_case(n, case, 19)
+    if variant == "mixed_near":
+        exe.near_field = oh.MIXED_NEAR_CLOSE
+    try:
+        got = gpu_rhs(exe, x, 1e-2, m, "fast" if variant == "fast" else "mixed")
+    finally:
+        exe.near_field = oh.MIXED_NEAR_DEFAULT
+    want = O.rhs(O.RHS_NBODY, x, 1e-2, m, threads=8)
+    assert np.array_equal(got[:3 * n], want[:3 * n])
+    scale = np.max(np.abs(want[3 * n:]))
+    err = np.max(np.abs(got[3 * n:] - want[3 * n:]))
+    assert err <= bar * scale, (variant, case, n, err / scale)
+
+
 @pytest.mark.parametrize("n", [2, 65, 1000])
 @pytest.mark.parametrize("eps", [1e-2, 0.0])
 def test_mixed_rhs_all_near_is_fp64(exe, n, eps):

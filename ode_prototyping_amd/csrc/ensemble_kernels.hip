@@ -172,7 +172,8 @@ __device__ __forceinline__ void exact_tile(const double* __restrict__ y,
   // the shared-reciprocal form when every position coordinate of the system
   // passes coord_shared_ok (one pass of the wave over its 3n coordinates)
   bool shared_ok = kSharedRcp;
-  if (kSharedRcp) shared_ok = __all(strided_all_shared_ok(y, 3L * n, lane, 64));
+  if (kSharedRcp)
+    shared_ok = __all(strided_all_shared_ok<kScanLoadsLean, int>(y, 3 * n, lane, 64));
   // targets of this wave: i0 .. s1 - 1 (lanes past n are clamped to n - 1)
   const int s1 = min(n, i0 + 64 * TPL);
   if (kSharedRcp && shared_ok) {
@@ -421,7 +422,7 @@ __device__ inline void mixed_tile(const double* __restrict__ y,
 // evaluation, a vote of the calling wave (every lane must call).
 __device__ inline double system_uniform_mass(const SolverState* st, const double* m, int n) {
   if (st) return st->mass_uniform;
-  return __all(lane_masses_uniform(m, n, threadIdx.x & 63, 64)) ? m[0] : 0.0;
+  return __all(lane_masses_uniform<kScanLoadsLean>(m, n, threadIdx.x & 63, 64)) ? m[0] : 0.0;
 }
 
 // One wave per tile slot.  kSpecial: shared reciprocal (exact) or the
@@ -551,7 +552,8 @@ __global__ __launch_bounds__(64 * C) void k_ens_rhs_exact_split(EnsBufs b,
   // passes coord_shared_ok (the waves test strided shares, then vote)
   bool shared_ok = kSharedRcp;
   if (kSharedRcp)
-    shared_ok = __syncthreads_and(strided_all_shared_ok(y, 3L * n, threadIdx.x, 64 * C)) != 0;
+    shared_ok =
+        __syncthreads_and(strided_all_shared_ok<kScanLoads, int>(y, 3 * n, threadIdx.x, 64 * C)) != 0;
   const int i0 = tile.i0;
   const int s1 = min(n, i0 + 64);
   // acc0: component `wave` (waves 0..2); acc1: component 2 on wave 0 when C == 2

@@ -454,23 +454,75 @@ void k_nbody_exact32(NBodyArgs a) {
 // ------------------------------------------------------------------- fast --
 constexpr int kFastWaves = 16;
 
-// Partners jbeg .. jend - 1 of target i, 8 independent chains per step
-// (wave-uniform scalar loads); kMass: per-term masses.
+// kFastSet partners' coordinates (and masses), wave-uniform in SGPRs.
+constexpr int kFastSet = 4;
+template <bool kMass>
+struct FastSet {
+  double c[3 * kFastSet];
+  double m[kMass ? kFastSet : 1];
+};
+
+template <bool kMass>
+__device__ __forceinline__ void fast_load(FastSet<kMass>& s, uniform_ptr yu, uniform_ptr mass,
+                                          int j) {
+  const uniform_ptr pp = yu + 3 * j;
+#pragma unroll
+  for (int e = 0; e < 3 * kFastSet; ++e) s.c[e] = pp[e];
+  if constexpr (kMass) {
+#pragma unroll
+    for (int u = 0; u < kFastSet; ++u) s.m[u] = mass[j + u];
+  }
+}
+
+template <bool kMass, bool kGuard>
+__device__ __forceinline__ void fast_set_terms(const FastSet<kMass>& s, int j, int i, double xi,
+                                               double yi, double zi, double eps2, double& ax,
+                                               double& ay, double& az) {
+#pragma unroll
+  for (int u = 0; u < kFastSet; ++u)
+    fast_term<kMass, kGuard>(s.c[3 * u], s.c[3 * u + 1], s.c[3 * u + 2], kMass ? s.m[u] : 1.0,
+                             j + u, i, xi, yi, zi, eps2, ax, ay, az);
+}
+
+// Waits for every scalar load issued so far.  Scalar loads complete out of
+// order, so the wait counter cannot single out an older set while a newer one
+// is in flight: the partner loop therefore drains the set it is about to use
+// (issued one set earlier) before it issues the next.  The scheduling
+// barriers keep the wait where it is written: after the previous set's terms
+// and before the next set's loads.
+__device__ __forceinline__ void scalar_loads_drain() {
+  __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0); vector and export counters untouched
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// Partners jbeg .. jend - 1 of target i in increasing j (wave-uniform scalar
+// loads); kMass: per-term masses.  Full sets of kFastSet partners alternate
+// between two SGPR sets, each loaded while the other's terms are computed.
 template <bool kMass, bool kGuard>
 __device__ __forceinline__ void fast_partners(uniform_ptr yu, uniform_ptr mass, int jbeg,
                                               int jend, int i, double xi, double yi,
                                               double zi, double eps2, double& ax, double& ay,
                                               double& az) {
-  constexpr int U = 8;
+  constexpr int U = kFastSet;
   int j = jbeg;
-  for (; j + U <= jend; j += U) {
-    const uniform_ptr pp = yu + 3 * j;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const double mj = kMass ? mass[j + u] : 1.0;
-      fast_term<kMass, kGuard>(pp[3 * u], pp[3 * u + 1], pp[3 * u + 2], mj, j + u, i, xi,
-                               yi, zi, eps2, ax, ay, az);
+  if (j + U <= jend) {
+    FastSet<kMass> a, b;
+    fast_load<kMass>(a, yu, mass, j);
+    // a holds partners j .. j + U - 1 (j + U <= jend); the next two sets are in range
+    for (; j + 3 * U <= jend; j += 2 * U) {
+      scalar_loads_drain();
+      fast_load<kMass>(b, yu, mass, j + U);
+      __builtin_amdgcn_sched_barrier(0);  // issue the loads before the terms
+      fast_set_terms<kMass, kGuard>(a, j, i, xi, yi, zi, eps2, ax, ay, az);
+      scalar_loads_drain();
+      fast_load<kMass>(a, yu, mass, j + 2 * U);
+      __builtin_amdgcn_sched_barrier(0);
+      fast_set_terms<kMass, kGuard>(b, j + U, i, xi, yi, zi, eps2, ax, ay, az);
     }
+    scalar_loads_drain();
+    fast_set_terms<kMass, kGuard>(a, j, i, xi, yi, zi, eps2, ax, ay, az);
+    j += U;
   }
   for (; j < jend; ++j) {
     const double mj = kMass ? mass[j] : 1.0;
@@ -482,7 +534,7 @@ __device__ __forceinline__ void fast_partners(uniform_ptr yu, uniform_ptr mass, 
 // The common mass when all n masses are equal (finite, nonzero), else 0
 // (a workgroup vote over strided shares).
 __device__ inline double workgroup_uniform_mass(const double* __restrict__ mass, int n) {
-  const bool ok = lane_masses_uniform(mass, n, threadIdx.x, blockDim.x);
+  const bool ok = lane_masses_uniform<kScanLoadsLean>(mass, n, threadIdx.x, blockDim.x);
   return __syncthreads_and(ok) != 0 ? mass[0] : 0.0;
 }
 

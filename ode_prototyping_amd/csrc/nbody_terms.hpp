@@ -98,44 +98,59 @@ __device__ inline bool point_shared_ok(double x, double y, double z) {
 }
 
 // Whole-array tests over a thread's strided share (elements first, first +
-// stride, ... < lim), with kScanLoads loads issued per round trip: a loop
-// that waits for every load would take one memory latency per element.
+// stride, ... < lim), with L loads issued per round trip: a loop that waits
+// for every load would take one memory latency per element.  Each load in
+// flight holds VGPRs for its value and address, and the kernel's VGPR count
+// is its largest anywhere, so kernels whose occupancy is set by VGPRs (the
+// reorderable variants) scan kScanLoadsLean at a time; kernels held to a few
+// waves per SIMD by their LDS or main loop anyway scan kScanLoads.
 constexpr int kScanLoads = 16;
+constexpr int kScanLoadsLean = 4;
 
 // Indices past lim load element e instead (in range: the loop runs while
 // e < lim) and are left out of the test, so every load is unconditional.
 
 // Every element passes coord_shared_ok.
-__device__ inline bool strided_all_shared_ok(const double* __restrict__ v, long lim,
-                                             long first, long stride) {
+// I: the index type (int where every index fits, which saves VGPRs); the
+// arguments convert to it rather than choose it.
+template <typename T>
+struct ScanIndex {
+  using type = T;
+};
+template <int L = kScanLoads, typename I = long>
+__device__ inline bool strided_all_shared_ok(const double* __restrict__ v,
+                                             typename ScanIndex<I>::type lim,
+                                             typename ScanIndex<I>::type first,
+                                             typename ScanIndex<I>::type stride) {
   bool ok = true;
-  for (long e = first; e < lim; e += kScanLoads * stride) {
-    double c[kScanLoads];
+  for (I e = first; e < lim; e += L * stride) {
+    double c[L];
 #pragma unroll
-    for (int u = 0; u < kScanLoads; ++u) {
-      const long x = e + u * stride;
+    for (int u = 0; u < L; ++u) {
+      const I x = e + u * stride;
       c[u] = v[x < lim ? x : e];
     }
 #pragma unroll
-    for (int u = 0; u < kScanLoads; ++u)
+    for (int u = 0; u < L; ++u)
       ok = ok & ((e + u * stride >= lim) | coord_shared_ok(c[u]));
   }
   return ok;
 }
 
 // Every element has the bit pattern b.
+template <int L = kScanLoads>
 __device__ inline bool strided_all_bits(const double* __restrict__ v, long lim, long first,
                                         long stride, uint64_t b) {
   bool ok = true;
-  for (long e = first; e < lim; e += kScanLoads * stride) {
-    uint64_t c[kScanLoads];
+  for (long e = first; e < lim; e += L * stride) {
+    uint64_t c[L];
 #pragma unroll
-    for (int u = 0; u < kScanLoads; ++u) {
+    for (int u = 0; u < L; ++u) {
       const long x = e + u * stride;
       c[u] = __builtin_bit_cast(uint64_t, v[x < lim ? x : e]);
     }
 #pragma unroll
-    for (int u = 0; u < kScanLoads; ++u) ok = ok & ((e + u * stride >= lim) | (c[u] == b));
+    for (int u = 0; u < L; ++u) ok = ok & ((e + u * stride >= lim) | (c[u] == b));
   }
   return ok;
 }
@@ -146,11 +161,12 @@ __device__ inline bool strided_all_bits(const double* __restrict__ v, long lim, 
 // Whether this lane's strided share of the masses all have mass[0]'s bits
 // and mass[0] is finite and nonzero; callers combine the lanes (__all or a
 // workgroup vote).
+template <int L = kScanLoads>
 __device__ inline bool lane_masses_uniform(const double* __restrict__ mass, int n, int first,
                                            int stride) {
   const double m0 = mass[0];
   const bool ok = __builtin_isfinite(m0) && m0 != 0.0;
-  return ok & strided_all_bits(mass, n, first, stride, __builtin_bit_cast(uint64_t, m0));
+  return ok & strided_all_bits<L>(mass, n, first, stride, __builtin_bit_cast(uint64_t, m0));
 }
 
 template <bool kMass, bool kGuard>

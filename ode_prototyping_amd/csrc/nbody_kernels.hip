@@ -497,16 +497,30 @@ __device__ __forceinline__ void scalar_loads_drain() {
 }
 
 // Partners jbeg .. jend - 1 of target i in increasing j (wave-uniform scalar
-// loads); kMass: per-term masses.  Full sets of kFastSet partners alternate
-// between two SGPR sets, each loaded while the other's terms are computed.
-template <bool kMass, bool kGuard>
+// loads); kMass: per-term masses.  kPrefetch: full sets of kFastSet partners
+// alternate between two SGPR sets, each loaded while the other's terms are
+// computed -- worth it with 4 waves per SIMD (one workgroup per CU), whose
+// scalar-load waits the other waves cannot cover; with 8 waves per SIMD the
+// extra waits cost more than they hide, and steps of 8 partners that load and
+// then wait are faster.  Both add the terms in the same order.
+template <bool kMass, bool kGuard, bool kPrefetch>
 __device__ __forceinline__ void fast_partners(uniform_ptr yu, uniform_ptr mass, int jbeg,
                                               int jend, int i, double xi, double yi,
                                               double zi, double eps2, double& ax, double& ay,
                                               double& az) {
   constexpr int U = kFastSet;
   int j = jbeg;
-  if (j + U <= jend) {
+  if (!kPrefetch) {
+    for (; j + 8 <= jend; j += 8) {
+      const uniform_ptr pp = yu + 3 * j;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const double mj = kMass ? mass[j + u] : 1.0;
+        fast_term<kMass, kGuard>(pp[3 * u], pp[3 * u + 1], pp[3 * u + 2], mj, j + u, i, xi,
+                                 yi, zi, eps2, ax, ay, az);
+      }
+    }
+  } else if (j + U <= jend) {
     FastSet<kMass> a, b;
     fast_load<kMass>(a, yu, mass, j);
     // a holds partners j .. j + U - 1 (j + U <= jend); the next two sets are in range
@@ -538,7 +552,7 @@ __device__ inline double workgroup_uniform_mass(const double* __restrict__ mass,
   return __syncthreads_and(ok) != 0 ? mass[0] : 0.0;
 }
 
-template <bool kMass, bool kGuard>
+template <bool kMass, bool kGuard, bool kPrefetch>
 __global__ __launch_bounds__(64 * kFastWaves) void k_nbody_fast(NBodyArgs a) {
   if (prologue_skip(a)) return;
   __shared__ double red[kFastWaves][3][64];
@@ -562,9 +576,11 @@ __global__ __launch_bounds__(64 * kFastWaves) void k_nbody_fast(NBodyArgs a) {
   const double m_uni = kMass ? workgroup_uniform_mass(a.mass, n) : 0.0;
   double ax = 0.0, ay = 0.0, az = 0.0;
   if (kMass && m_uni == 0.0)
-    fast_partners<true, kGuard>(yu, mass, jbeg, jend, i, xi, yi, zi, eps2, ax, ay, az);
+    fast_partners<true, kGuard, kPrefetch>(yu, mass, jbeg, jend, i, xi, yi, zi, eps2, ax, ay,
+                                           az);
   else
-    fast_partners<false, kGuard>(yu, mass, jbeg, jend, i, xi, yi, zi, eps2, ax, ay, az);
+    fast_partners<false, kGuard, kPrefetch>(yu, mass, jbeg, jend, i, xi, yi, zi, eps2, ax, ay,
+                                            az);
   red[wave][0][lane] = ax;
   red[wave][1][lane] = ay;
   red[wave][2][lane] = az;
@@ -837,19 +853,20 @@ hipError_t launch_nbody_rhs(const NBodyArgs& a, int variant, hipStream_t s) {
       }
       break;
     }
-    case ODEHIP_NBODY_FAST:
-      if (mass) {
-        if (guard)
-          hipLaunchKernelGGL((k_nbody_fast<true, true>), dim3(blocks), dim3(64 * kFastWaves), 0, s, a);
-        else
-          hipLaunchKernelGGL((k_nbody_fast<true, false>), dim3(blocks), dim3(64 * kFastWaves), 0, s, a);
-      } else {
-        if (guard)
-          hipLaunchKernelGGL((k_nbody_fast<false, true>), dim3(blocks), dim3(64 * kFastWaves), 0, s, a);
-        else
-          hipLaunchKernelGGL((k_nbody_fast<false, false>), dim3(blocks), dim3(64 * kFastWaves), 0, s, a);
-      }
+    case ODEHIP_NBODY_FAST: {
+      // one workgroup per CU at most: 4 waves per SIMD, prefetching partner loop
+      const bool prefetch = static_cast<long>(blocks) <= device_cus();
+      auto* kern = mass ? (guard ? (prefetch ? k_nbody_fast<true, true, true>
+                                             : k_nbody_fast<true, true, false>)
+                                 : (prefetch ? k_nbody_fast<true, false, true>
+                                             : k_nbody_fast<true, false, false>))
+                        : (guard ? (prefetch ? k_nbody_fast<false, true, true>
+                                             : k_nbody_fast<false, true, false>)
+                                 : (prefetch ? k_nbody_fast<false, false, true>
+                                             : k_nbody_fast<false, false, false>));
+      hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * kFastWaves), 0, s, a);
       break;
+    }
     case ODEHIP_NBODY_MIXED: {
       const bool guardf = !(static_cast<float>(a.eps2) >= 1e-12f);
       const bool near = a.near2 > 0.0f;

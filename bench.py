@@ -43,11 +43,11 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (and matrix) peak
 FLOP_PER_INTERACTION = 20
 # FP64 VALU wave-instructions per 64 directed interactions of the FP64 RHS
-# kernels (SQ_INSTS_VALU x 64 / N^2 at N=16384 with masses 1/N = 2^-14, which the exact
-# kernel takes as unit-mass terms and one scale per sum;
+# kernels (SQ_INSTS_VALU x 64 / N^2 at N=16384 with masses 1/N = 2^-14, which both kernels
+# take as unit-mass terms and one scale per sum;
 # profiles/nbody_rhs_n16384_valu_counters.txt) and how many of them are quarter-rate
 # transcendentals (v_rsq_f64, v_rcp_f64).
-VALU_PER_64_INTERACTIONS = {"exact": (38.16, 2), "fast": (18.02, 1)}
+VALU_PER_64_INTERACTIONS = {"exact": (38.38, 2), "fast": (17.18, 1)}
 SIMDS = 1024  # 256 CUs x 4 SIMDs
 
 

@@ -28,10 +28,10 @@ def test_more_than_one_gpu_is_refused():
 def test_instruction_bound_closed_form():
     import bench
     b = bench.instruction_bound("exact", 16384, 2000.0, 0.5)
-    # 38.16 VALU instructions per 64 interactions (the counters of the power-of-two
-    # mass path): 36.16 full-rate x 4 + 2 quarter-rate x 16 cycles
-    assert b["cycles_per_64_interactions"] == pytest.approx(36.16 * 4 + 32)
-    want_ms = 16384 ** 2 / 64 / 1024 * (36.16 * 4 + 32) / 2.0e6
+    # 38.38 VALU instructions per 64 interactions (the counters of the power-of-two
+    # mass path): 36.38 full-rate x 4 + 2 quarter-rate x 16 cycles
+    assert b["cycles_per_64_interactions"] == pytest.approx(36.38 * 4 + 32)
+    want_ms = 16384 ** 2 / 64 / 1024 * (36.38 * 4 + 32) / 2.0e6
     assert b["bound_ms"] == pytest.approx(want_ms)
     assert b["frac"] == pytest.approx(want_ms / 0.5)
     # twice the clock halves the bound; four times the particles quadruples it twice

@@ -134,6 +134,24 @@ def test_fast_mixed_rhs_mass_forms(exe, variant, bar, case, n):
     assert err <= bar * scale, (variant, case, n, err / scale)
 
 
+@pytest.mark.parametrize("n", [16384, 16385, 20001])
+@pytest.mark.parametrize("variant,bar", [("fast", 5e-14), ("mixed", 2e-6)])
+@pytest.mark.parametrize("case", ["equal", "spread"])
+def test_fast_mixed_rhs_either_side_of_one_workgroup_per_cu(exe, n, variant, bar, case):
+    """The fast kernel's partner loop follows the grid: with at most one 64-target
+    workgroup per CU (n <= 16384 on 256 CUs) it prefetches sets of partners, with more it
+    loads 8 partners per step and waits; both add the terms in increasing j and meet the
+    same bar against the oracle, with equal and with per-term masses."""
+    x, _ = ic(n, seed=23)
+    m = mass_case(n, case, 23)
+    got = gpu_rhs(exe, x, 1e-2, m, variant)
+    want = O.rhs(O.RHS_NBODY, x, 1e-2, m, threads=8)
+    assert np.array_equal(got[:3 * n], want[:3 * n])
+    scale = np.max(np.abs(want[3 * n:]))
+    err = np.max(np.abs(got[3 * n:] - want[3 * n:]))
+    assert err <= bar * scale, (variant, case, n, err / scale)
+
+
 @pytest.mark.parametrize("n", [2, 65, 1000])
 @pytest.mark.parametrize("eps", [1e-2, 0.0])
 def test_mixed_rhs_all_near_is_fp64(exe, n, eps):

@@ -170,7 +170,8 @@ __device__ __forceinline__ void exact_tile(const double* __restrict__ y,
     ax[t] = ay[t] = az[t] = 0.0;
   }
   // the shared-reciprocal form when every position coordinate of the system
-  // passes coord_shared_ok (one pass of the wave over its 3n coordinates)
+  // passes coord_shared_ok (one pass of the wave over its 3n coordinates; int
+  // indices: a system holds at most 2^24 bodies)
   bool shared_ok = kSharedRcp;
   if (kSharedRcp)
     shared_ok = __all(strided_all_shared_ok<kScanLoadsLean, int>(y, 3 * n, lane, 64));
@@ -549,7 +550,8 @@ __global__ __launch_bounds__(64 * C) void k_ens_rhs_exact_split(EnsBufs b,
   double xi[1], yi[1], zi[1];
   tile_targets<1>(y, n, tile.i0, ic, xi, yi, zi);
   // the shared-reciprocal form when every position coordinate of the system
-  // passes coord_shared_ok (the waves test strided shares, then vote)
+  // passes coord_shared_ok (the waves test strided shares, then vote; int
+  // indices: a system holds at most 2^24 bodies)
   bool shared_ok = kSharedRcp;
   if (kSharedRcp)
     shared_ok =

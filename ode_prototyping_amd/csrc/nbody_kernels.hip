@@ -510,7 +510,7 @@ __device__ __forceinline__ void fast_partners(uniform_ptr yu, uniform_ptr mass, 
                                               double& az) {
   constexpr int U = kFastSet;
   int j = jbeg;
-  if (!kPrefetch) {
+  if constexpr (!kPrefetch) {
     for (; j + 8 <= jend; j += 8) {
       const uniform_ptr pp = yu + 3 * j;
 #pragma unroll

@@ -437,8 +437,15 @@ void k_nbody_exact32(NBodyArgs a) {
   const int fc = wave == 0 ? slot : 2;
   const bool folds = (wave == 0) || (wave == 1 && slot == 0);
   const bool shared_ok = kSharedRcp && all_positions_shared_ok(y, n);
+  // masses that are all one power of two: unit-mass terms and one exact scale
+  // of each finished sum (uniform_pow2_mass), as in k_nbody_exact
+  const double m_uni = kMass && kSharedRcp && shared_ok ? uniform_pow2_mass(mass, n) : 0.0;
   double acc;
-  if (kSharedRcp && shared_ok)
+  if (kSharedRcp && shared_ok && kMass && m_uni != 0.0)
+    acc = e32_tiles<false, true, W>(terms, y, mass, n, a.eps2, xi, yi, zi, wave, slot, tgt, i0,
+                                    i, fc) *
+          m_uni;
+  else if (kSharedRcp && shared_ok)
     acc = e32_tiles<kMass, true, W>(terms, y, mass, n, a.eps2, xi, yi, zi, wave, slot, tgt, i0,
                                     i, fc);
   else

@@ -339,20 +339,24 @@ def test_exact_rhs_extreme_inputs_bit_identical(exe, eps, n):
     assert np.array_equal(got[finite].view(np.uint64), want[finite].view(np.uint64))
 
 
-@pytest.mark.parametrize("case", ["pow2", "pow2_first_only", "not_pow2", "pow2_out_of_range",
-                                  "pow2_extreme_inputs"])
-def test_exact_rhs_uniform_pow2_masses_bit_identical(exe, case):
+@pytest.mark.parametrize("case", ["pow2", "pow2_first_only", "pow2_last_differs", "not_pow2",
+                                  "pow2_out_of_range", "pow2_extreme_inputs"])
+@pytest.mark.parametrize("n", [12289, 4097])
+def test_exact_rhs_uniform_pow2_masses_bit_identical(exe, case, n):
     """Masses that are all one power of two 2^k (|k| <= 300) take the unit-mass terms and
     one exact scale of each sum: still bit-identical to the oracle's per-term products.
-    Arrays that only start with a power of two, a uniform mass that is not one, a power
-    of two outside the range, and a uniform power of two with coordinates outside the
-    shared-reciprocal range (IEEE path) take the per-term path, equally exact."""
-    n = 12289  # the 64-target kernel, with a partial last tile
+    Arrays that only start with a power of two, that end with another power of two, a
+    uniform mass that is not one, a power of two outside the range, and a uniform power
+    of two with coordinates outside the shared-reciprocal range (IEEE path) take the
+    per-term path, equally exact.  12289 bodies: the 64-target kernel, 4097: the 32-target
+    one; both with a partial last tile."""
     rng = np.random.default_rng(99)
     x, _ = ic(n, seed=5)
     m = np.full(n, 2.0 ** -14)
     if case == "pow2_first_only":
         m[1:] = rng.uniform(0.5, 1.5, n - 1) / n
+    elif case == "pow2_last_differs":
+        m[-1] = 2.0 ** -13
     elif case == "not_pow2":
         m[:] = 1.0 / 12289.0
     elif case == "pow2_out_of_range":

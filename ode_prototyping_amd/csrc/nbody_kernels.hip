@@ -288,6 +288,11 @@ __global__ __launch_bounds__(64 * kExactWaves) void k_nbody_exact(NBodyArgs a) {
 // 2 x 6W x 3 x 32 doubles (144 KB at W = 16: one workgroup per CU; 72 KB at
 // W = 8: two).
 constexpr int kE32JW = 3;  // partners per slot per tile
+// The launcher takes the instance with the power-of-two mass path from this
+// many bodies on.  Below, the second tile loop (and the register spill and
+// scratch set-up it brings) costs more per launch than the 3 products per
+// interaction it saves: 64-body solves ran 3-11 % slower with it.
+constexpr int kE32UniformMassMinN = 4096;
 template <int W>
 constexpr int e32_tj = 2 * kE32JW * W;  // partners per tile
 
@@ -415,7 +420,8 @@ __device__ __forceinline__ double e32_tiles(double (*terms)[e32_tj<W>][3][32],
 }
 
 // At W = 8 two workgroups share a CU: 4 waves per SIMD, <= 128 VGPRs each.
-template <bool kMass, bool kSharedRcp, int W>
+// kUniMass: the power-of-two mass path (with kMass and kSharedRcp).
+template <bool kMass, bool kSharedRcp, int W, bool kUniMass>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4)))
 void k_nbody_exact32(NBodyArgs a) {
   if (prologue_skip(a)) return;
@@ -439,9 +445,11 @@ void k_nbody_exact32(NBodyArgs a) {
   const bool shared_ok = kSharedRcp && all_positions_shared_ok(y, n);
   // masses that are all one power of two: unit-mass terms and one exact scale
   // of each finished sum (uniform_pow2_mass), as in k_nbody_exact
-  const double m_uni = kMass && kSharedRcp && shared_ok ? uniform_pow2_mass(mass, n) : 0.0;
+  double m_uni = 0.0;
+  if constexpr (kUniMass && kMass && kSharedRcp)
+    m_uni = shared_ok ? uniform_pow2_mass(mass, n) : 0.0;
   double acc;
-  if (kSharedRcp && shared_ok && kMass && m_uni != 0.0)
+  if (kUniMass && kMass && kSharedRcp && shared_ok && m_uni != 0.0)
     acc = e32_tiles<false, true, W>(terms, y, mass, n, a.eps2, xi, yi, zi, wave, slot, tgt, i0,
                                     i, fc) *
           m_uni;
@@ -834,16 +842,19 @@ hipError_t launch_nbody_rhs(const NBodyArgs& a, int variant, hipStream_t s) {
       if (0.62 * static_cast<double>(r32) < static_cast<double>(r64)) {
         const unsigned b32 = static_cast<unsigned>((a.n + 31) / 32);
         const dim3 th(64 * 16);
+        const bool uni = a.n >= kE32UniformMassMinN;
         if (mass) {
-          if (shared)
-            hipLaunchKernelGGL((k_nbody_exact32<true, true, 16>), dim3(b32), th, 0, s, a);
+          if (shared && uni)
+            hipLaunchKernelGGL((k_nbody_exact32<true, true, 16, true>), dim3(b32), th, 0, s, a);
+          else if (shared)
+            hipLaunchKernelGGL((k_nbody_exact32<true, true, 16, false>), dim3(b32), th, 0, s, a);
           else
-            hipLaunchKernelGGL((k_nbody_exact32<true, false, 16>), dim3(b32), th, 0, s, a);
+            hipLaunchKernelGGL((k_nbody_exact32<true, false, 16, false>), dim3(b32), th, 0, s, a);
         } else {
           if (shared)
-            hipLaunchKernelGGL((k_nbody_exact32<false, true, 16>), dim3(b32), th, 0, s, a);
+            hipLaunchKernelGGL((k_nbody_exact32<false, true, 16, false>), dim3(b32), th, 0, s, a);
           else
-            hipLaunchKernelGGL((k_nbody_exact32<false, false, 16>), dim3(b32), th, 0, s, a);
+            hipLaunchKernelGGL((k_nbody_exact32<false, false, 16, false>), dim3(b32), th, 0, s, a);
         }
         break;
       }

@@ -341,7 +341,7 @@ def test_exact_rhs_extreme_inputs_bit_identical(exe, eps, n):
 
 @pytest.mark.parametrize("case", ["pow2", "pow2_first_only", "pow2_last_differs", "not_pow2",
                                   "pow2_out_of_range", "pow2_extreme_inputs"])
-@pytest.mark.parametrize("n", [12289, 4097])
+@pytest.mark.parametrize("n", [12289, 4097, 4095])
 def test_exact_rhs_uniform_pow2_masses_bit_identical(exe, case, n):
     """Masses that are all one power of two 2^k (|k| <= 300) take the unit-mass terms and
     one exact scale of each sum: still bit-identical to the oracle's per-term products.
@@ -349,7 +349,8 @@ def test_exact_rhs_uniform_pow2_masses_bit_identical(exe, case, n):
     uniform mass that is not one, a power of two outside the range, and a uniform power
     of two with coordinates outside the shared-reciprocal range (IEEE path) take the
     per-term path, equally exact.  12289 bodies: the 64-target kernel, 4097: the 32-target
-    one; both with a partial last tile."""
+    one, both with a partial last tile; 4095: the 32-target kernel's instance without the
+    power-of-two path (below 4096 bodies), per-term products for every case."""
     rng = np.random.default_rng(99)
     x, _ = ic(n, seed=5)
     m = np.full(n, 2.0 ** -14)
